@@ -1,0 +1,14 @@
+"""ebgeometry_amd -- MI355X-native batched signed-distance queries.
+
+A from-scratch gfx950 implementation of the reference library's signed-distance query path
+(packed-BVH traversal over triangle meshes and CSG primitives, the point-triangle
+closest-feature distance and the pseudo-normal sign test), behind the reference's
+ImplicitFunction / SignedDistanceFunction / TriMeshSDF interface. Compute runs in
+libebgpu.so (hand-written HIP kernels + C ABI, include/ebgpu.h); this package is the host
+mirror of the reference API over that ABI.
+"""
+from .sdf import (Build, ImplicitFunction, SignedDistanceFunction, TriMeshSDF, morton_codes, morton_sort,
+                  pseudo_normals)
+
+__all__ = ["Build", "ImplicitFunction", "SignedDistanceFunction", "TriMeshSDF", "morton_codes", "morton_sort",
+           "pseudo_normals"]

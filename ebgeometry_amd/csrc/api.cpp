@@ -1,0 +1,496 @@
+// C-ABI layer: handle lifecycle, device memory, the query pipeline.
+//
+// Query pipeline for one batch (all on the caller's stream, no host synchronisation):
+//   1. Morton presort: query bbox -> 63-bit codes -> stable radix sort -> permutation
+//   2. [learned bound] MLP forward -> per-query seeded cull radius
+//   3. FP32 packet traversal over the permutation (writes d, face id by query index)
+//   4. [learned bound] FP32 traversal again, unseeded, for queries whose seeded radius held
+//      no triangle (exactness fallback)
+//   5. FP64 traversal for queries with |d| < recheck_eps * diag (sign is ill-conditioned in
+//      FP32 there), seeded with |d32| + slack
+#include "../../include/ebgpu.h"
+#include "kernels.hpp"
+#include "mesh_build.hpp"
+#include "mlp.hpp"
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstring>
+#include <mutex>
+#include <string>
+#include <vector>
+
+namespace {
+
+thread_local std::string g_last_error;
+
+ebg_status fail(ebg_status s, const std::string& msg)
+{
+  g_last_error = msg;
+  return s;
+}
+
+#define EBG_HIP(call)                                                                                        \
+  do {                                                                                                       \
+    const hipError_t e_ = (call);                                                                            \
+    if (e_ != hipSuccess)                                                                                    \
+      return fail(e_ == hipErrorOutOfMemory ? EBG_OOM : EBG_HIP_ERROR,                                      \
+                  std::string(#call) + ": " + hipGetErrorString(e_));                                        \
+  } while (0)
+
+struct DevBuf
+{
+  void*       p     = nullptr;
+  std::size_t bytes = 0;
+  ~DevBuf() { release(); }
+  void release()
+  {
+    if (p) (void)hipFree(p);
+    p     = nullptr;
+    bytes = 0;
+  }
+  hipError_t reserve(std::size_t b)
+  {
+    if (b <= bytes) return hipSuccess;
+    release();
+    hipError_t e = hipMalloc(&p, b);
+    if (e == hipSuccess) bytes = b;
+    else p = nullptr;
+    return e;
+  }
+  template <class T>
+  T* as() const
+  {
+    return static_cast<T*>(p);
+  }
+};
+
+template <class T>
+hipError_t upload(DevBuf& b, const std::vector<T>& v)
+{
+  const std::size_t bytes = std::max<std::size_t>(64, v.size() * sizeof(T));
+  hipError_t        e     = b.reserve(bytes);
+  if (e != hipSuccess) return e;
+  if (!v.empty()) e = hipMemcpy(b.p, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice);
+  return e;
+}
+
+// Query workspace, grown on demand.
+struct Workspace
+{
+  DevBuf codes, codes_sorted, perm, sort_scratch, box, lists, seeds;
+  uint64_t cap = 0;
+
+  hipError_t ensure(uint64_t n)
+  {
+    if (n <= cap && cap) return hipSuccess;
+    const uint64_t m = std::max<uint64_t>(n, 1024);
+    hipError_t     e;
+    if ((e = codes.reserve(8 * m)) != hipSuccess) return e;
+    if ((e = codes_sorted.reserve(8 * m)) != hipSuccess) return e;
+    if ((e = perm.reserve(4 * m)) != hipSuccess) return e;
+    if ((e = sort_scratch.reserve(ebgpu::radix_sort_scratch_bytes(m))) != hipSuccess) return e;
+    if ((e = box.reserve(sizeof(double) * 6 * (ebgpu::kBboxParts + 1))) != hipSuccess) return e;
+    // recheck list, fallback list, 2 counters (+pad)
+    if ((e = lists.reserve(8 * m + 256)) != hipSuccess) return e;
+    if ((e = seeds.reserve(4 * m)) != hipSuccess) return e;
+    cap = m;
+    return hipSuccess;
+  }
+  uint32_t* recheck_list() const { return lists.as<uint32_t>(); }
+  uint32_t* fallback_list() const { return lists.as<uint32_t>() + cap; }
+  uint32_t* counters() const { return lists.as<uint32_t>() + 2 * cap; }
+};
+
+} // namespace
+
+struct ebg_mesh
+{
+  int         K = 4, leaf = 4, banks = 1, stack_need = 0;
+  std::size_t nv = 0, nt = 0, num_nodes = 0, num_leaves = 0;
+  double      lo[3]{}, hi[3]{}, diag = 0;
+  double      build_seconds = 0;
+  DevBuf      nodes, tris, tripos, feat, vn32, en32, vn64, en64, fn64;
+  ebgpu::MeshDev dev{};
+  std::size_t device_bytes = 0;
+  std::mutex  mu;
+  Workspace   ws;
+  hipEvent_t  ev0 = nullptr, ev1 = nullptr;
+  bool        timed = false;
+  ebgpu::Mlp  mlp;
+
+  ~ebg_mesh()
+  {
+    if (ev0) (void)hipEventDestroy(ev0);
+    if (ev1) (void)hipEventDestroy(ev1);
+  }
+};
+
+namespace ebgpu {
+// Exposed to the other translation units of the library (MLP training, batch, scene).
+const MeshDev& mesh_dev(const ebg_mesh* m) { return m->dev; }
+int            mesh_K(const ebg_mesh* m) { return m->K; }
+int            mesh_banks(const ebg_mesh* m) { return m->banks; }
+double         mesh_diag(const ebg_mesh* m) { return m->diag; }
+const double*  mesh_lo(const ebg_mesh* m) { return m->lo; }
+const double*  mesh_hi(const ebg_mesh* m) { return m->hi; }
+Mlp&           mesh_mlp(ebg_mesh* m) { return m->mlp; }
+const Mlp&     mesh_mlp(const ebg_mesh* m) { return m->mlp; }
+void           set_last_error(const std::string& s) { g_last_error = s; }
+} // namespace ebgpu
+
+extern "C" {
+
+const char* ebg_last_error(void) { return g_last_error.c_str(); }
+
+void ebg_query_opts_default(ebg_query_opts* o)
+{
+  if (!o) return;
+  std::memset(o, 0, sizeof(*o));
+  o->presort      = 1;
+  o->fp64_recheck = 1;
+  o->recheck_eps  = 1e-4;
+  o->bound_scale  = 1.25f;
+  o->bound_offset = 1e-3f;
+}
+
+static ebg_status upload_mesh(ebg_mesh* m, ebgpu::PackedMesh& pm)
+{
+  EBG_HIP(upload(m->nodes, pm.nodes));
+  EBG_HIP(upload(m->tris, pm.tris));
+  EBG_HIP(upload(m->tripos, pm.tripos));
+  EBG_HIP(upload(m->feat, pm.feat));
+  EBG_HIP(upload(m->vn32, pm.vn32));
+  EBG_HIP(upload(m->en32, pm.en32));
+  EBG_HIP(upload(m->vn64, pm.vn64));
+  EBG_HIP(upload(m->en64, pm.en64));
+  EBG_HIP(upload(m->fn64, pm.fn64));
+  m->K          = pm.K;
+  m->leaf       = pm.leaf_size;
+  m->num_nodes  = pm.num_nodes;
+  m->num_leaves = pm.num_leaves;
+  m->stack_need = pm.stack_need;
+  m->banks      = (pm.stack_need + 63) / 64;
+  if (m->banks > 4)
+    return fail(EBG_STACK_OVERFLOW, "tree needs a traversal stack of " + std::to_string(pm.stack_need) +
+                                      " entries (max 256); use a smaller branching factor");
+  if (m->banks == 3) m->banks = 4;
+  for (int c = 0; c < 3; c++) {
+    m->lo[c] = pm.lo[c];
+    m->hi[c] = pm.hi[c];
+  }
+  m->diag = pm.diag;
+  ebgpu::MeshDev& d = m->dev;
+  d.nodes           = m->nodes.p;
+  d.tris            = m->tris.as<ebgpu::TriRec>();
+  d.tripos          = m->tripos.as<float>();
+  d.feat            = m->feat.as<uint32_t>();
+  d.vn32            = m->vn32.as<float>();
+  d.en32            = m->en32.as<float>();
+  d.vn64            = m->vn64.as<double>();
+  d.en64            = m->en64.as<double>();
+  d.fn64            = m->fn64.as<double>();
+  for (int c = 0; c < 3; c++) {
+    d.root_lo[c] = pm.lo[c];
+    d.root_hi[c] = pm.hi[c];
+  }
+  d.diag          = float(pm.diag);
+  d.num_tris      = uint32_t(m->nt);
+  m->device_bytes = m->nodes.bytes + m->tris.bytes + m->tripos.bytes + m->feat.bytes + m->vn32.bytes +
+                    m->en32.bytes + m->vn64.bytes + m->en64.bytes + m->fn64.bytes;
+  return EBG_OK;
+}
+
+ebg_status ebg_mesh_create(const float* xyz, uint64_t nv, const uint32_t* tri, uint64_t nt,
+                           const ebg_build_opts* opts, ebg_mesh** out)
+{
+  if (!out) return fail(EBG_INVALID_ARG, "out is null");
+  *out = nullptr;
+  if (!xyz || !tri || nv < 3 || nt < 1) return fail(EBG_INVALID_ARG, "empty mesh");
+  if (nt > 0x7FFFFFFFull / 3 || nv > 0xFFFFFFFFull) return fail(EBG_INVALID_ARG, "mesh too large");
+  for (uint64_t i = 0; i < 3 * nv; i++)
+    if (!std::isfinite(xyz[i])) return fail(EBG_INVALID_ARG, "non-finite vertex coordinate");
+  for (uint64_t i = 0; i < 3 * nt; i++)
+    if (tri[i] >= nv) return fail(EBG_INVALID_ARG, "triangle index out of range");
+  ebg_build_opts o{};
+  if (opts) o = *opts;
+  const int K    = o.branching ? o.branching : 4;
+  const int leaf = o.leaf_size ? o.leaf_size : 4;
+  if (K != 4 && K != 8) return fail(EBG_INVALID_ARG, "branching must be 4 or 8");
+  if (leaf < 1 || leaf > ebgpu::kMaxLeaf) return fail(EBG_INVALID_ARG, "leaf_size must be 1..16");
+
+  const auto t0 = std::chrono::steady_clock::now();
+  ebgpu::PseudoNormals pn;
+  std::string          err;
+  if (!ebgpu::compute_pseudo_normals(xyz, nv, tri, nt, pn, err)) return fail(EBG_INVALID_ARG, err);
+  ebgpu::PackedMesh pm;
+  if (o.builder == EBG_BUILD_MORTON) {
+    return fail(EBG_UNSUPPORTED, "Morton builder not available in this build");
+  }
+  if (!ebgpu::build_sah_mesh(xyz, nv, tri, nt, K, leaf, pn, pm, err)) return fail(EBG_INVALID_ARG, err);
+
+  auto* m = new ebg_mesh;
+  m->nv   = nv;
+  m->nt   = nt;
+  const ebg_status st = upload_mesh(m, pm);
+  if (st != EBG_OK) {
+    delete m;
+    return st;
+  }
+  m->build_seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+  *out             = m;
+  return EBG_OK;
+}
+
+ebg_status ebg_mesh_destroy(ebg_mesh* mesh)
+{
+  delete mesh;
+  return EBG_OK;
+}
+
+ebg_status ebg_pseudo_normals_host(const float* xyz, uint64_t nv, const uint32_t* tri, uint64_t nt, double* fn,
+                                   double* vn, double* en)
+{
+  if (!xyz || !tri || !fn || !vn || !en || nv < 3 || nt < 1) return fail(EBG_INVALID_ARG, "bad arguments");
+  for (uint64_t i = 0; i < 3 * nt; i++)
+    if (tri[i] >= nv) return fail(EBG_INVALID_ARG, "triangle index out of range");
+  ebgpu::PseudoNormals pn;
+  std::string          err;
+  if (!ebgpu::compute_pseudo_normals(xyz, nv, tri, nt, pn, err)) return fail(EBG_INVALID_ARG, err);
+  for (uint64_t f = 0; f < nt; f++) {
+    for (int c = 0; c < 3; c++) fn[3 * f + c] = pn.face[3 * f + c];
+    for (int v = 0; v < 3; v++)
+      for (int c = 0; c < 3; c++) {
+        vn[9 * f + 3 * v + c] = pn.vert[3 * tri[3 * f + v] + c];
+        en[9 * f + 3 * v + c] = pn.edge[9 * f + 3 * v + c];
+      }
+  }
+  return EBG_OK;
+}
+
+ebg_status ebg_mesh_get_info(const ebg_mesh* m, ebg_mesh_info* info)
+{
+  if (!m || !info) return fail(EBG_INVALID_ARG, "null argument");
+  std::memset(info, 0, sizeof(*info));
+  for (int c = 0; c < 3; c++) {
+    info->bbox_lo[c] = m->lo[c];
+    info->bbox_hi[c] = m->hi[c];
+  }
+  info->diag           = m->diag;
+  info->num_vertices   = m->nv;
+  info->num_triangles  = m->nt;
+  info->num_nodes      = m->num_nodes;
+  info->num_leaves     = m->num_leaves;
+  info->branching      = m->K;
+  info->leaf_size      = m->leaf;
+  info->stack_need     = m->stack_need;
+  info->stack_capacity = 64 * m->banks;
+  info->device_bytes   = m->device_bytes;
+  info->build_seconds  = m->build_seconds;
+  return EBG_OK;
+}
+
+ebg_status ebg_query_sdf(const ebg_mesh* cm, const float* d_pts, uint64_t n, float* d_dist, uint32_t* d_tri,
+                         const ebg_query_opts* opts)
+{
+  if (!cm) return fail(EBG_INVALID_ARG, "mesh is null");
+  if (n == 0) return EBG_OK;
+  if (!d_pts || !d_dist) return fail(EBG_INVALID_ARG, "null point or output array");
+  if (n > 0xFFFFFFFFull) return fail(EBG_INVALID_ARG, "at most 2^32-1 queries per call");
+  ebg_mesh*      m = const_cast<ebg_mesh*>(cm);
+  ebg_query_opts o;
+  ebg_query_opts_default(&o);
+  if (opts) o = *opts;
+  if (o.learned_bound && !m->mlp.trained) return fail(EBG_NOT_READY, "learned bound requested but no network trained");
+  hipStream_t s = static_cast<hipStream_t>(o.stream);
+
+  std::lock_guard<std::mutex> lock(m->mu);
+  Workspace&                  ws = m->ws;
+  EBG_HIP(ws.ensure(n));
+  if (!m->ev0) {
+    EBG_HIP(hipEventCreate(&m->ev0));
+    EBG_HIP(hipEventCreate(&m->ev1));
+  }
+  uint32_t* cnt = ws.counters(); // [0] recheck, [1] fallback
+  EBG_HIP(ebgpu::launch_fill_u32(cnt, 2, 0u, s));
+
+  const uint32_t* idx = nullptr;
+  if (o.presort && n > 1) {
+    EBG_HIP(ebgpu::launch_bbox_f32(d_pts, n, ws.box.as<float>(), ws.box.as<float>() + 8, s));
+    EBG_HIP(ebgpu::launch_codes_f32(d_pts, n, ws.box.as<float>(), ws.codes.as<uint64_t>(), s));
+    EBG_HIP(ebgpu::radix_sort_pairs(ws.codes.as<uint64_t>(), nullptr, ws.codes_sorted.as<uint64_t>(),
+                                    ws.perm.as<uint32_t>(), n, 63, ws.sort_scratch.p, s));
+    idx = ws.perm.as<uint32_t>();
+  }
+
+  const float* seeds = nullptr;
+  if (o.learned_bound) {
+    EBG_HIP(ebgpu::mlp_seed_radius(m->mlp, m->dev, d_pts, idx, n, o.bound_scale, float(o.bound_offset * m->diag),
+                                   ws.seeds.as<float>(), s));
+    seeds = ws.seeds.as<float>();
+  }
+
+  const bool recheck = o.fp64_recheck || o.fp64_all;
+  ebgpu::QueryArgs a{};
+  a.pts            = d_pts;
+  a.idx            = idx;
+  a.n              = n;
+  a.seed_r         = seeds;
+  a.out_d          = d_dist;
+  a.out_tri        = d_tri;
+  a.recheck_abs    = o.fp64_all ? INFINITY : (recheck ? float(o.recheck_eps * m->diag) : 0.0f);
+  a.recheck_list   = recheck ? ws.recheck_list() : nullptr;
+  a.recheck_count  = cnt;
+  a.fallback_list  = seeds ? ws.fallback_list() : nullptr;
+  a.fallback_count = cnt + 1;
+  a.work           = reinterpret_cast<unsigned long long*>(o.work);
+  const bool count = o.work != nullptr;
+
+  EBG_HIP(hipEventRecord(m->ev0, s));
+  EBG_HIP(ebgpu::launch_sdf32(m->dev, m->K, m->banks, a, count, s));
+  if (seeds) {
+    ebgpu::QueryArgs f = a;
+    f.idx              = ws.fallback_list();
+    f.count_ptr        = cnt + 1;
+    f.seed_r           = nullptr;
+    f.fallback_list    = nullptr;
+    EBG_HIP(ebgpu::launch_sdf32(m->dev, m->K, m->banks, f, count, s));
+  }
+  EBG_HIP(hipEventRecord(m->ev1, s));
+  m->timed = true;
+
+  if (recheck) {
+    ebgpu::QueryArgs r = a;
+    r.idx              = ws.recheck_list();
+    r.count_ptr        = cnt;
+    r.seed_r           = nullptr;
+    r.recheck_list     = nullptr;
+    r.fallback_list    = nullptr;
+    r.work             = nullptr;
+    // bound (|d32| (1 + rel) + abs)^2 always contains the FP64 minimum: FP32 distances are
+    // accurate to a few ulp of the coordinates
+    r.slack_abs = float(1e-5 * m->diag);
+    r.slack_rel = 1e-4f;
+    EBG_HIP(ebgpu::launch_sdf64(m->dev, m->K, m->banks, r, s));
+  }
+  if (count) EBG_HIP(ebgpu::launch_add_counts(a.work, recheck ? cnt : nullptr, seeds ? cnt + 1 : nullptr, s));
+  return EBG_OK;
+}
+
+double ebg_mesh_last_traversal_ms(const ebg_mesh* cm)
+{
+  ebg_mesh* m = const_cast<ebg_mesh*>(cm);
+  if (!m || !m->timed) return -1.0;
+  float ms = -1.0f;
+  if (hipEventSynchronize(m->ev1) != hipSuccess) return -1.0;
+  if (hipEventElapsedTime(&ms, m->ev0, m->ev1) != hipSuccess) return -1.0;
+  return ms;
+}
+
+ebg_status ebg_query_sdf_host(const ebg_mesh* m, const float* pts, uint64_t n, float* dist, uint32_t* tri,
+                              const ebg_query_opts* opts)
+{
+  if (!m) return fail(EBG_INVALID_ARG, "mesh is null");
+  if (n == 0) return EBG_OK;
+  if (!pts || !dist) return fail(EBG_INVALID_ARG, "null point or output array");
+  DevBuf p, d, t;
+  EBG_HIP(p.reserve(12 * n));
+  EBG_HIP(d.reserve(4 * n));
+  if (tri) EBG_HIP(t.reserve(4 * n));
+  ebg_query_opts o;
+  ebg_query_opts_default(&o);
+  if (opts) o = *opts;
+  hipStream_t s = static_cast<hipStream_t>(o.stream);
+  EBG_HIP(hipMemcpyAsync(p.p, pts, 12 * n, hipMemcpyHostToDevice, s));
+  const ebg_status st = ebg_query_sdf(m, p.as<float>(), n, d.as<float>(), tri ? t.as<uint32_t>() : nullptr, &o);
+  if (st != EBG_OK) return st;
+  EBG_HIP(hipMemcpyAsync(dist, d.p, 4 * n, hipMemcpyDeviceToHost, s));
+  if (tri) EBG_HIP(hipMemcpyAsync(tri, t.p, 4 * n, hipMemcpyDeviceToHost, s));
+  EBG_HIP(hipStreamSynchronize(s));
+  return EBG_OK;
+}
+
+// ------------------------------------------------------------------------------------------
+// SFC
+// ------------------------------------------------------------------------------------------
+
+ebg_status ebg_morton_codes_f32(const float* d_pts, uint64_t n, uint64_t* d_codes, void* stream)
+{
+  if (n == 0) return EBG_OK;
+  if (!d_pts || !d_codes) return fail(EBG_INVALID_ARG, "null pointer");
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  DevBuf      box;
+  EBG_HIP(box.reserve(sizeof(float) * 6 * (ebgpu::kBboxParts + 2)));
+  EBG_HIP(ebgpu::launch_bbox_f32(d_pts, n, box.as<float>(), box.as<float>() + 8, s));
+  EBG_HIP(ebgpu::launch_codes_f32(d_pts, n, box.as<float>(), d_codes, s));
+  EBG_HIP(hipStreamSynchronize(s));
+  return EBG_OK;
+}
+
+ebg_status ebg_morton_codes_f64(const double* d_pts, uint64_t n, uint64_t* d_codes, void* stream)
+{
+  if (n == 0) return EBG_OK;
+  if (!d_pts || !d_codes) return fail(EBG_INVALID_ARG, "null pointer");
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  DevBuf      box;
+  EBG_HIP(box.reserve(sizeof(double) * 6 * (ebgpu::kBboxParts + 2)));
+  EBG_HIP(ebgpu::launch_bbox_f64(d_pts, n, box.as<double>(), box.as<double>() + 8, s));
+  EBG_HIP(ebgpu::launch_codes_f64(d_pts, n, box.as<double>(), d_codes, s));
+  EBG_HIP(hipStreamSynchronize(s));
+  return EBG_OK;
+}
+
+ebg_status ebg_morton_sort(const uint64_t* d_codes, uint64_t n, uint32_t* d_perm, uint64_t* d_sorted, void* stream)
+{
+  if (n == 0) return EBG_OK;
+  if (!d_codes || !d_perm) return fail(EBG_INVALID_ARG, "null pointer");
+  if (n > 0xFFFFFFFFull) return fail(EBG_INVALID_ARG, "at most 2^32-1 keys");
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  DevBuf      scratch, kout;
+  EBG_HIP(scratch.reserve(ebgpu::radix_sort_scratch_bytes(n)));
+  uint64_t* ko = d_sorted;
+  if (!ko) {
+    EBG_HIP(kout.reserve(8 * n));
+    ko = kout.as<uint64_t>();
+  }
+  EBG_HIP(ebgpu::radix_sort_pairs(d_codes, nullptr, ko, d_perm, n, 63, scratch.p, s));
+  EBG_HIP(hipStreamSynchronize(s));
+  return EBG_OK;
+}
+
+} // extern "C"
+
+// ------------------------------------------------------------------------------------------
+// learned bound (training lives in mlp_train.hip)
+// ------------------------------------------------------------------------------------------
+extern "C" {
+
+void ebg_mlp_opts_default(ebg_mlp_opts* o)
+{
+  if (!o) return;
+  std::memset(o, 0, sizeof(*o));
+  o->num_samples = 4000000;
+  o->epochs      = 8;
+  o->batch       = 4096;
+  o->lr          = 2e-3f;
+  o->seed        = 12345;
+  o->near_sigma  = 0.02f;
+  o->box_scale   = 1.2f;
+}
+
+uint64_t ebg_mlp_param_count(void) { return uint64_t(ebgpu::kMlpParams); }
+
+ebg_status ebg_mlp_train(ebg_mesh*, const ebg_mlp_opts*, float*) { return fail(EBG_UNSUPPORTED, "not yet"); }
+
+ebg_status ebg_mlp_predict(const ebg_mesh*, const float*, uint64_t, float*, void*)
+{
+  return fail(EBG_UNSUPPORTED, "not yet");
+}
+
+ebg_status ebg_mlp_get_params(const ebg_mesh*, float*) { return fail(EBG_UNSUPPORTED, "not yet"); }
+ebg_status ebg_mlp_set_params(ebg_mesh*, const float*) { return fail(EBG_UNSUPPORTED, "not yet"); }
+
+} // extern "C"

@@ -1,0 +1,80 @@
+// Internal layouts shared by the host builder, the C-ABI layer and the HIP kernels.
+//
+// HBM layout of one mesh (all arrays 64-byte aligned, in leaf order where per-triangle):
+//   nodes   Node<K>[num_nodes]   K child boxes as SoA (lo.x[K] .. hi.z[K]) + child refs.
+//                                K=4: 128 B, K=8: 256 B. Read with wave-uniform (scalar)
+//                                loads: one fetch serves all 64 queries of a wavefront.
+//   tris    TriRec[num_tris]     128 B FP32 record per triangle with the quantities the
+//                                point-triangle kernel needs precomputed (edges, face normal,
+//                                edge x normal, 1/|edge|^2). Also read with scalar loads.
+//   tripos  float[12*num_tris]   raw vertex positions (v0,v1,v2,pad) for the FP64 path.
+//   feat    uint32[8*num_tris]   vertex ids, half-edge ids, original face id: lets the sign
+//                                step fetch the one pseudo-normal a query needs.
+//   vn32/vn64, en32/en64         vertex / half-edge pseudo-normals (float4 / double4).
+//   fn64    double4[num_tris]    FP64 face normals in leaf order (FP64 recheck path).
+#pragma once
+
+#include <cstddef>
+#include <cstdint>
+
+namespace ebgpu {
+
+constexpr uint32_t kInvalid = 0xFFFFFFFFu;
+constexpr int      kMaxLeaf = 16;
+
+// Feature codes of the closest point on a triangle.
+enum Feature : uint32_t
+{
+  kFace = 0,
+  kVert0 = 1,
+  kVert1 = 2,
+  kVert2 = 3,
+  kEdge0 = 4, // v0 -> v1
+  kEdge1 = 5, // v1 -> v2
+  kEdge2 = 6  // v2 -> v0
+};
+
+template <int K>
+struct alignas(64) Node
+{
+  float    lox[K], loy[K], loz[K], hix[K], hiy[K], hiz[K];
+  uint32_t ref[K]; // cnt==0: child node index; cnt in 1..16: first triangle of the leaf
+  uint32_t cnt[K]; // 0: interior child; 1..16: leaf triangle count; kInvalid: empty slot
+};
+static_assert(sizeof(Node<4>) == 128, "Node<4> layout");
+static_assert(sizeof(Node<8>) == 256, "Node<8> layout");
+
+struct alignas(64) TriRec
+{
+  float    v0[3];  // first vertex
+  float    e0[3];  // v1 - v0   (the reference's v21)
+  float    e1[3];  // v2 - v1   (v32)
+  float    e2[3];  // v0 - v2   (v13)
+  float    n[3];   // unit face normal (x2-x0)x(x2-x1)/|.|
+  float    c0[3];  // e0 x n: sign of c0.(p-v0) tests the face region
+  float    c1[3];  // e1 x n
+  float    c2[3];  // e2 x n
+  float    inv[3]; // 1/|e_i|^2
+  uint32_t face;   // original face index
+  float    pad[4];
+};
+static_assert(sizeof(TriRec) == 128, "TriRec layout");
+
+// Device view of a mesh, passed by value to kernels.
+struct MeshDev
+{
+  const void*     nodes;
+  const TriRec*   tris;
+  const float*    tripos; // 12 per triangle
+  const uint32_t* feat;   // 8 per triangle: vid0..2, he0..2, face, 0
+  const float*    vn32;   // float4 per vertex
+  const float*    en32;   // float4 per half-edge (3*face + i)
+  const double*   vn64;   // double4 per vertex
+  const double*   en64;   // double4 per half-edge
+  const double*   fn64;   // double4 per triangle (leaf order): FP64 face normal
+  float           root_lo[3], root_hi[3];
+  float           diag;
+  uint32_t        num_tris;
+};
+
+} // namespace ebgpu

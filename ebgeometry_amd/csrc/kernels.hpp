@@ -1,0 +1,54 @@
+// Host-side launchers of the HIP kernels (defined in the .hip translation units).
+#pragma once
+
+#include "ebgpu_internal.hpp"
+
+#include <hip/hip_runtime.h>
+
+namespace ebgpu {
+
+// Arguments of one traversal launch over a list of queries.
+struct QueryArgs
+{
+  const float*        pts;       // original point array (3 floats per query)
+  const uint32_t*     idx;       // query index per thread (presort order or a compacted list); null = identity
+  const uint32_t*     count_ptr; // device count of idx entries (list launches); null = use n
+  uint64_t            n;         // threads to launch (upper bound when count_ptr is set)
+  const float*        seed_r;    // per-query seeded cull radius (indexed by query); null = +inf
+  float*              out_d;     // indexed by query
+  uint32_t*           out_tri;   // indexed by query; may be null
+  float               recheck_abs; // |d| below this goes to the recheck list (0 = off)
+  uint32_t*           recheck_list;
+  uint32_t*           recheck_count;
+  uint32_t*           fallback_list;  // queries whose seeded radius held no triangle
+  uint32_t*           fallback_count;
+  unsigned long long* work;      // [inner nodes, triangles] accumulated (null = not counted)
+  float               slack_abs; // FP64 recheck: bound = (|d| + slack_abs + slack_rel*|d|)^2
+  float               slack_rel;
+};
+
+// FP32 packet traversal. K = 4 or 8; banks = stack capacity / 64 (1..4).
+hipError_t launch_sdf32(const MeshDev& m, int K, int banks, const QueryArgs& a, bool count, hipStream_t s);
+// FP64 traversal that recomputes distance, closest face and sign exactly as the reference's
+// double-precision lane kernel; seeded from the FP32 result (bound |d| + slack).
+hipError_t launch_sdf64(const MeshDev& m, int K, int banks, const QueryArgs& a, hipStream_t s);
+
+// util.hip
+hipError_t launch_fill_u32(uint32_t* p, uint64_t n, uint32_t v, hipStream_t s);
+hipError_t launch_add_counts(unsigned long long* work, const uint32_t* recheck, const uint32_t* fallback, hipStream_t s);
+
+// Morton / sort (morton.hip)
+// Bounding box of n points into d_box6 (lo xyz, hi xyz); part: kBboxParts*6 values of scratch.
+constexpr int kBboxParts = 1024;
+hipError_t launch_bbox_f32(const float* pts, uint64_t n, float* d_box6, void* part, hipStream_t s);
+hipError_t launch_bbox_f64(const double* pts, uint64_t n, double* d_box6, void* part, hipStream_t s);
+hipError_t launch_codes_f32(const float* pts, uint64_t n, const float* d_box6, uint64_t* codes, hipStream_t s);
+hipError_t launch_codes_f64(const double* pts, uint64_t n, const double* d_box6, uint64_t* codes, hipStream_t s);
+// Bytes of scratch radix_sort needs for n keys.
+std::size_t radix_sort_scratch_bytes(uint64_t n);
+// Stable LSD sort of (keys, values); values_in may be null (identity). Results land in
+// keys_out / values_out. 'bits' low bits of the keys are significant (<= 64).
+hipError_t radix_sort_pairs(const uint64_t* keys_in, const uint32_t* values_in, uint64_t* keys_out,
+                            uint32_t* values_out, uint64_t n, int bits, void* scratch, hipStream_t s);
+
+} // namespace ebgpu

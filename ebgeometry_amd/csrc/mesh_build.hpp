@@ -1,0 +1,56 @@
+// Host-side mesh preparation: pseudo-normals and the wide BVH, packed for upload.
+#pragma once
+
+#include "ebgpu_internal.hpp"
+
+#include <string>
+#include <vector>
+
+namespace ebgpu {
+
+struct PseudoNormals
+{
+  std::vector<double> face; // 3 per face
+  std::vector<double> vert; // 3 per vertex
+  std::vector<double> edge; // 3 per half-edge (3*face + i, edge v_i -> v_{i+1})
+};
+
+// Angle-weighted pseudo-normals with the reference's formulas and summation order:
+// face (x2-x0)x(x2-x1)/|.| (Source/EBGeometry_DCEL_FaceImplem.hpp:138-165), edge =
+// normalise(n_face + n_pairface) (DCEL_EdgeImplem.hpp:130-165), vertex = normalise(sum over
+// incident faces in facet order of acos(clamp(e1.e2)) n_face) (DCEL_VertexImplem.hpp:190-300).
+// Returns false (with a message) for degenerate input.
+bool compute_pseudo_normals(const float* xyz, std::size_t nv, const uint32_t* tri, std::size_t nt,
+                            PseudoNormals& out, std::string& err);
+
+// Packed, upload-ready mesh.
+struct PackedMesh
+{
+  int                   K = 4, leaf_size = 4;
+  std::size_t           num_nodes = 0, num_leaves = 0;
+  int                   stack_need = 0;
+  float                 lo[3], hi[3];
+  double                diag = 0;
+  std::vector<uint8_t>  nodes;  // Node<K>[num_nodes]
+  std::vector<TriRec>   tris;   // leaf order
+  std::vector<float>    tripos; // 12 per triangle, leaf order
+  std::vector<uint32_t> feat;   // 8 per triangle, leaf order
+  std::vector<float>    vn32, en32;
+  std::vector<double>   vn64, en64, fn64;
+};
+
+// Binned-SAH binary build (32 bins, 3 axes, cost A_L N_L + A_R N_R as in the reference's
+// SAH2WaySplit, Source/EBGeometry_BVH.hpp:521-641) with leaves of at most leaf_size
+// triangles, collapsed into a K-wide tree (largest-area child expanded first).
+bool build_sah_mesh(const float* xyz, std::size_t nv, const uint32_t* tri, std::size_t nt, int K,
+                    int leaf_size, const PseudoNormals& pn, PackedMesh& out, std::string& err);
+
+// Fill the per-triangle arrays (TriRec, tripos, feat) for triangles given in leaf order
+// (order[i] = original face index) and the normal arrays. Shared by both builders.
+void pack_triangles(const float* xyz, const uint32_t* tri, const std::vector<uint32_t>& order,
+                    const PseudoNormals& pn, std::size_t nv, PackedMesh& out);
+
+// Maximum traversal stack depth any query can need in a packed K-wide tree.
+int compute_stack_need(const PackedMesh& m);
+
+} // namespace ebgpu

@@ -1,0 +1,338 @@
+// Morton codes and a stable LSD radix sort on gfx950.
+//
+// Codes reproduce SFC::computeBins<T> + SFC::Morton::encode bit for bit
+// (Source/EBGeometry_SFCImplem.hpp:31-58, 228-276):
+//   delta_c = (1/ValidSpan) * (max_c - min_c)  (Vec3T/T multiplies by the rounded reciprocal,
+//             Source/EBGeometry_VecImplem.hpp:303-309), delta_c == 0 -> 1,
+//   bin_c   = clamp(floor((p_c - min_c) / delta_c), 0, ValidSpan)  (correctly rounded divide),
+//   code    = spread(x) | spread(y) << 1 | spread(z) << 2, 21 bits per axis.
+// The sort replaces the reference's std::sort on codes (:279-300) with a stable LSD radix sort,
+// so equal codes keep increasing index order.
+#include "kernels.hpp"
+
+#include <hip/hip_runtime.h>
+
+namespace ebgpu {
+
+namespace {
+
+constexpr uint32_t kValidSpan = 2097151u; // 2^21 - 1
+
+__device__ __forceinline__ uint64_t spread3(uint64_t a)
+{
+  uint64_t b = a & 0x1fffffull;
+  b          = (b | b << 32) & 0x1f00000000ffffull;
+  b          = (b | b << 16) & 0x1f0000ff0000ffull;
+  b          = (b | b << 8) & 0x100f00f00f00f00full;
+  b          = (b | b << 4) & 0x10c30c30c30c30c3ull;
+  b          = (b | b << 2) & 0x1249249249249249ull;
+  return b;
+}
+
+// ------------------------------- bounding box -------------------------------------------
+
+template <class T>
+__device__ __forceinline__ T tmin(T a, T b) { return a < b ? a : b; }
+template <class T>
+__device__ __forceinline__ T tmax(T a, T b) { return a > b ? a : b; }
+
+template <class T>
+__global__ __launch_bounds__(256) void bbox_partial(const T* __restrict__ p, uint64_t n, T* __restrict__ part)
+{
+  __shared__ T sh[6][256];
+  T            lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
+  for (uint64_t i = (uint64_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (uint64_t)gridDim.x * 256)
+    for (int c = 0; c < 3; c++) {
+      const T v = p[3 * i + c];
+      lo[c]     = tmin(lo[c], v);
+      hi[c]     = tmax(hi[c], v);
+    }
+  for (int c = 0; c < 3; c++) {
+    sh[c][threadIdx.x]     = lo[c];
+    sh[3 + c][threadIdx.x] = hi[c];
+  }
+  __syncthreads();
+  for (int s = 128; s > 0; s >>= 1) {
+    if (threadIdx.x < (unsigned)s)
+      for (int c = 0; c < 3; c++) {
+        sh[c][threadIdx.x]     = tmin(sh[c][threadIdx.x], sh[c][threadIdx.x + s]);
+        sh[3 + c][threadIdx.x] = tmax(sh[3 + c][threadIdx.x], sh[3 + c][threadIdx.x + s]);
+      }
+    __syncthreads();
+  }
+  if (threadIdx.x < 6) part[6 * blockIdx.x + threadIdx.x] = sh[threadIdx.x][0];
+}
+
+template <class T>
+__global__ __launch_bounds__(256) void bbox_final(const T* __restrict__ part, int nparts, T* __restrict__ box)
+{
+  if (threadIdx.x >= 6) return;
+  const int c = threadIdx.x;
+  T         v = c < 3 ? T(INFINITY) : T(-INFINITY);
+  for (int i = 0; i < nparts; i++) v = c < 3 ? tmin(v, part[6 * i + c]) : tmax(v, part[6 * i + c]);
+  box[c] = v;
+}
+
+template <class T>
+hipError_t bbox_impl(const T* pts, uint64_t n, T* box, T* part, hipStream_t s)
+{
+  const int blocks = (int)std::min<uint64_t>(kBboxParts, std::max<uint64_t>(1, (n + 255) / 256));
+  hipLaunchKernelGGL(bbox_partial<T>, dim3(blocks), dim3(256), 0, s, pts, n, part);
+  hipLaunchKernelGGL(bbox_final<T>, dim3(1), dim3(64), 0, s, part, blocks, box);
+  return hipGetLastError();
+}
+
+// ------------------------------- codes ---------------------------------------------------
+
+__global__ __launch_bounds__(256) void codes_f32(const float* __restrict__ p, uint64_t n, const float* __restrict__ box,
+                                                 uint64_t* __restrict__ codes)
+{
+  const uint64_t i = (uint64_t)blockIdx.x * 256 + threadIdx.x;
+  if (i >= n) return;
+  const float is = 1.0f / (float)kValidSpan;
+  uint64_t    code = 0;
+#pragma unroll
+  for (int c = 0; c < 3; c++) {
+    const float lo = box[c];
+    float       dl = is * (box[3 + c] - lo);
+    if (dl == 0.0f) dl = 1.0f;
+    const float f = floorf((p[3 * i + c] - lo) / dl); // IEEE division (built without fast-math)
+    const uint32_t b = (uint32_t)(f < 0.0f ? 0.0f : (f > (float)kValidSpan ? (float)kValidSpan : f));
+    code |= spread3(b) << c;
+  }
+  codes[i] = code;
+}
+
+__global__ __launch_bounds__(256) void codes_f64(const double* __restrict__ p, uint64_t n, const double* __restrict__ box,
+                                                 uint64_t* __restrict__ codes)
+{
+  const uint64_t i = (uint64_t)blockIdx.x * 256 + threadIdx.x;
+  if (i >= n) return;
+  const double is = 1.0 / (double)kValidSpan;
+  uint64_t     code = 0;
+#pragma unroll
+  for (int c = 0; c < 3; c++) {
+    const double lo = box[c];
+    double       dl = is * (box[3 + c] - lo);
+    if (dl == 0.0) dl = 1.0;
+    const double f = floor((p[3 * i + c] - lo) / dl);
+    const uint32_t b = (uint32_t)(f < 0.0 ? 0.0 : (f > (double)kValidSpan ? (double)kValidSpan : f));
+    code |= spread3(b) << c;
+  }
+  codes[i] = code;
+}
+
+// ------------------------------- radix sort ----------------------------------------------
+
+constexpr int kSortThreads = 256;
+constexpr int kSortItems   = 16;
+constexpr int kTile        = kSortThreads * kSortItems; // 4096 keys per workgroup
+
+__global__ __launch_bounds__(kSortThreads) void radix_hist(const uint64_t* __restrict__ keys, uint64_t n, int shift,
+                                                           uint32_t* __restrict__ hist, uint32_t nblocks)
+{
+  __shared__ uint32_t h[256];
+  h[threadIdx.x] = 0;
+  __syncthreads();
+  const uint64_t base = (uint64_t)blockIdx.x * kTile;
+#pragma unroll 4
+  for (int r = 0; r < kSortItems; r++) {
+    const uint64_t i = base + (uint64_t)r * kSortThreads + threadIdx.x;
+    if (i < n) atomicAdd(&h[(uint32_t)(keys[i] >> shift) & 255u], 1u);
+  }
+  __syncthreads();
+  hist[(uint64_t)threadIdx.x * nblocks + blockIdx.x] = h[threadIdx.x];
+}
+
+// exclusive scan of m uint32 values in place: three passes over chunks of 1024
+constexpr int kScanChunk = 1024;
+
+__global__ __launch_bounds__(256) void scan_reduce(const uint32_t* __restrict__ a, uint64_t m, uint32_t* __restrict__ sums)
+{
+  __shared__ uint32_t sh[256];
+  const uint64_t      base = (uint64_t)blockIdx.x * kScanChunk;
+  uint32_t            s    = 0;
+  for (int k = 0; k < 4; k++) {
+    const uint64_t i = base + (uint64_t)threadIdx.x * 4 + k;
+    if (i < m) s += a[i];
+  }
+  sh[threadIdx.x] = s;
+  __syncthreads();
+  for (int o = 128; o > 0; o >>= 1) {
+    if (threadIdx.x < (unsigned)o) sh[threadIdx.x] += sh[threadIdx.x + o];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) sums[blockIdx.x] = sh[0];
+}
+
+__global__ __launch_bounds__(1024) void scan_sums(uint32_t* __restrict__ sums, uint32_t nsum)
+{
+  __shared__ uint32_t sh[1024];
+  __shared__ uint32_t carry;
+  if (threadIdx.x == 0) carry = 0;
+  __syncthreads();
+  for (uint32_t base = 0; base < nsum; base += 1024) {
+    const uint32_t i = base + threadIdx.x;
+    const uint32_t v = i < nsum ? sums[i] : 0u;
+    sh[threadIdx.x]  = v;
+    __syncthreads();
+    for (int o = 1; o < 1024; o <<= 1) {
+      const uint32_t t = threadIdx.x >= (unsigned)o ? sh[threadIdx.x - o] : 0u;
+      __syncthreads();
+      sh[threadIdx.x] += t;
+      __syncthreads();
+    }
+    if (i < nsum) sums[i] = carry + sh[threadIdx.x] - v;
+    __syncthreads();
+    if (threadIdx.x == 1023) carry += sh[1023];
+    __syncthreads();
+  }
+}
+
+__global__ __launch_bounds__(256) void scan_apply(uint32_t* __restrict__ a, uint64_t m, const uint32_t* __restrict__ sums)
+{
+  __shared__ uint32_t sh[256];
+  const uint64_t      base = (uint64_t)blockIdx.x * kScanChunk + (uint64_t)threadIdx.x * 4;
+  uint32_t            v[4];
+  uint32_t            s = 0;
+  for (int k = 0; k < 4; k++) {
+    v[k] = (base + k < m) ? a[base + k] : 0u;
+    s += v[k];
+  }
+  sh[threadIdx.x] = s;
+  __syncthreads();
+  for (int o = 1; o < 256; o <<= 1) {
+    const uint32_t t = threadIdx.x >= (unsigned)o ? sh[threadIdx.x - o] : 0u;
+    __syncthreads();
+    sh[threadIdx.x] += t;
+    __syncthreads();
+  }
+  uint32_t run = sums[blockIdx.x] + sh[threadIdx.x] - s;
+  for (int k = 0; k < 4; k++) {
+    if (base + k < m) a[base + k] = run;
+    run += v[k];
+  }
+}
+
+// Stable scatter: keys of the tile are ranked in index order, 256 at a time; within a
+// wavefront the rank among equal digits comes from 8 ballots (one per digit bit).
+__global__ __launch_bounds__(kSortThreads) void radix_scatter(const uint64_t* __restrict__ kin,
+                                                              const uint32_t* __restrict__ vin, uint64_t n, int shift,
+                                                              const uint32_t* __restrict__ offs, uint32_t nblocks,
+                                                              uint64_t* __restrict__ kout, uint32_t* __restrict__ vout)
+{
+  constexpr int       W = kSortThreads / 64;
+  __shared__ uint32_t base[256];
+  __shared__ uint32_t wcnt[W][256];
+  const int           lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  base[threadIdx.x] = offs[(uint64_t)threadIdx.x * nblocks + blockIdx.x];
+  for (int w = 0; w < W; w++) wcnt[w][threadIdx.x] = 0;
+  __syncthreads();
+  const uint64_t tile = (uint64_t)blockIdx.x * kTile;
+  const uint64_t lt   = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+  for (int r = 0; r < kSortItems; r++) {
+    const uint64_t i     = tile + (uint64_t)r * kSortThreads + threadIdx.x;
+    const bool     valid = i < n;
+    uint64_t       key   = valid ? kin[i] : 0ull;
+    uint32_t       val   = valid ? (vin ? vin[i] : (uint32_t)i) : 0u;
+    const uint32_t d     = (uint32_t)(key >> shift) & 255u;
+    uint64_t       peers = __ballot(valid);
+#pragma unroll
+    for (int b = 0; b < 8; b++) {
+      const uint64_t bal = __ballot((d >> b) & 1u);
+      peers &= ((d >> b) & 1u) ? bal : ~bal;
+    }
+    const uint32_t rank   = (uint32_t)__popcll(peers & lt);
+    const uint32_t ncount = (uint32_t)__popcll(peers);
+    if (valid && rank == 0) wcnt[wave][d] = ncount;
+    __syncthreads();
+    if (valid) {
+      uint32_t pre = 0;
+      for (int w = 0; w < wave; w++) pre += wcnt[w][d];
+      const uint32_t pos = base[d] + pre + rank;
+      kout[pos]          = key;
+      vout[pos]          = val;
+    }
+    __syncthreads();
+    uint32_t tot = 0;
+    for (int w = 0; w < W; w++) {
+      tot += wcnt[w][threadIdx.x];
+      wcnt[w][threadIdx.x] = 0;
+    }
+    base[threadIdx.x] += tot;
+    __syncthreads();
+  }
+}
+
+} // namespace
+
+hipError_t launch_bbox_f32(const float* pts, uint64_t n, float* d_box6, void* part, hipStream_t s)
+{
+  return bbox_impl<float>(pts, n, d_box6, static_cast<float*>(part), s);
+}
+hipError_t launch_bbox_f64(const double* pts, uint64_t n, double* d_box6, void* part, hipStream_t s)
+{
+  return bbox_impl<double>(pts, n, d_box6, static_cast<double*>(part), s);
+}
+
+hipError_t launch_codes_f32(const float* pts, uint64_t n, const float* box, uint64_t* codes, hipStream_t s)
+{
+  if (n == 0) return hipSuccess;
+  hipLaunchKernelGGL(codes_f32, dim3((uint32_t)((n + 255) / 256)), dim3(256), 0, s, pts, n, box, codes);
+  return hipGetLastError();
+}
+hipError_t launch_codes_f64(const double* pts, uint64_t n, const double* box, uint64_t* codes, hipStream_t s)
+{
+  if (n == 0) return hipSuccess;
+  hipLaunchKernelGGL(codes_f64, dim3((uint32_t)((n + 255) / 256)), dim3(256), 0, s, pts, n, box, codes);
+  return hipGetLastError();
+}
+
+static uint64_t round_up(uint64_t x, uint64_t a) { return (x + a - 1) / a * a; }
+
+std::size_t radix_sort_scratch_bytes(uint64_t n)
+{
+  const uint64_t nb   = (n + kTile - 1) / kTile;
+  const uint64_t m    = 256 * nb;
+  const uint64_t nsum = (m + kScanChunk - 1) / kScanChunk;
+  return round_up(n * 8, 256) + round_up(n * 4, 256) + round_up(m * 4, 256) + round_up(nsum * 4, 256) + 256;
+}
+
+hipError_t radix_sort_pairs(const uint64_t* keys_in, const uint32_t* values_in, uint64_t* keys_out,
+                            uint32_t* values_out, uint64_t n, int bits, void* scratch, hipStream_t s)
+{
+  if (n == 0) return hipSuccess;
+  if (n > 0xFFFFFFFFull) return hipErrorInvalidValue;
+  const uint32_t nb    = (uint32_t)((n + kTile - 1) / kTile);
+  const uint64_t m     = 256ull * nb;
+  const uint32_t nsum  = (uint32_t)((m + kScanChunk - 1) / kScanChunk);
+  char*          p     = static_cast<char*>(scratch);
+  uint64_t*      ktmp  = reinterpret_cast<uint64_t*>(p);
+  p += round_up(n * 8, 256);
+  uint32_t* vtmp = reinterpret_cast<uint32_t*>(p);
+  p += round_up(n * 4, 256);
+  uint32_t* hist = reinterpret_cast<uint32_t*>(p);
+  p += round_up(m * 4, 256);
+  uint32_t* sums = reinterpret_cast<uint32_t*>(p);
+
+  const int passes = (bits + 7) / 8;
+  const uint64_t* ksrc = keys_in;
+  const uint32_t* vsrc = values_in;
+  for (int pass = 0; pass < passes; pass++) {
+    // the last pass must land in the output buffers
+    const bool to_out = ((passes - 1 - pass) % 2) == 0;
+    uint64_t*  kdst   = to_out ? keys_out : ktmp;
+    uint32_t*  vdst   = to_out ? values_out : vtmp;
+    const int  shift  = 8 * pass;
+    hipLaunchKernelGGL(radix_hist, dim3(nb), dim3(kSortThreads), 0, s, ksrc, n, shift, hist, nb);
+    hipLaunchKernelGGL(scan_reduce, dim3(nsum), dim3(256), 0, s, hist, m, sums);
+    hipLaunchKernelGGL(scan_sums, dim3(1), dim3(1024), 0, s, sums, nsum);
+    hipLaunchKernelGGL(scan_apply, dim3(nsum), dim3(256), 0, s, hist, m, sums);
+    hipLaunchKernelGGL(radix_scatter, dim3(nb), dim3(kSortThreads), 0, s, ksrc, vsrc, n, shift, hist, nb, kdst, vdst);
+    ksrc = kdst;
+    vsrc = vdst;
+  }
+  return hipGetLastError();
+}
+
+} // namespace ebgpu

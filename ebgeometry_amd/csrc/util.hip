@@ -1,0 +1,40 @@
+// Small device utilities used by the query pipeline. Counters are cleared with a kernel (not
+// hipMemsetAsync) so that a captured hipGraph replays them correctly.
+#include "kernels.hpp"
+
+#include <hip/hip_runtime.h>
+
+namespace ebgpu {
+
+namespace {
+
+__global__ void fill_u32_kernel(uint32_t* p, uint64_t n, uint32_t v)
+{
+  const uint64_t i = (uint64_t)blockIdx.x * 256 + threadIdx.x;
+  if (i < n) p[i] = v;
+}
+
+__global__ void add_counts_kernel(unsigned long long* work, const uint32_t* a, const uint32_t* b)
+{
+  if (threadIdx.x == 0) {
+    if (a) work[2] += *a;
+    if (b) work[3] += *b;
+  }
+}
+
+} // namespace
+
+hipError_t launch_fill_u32(uint32_t* p, uint64_t n, uint32_t v, hipStream_t s)
+{
+  if (n == 0) return hipSuccess;
+  hipLaunchKernelGGL(fill_u32_kernel, dim3((uint32_t)((n + 255) / 256)), dim3(256), 0, s, p, n, v);
+  return hipGetLastError();
+}
+
+hipError_t launch_add_counts(unsigned long long* work, const uint32_t* recheck, const uint32_t* fallback, hipStream_t s)
+{
+  hipLaunchKernelGGL(add_counts_kernel, dim3(1), dim3(64), 0, s, work, recheck, fallback);
+  return hipGetLastError();
+}
+
+} // namespace ebgpu

@@ -1,0 +1,181 @@
+/*
+ * ebgpu -- MI355X (gfx950) batched signed-distance queries: C ABI.
+ *
+ * Drop-in boundary for the reference's signed-distance query path. The reference exposes
+ * only a per-point virtual call,
+ *     ImplicitFunction<T>::value(const Vec3T<T>&) const      (Source/EBGeometry_ImplicitFunction.hpp:51-52)
+ *     SignedDistanceFunction<T>::signedDistance(...)          (Source/EBGeometry_SignedDistanceFunction.hpp:54-62)
+ * on objects built by
+ *     TriMeshSDF(mesh, BVH::Build, maxLeafGroups)             (Source/EBGeometry_MeshDistanceFunctions.hpp:376-387)
+ *     BVHUnionIF(prims, bvs)                                  (Source/EBGeometry_CSG.hpp:401-440)
+ *     Translate / Rotate / Scale                              (Source/EBGeometry_Transform.hpp:44-70)
+ * This header gives the batched equivalent: opaque handles built once, then queries over
+ * arrays of points. Plain pointers and sizes only.
+ *
+ * Conventions
+ *   - Signed distance is negative inside (outward-oriented triangles: the face normal is
+ *     (x2-x0)x(x2-x1), Source/EBGeometry_TriangleImplem.hpp:94-106).
+ *   - Points and vertices are float32 xyz triples. Distances are returned as float32.
+ *   - Functions named *_host take host pointers and are synchronous. All others take device
+ *     pointers and enqueue on `stream` (a hipStream_t; NULL = the null stream).
+ *   - Every call returns an ebg_status; ebg_last_error() gives a message for the calling
+ *     thread. Unlike the reference (which aborts on a failed EBGEOMETRY_EXPECT,
+ *     Source/EBGeometry_Macros.hpp:48-63), nothing aborts.
+ *   - Handles are reentrant for queries enqueued on one stream. Queries on the same handle from
+ *     different streams must be ordered by the caller (they share the handle's workspace).
+ */
+#ifndef EBGPU_H
+#define EBGPU_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum ebg_status
+{
+  EBG_OK             = 0,
+  EBG_INVALID_ARG    = 1,
+  EBG_OOM            = 2,
+  EBG_HIP_ERROR      = 3,
+  EBG_STACK_OVERFLOW = 4, /* a traversal needed more stack than the build guaranteed */
+  EBG_NOT_READY      = 5, /* e.g. learned bound requested before a network was trained */
+  EBG_UNSUPPORTED    = 6
+} ebg_status;
+
+typedef struct ebg_mesh  ebg_mesh;
+typedef struct ebg_scene ebg_scene;
+typedef struct ebg_batch ebg_batch;
+
+/* Message describing the last failure on the calling thread ("" if none). */
+const char* ebg_last_error(void);
+
+/* ---------------------------------------------------------------------------------------- */
+/* Mesh build -- replaces TriMeshSDF(mesh, BVH::Build::SAH|Morton, maxLeafGroups)             */
+/* (Source/EBGeometry_MeshDistanceFunctionsImplem.hpp:417-460) and the DCEL pseudo-normals   */
+/* it copies per triangle (Source/EBGeometry_DCEL_{Face,Edge,Vertex}Implem.hpp).              */
+/* ---------------------------------------------------------------------------------------- */
+
+enum ebg_builder
+{
+  EBG_BUILD_SAH    = 0, /* binned SAH on the host (default; best traversal quality)            */
+  EBG_BUILD_MORTON = 1  /* Morton codes + radix sort + balanced K-ary tree on the GPU
+                           (the reference's Build::Morton, Source/EBGeometry_BVHImplem.hpp:207-312) */
+};
+
+typedef struct ebg_build_opts
+{
+  int32_t builder;    /* enum ebg_builder                                                   */
+  int32_t branching;  /* K: 4 or 8 children per node (0 = 4)                               */
+  int32_t leaf_size;  /* SAH: max triangles per leaf, 1..16 (0 = 4)                          */
+  int32_t reserved;
+} ebg_build_opts;
+
+typedef struct ebg_mesh_info
+{
+  double   bbox_lo[3], bbox_hi[3];
+  double   diag;              /* bbox diagonal                                               */
+  uint64_t num_vertices, num_triangles, num_nodes, num_leaves;
+  int32_t  branching, leaf_size, stack_need, stack_capacity;
+  uint64_t device_bytes;      /* HBM held by the handle (excluding query workspace)          */
+  double   build_seconds;
+} ebg_mesh_info;
+
+/* Triangles are (v0,v1,v2) vertex-index triples forming a closed, consistently outward
+ * oriented 2-manifold (like the reference's DCEL meshes). Host pointers; data is copied. */
+ebg_status ebg_mesh_create(const float* xyz, uint64_t num_vertices, const uint32_t* tri,
+                           uint64_t num_triangles, const ebg_build_opts* opts, ebg_mesh** out);
+ebg_status ebg_mesh_destroy(ebg_mesh* mesh);
+ebg_status ebg_mesh_get_info(const ebg_mesh* mesh, ebg_mesh_info* info);
+
+/* Host-only (no GPU needed): the pseudo-normals ebg_mesh_create computes, per face: face normal
+ * (3 doubles), vertex normals of v0,v1,v2 (9), normals of the edges v0->v1, v1->v2, v2->v0 (9).
+ * Replaces MeshT::reconcile(VertexNormalWeight::Angle) (Source/EBGeometry_DCEL_MeshImplem.hpp:304-309)
+ * as it is read back per triangle by TriMeshSDF (MeshDistanceFunctionsImplem.hpp:429-452). */
+ebg_status ebg_pseudo_normals_host(const float* xyz, uint64_t num_vertices, const uint32_t* tri,
+                                   uint64_t num_triangles, double* face_n, double* vert_n,
+                                   double* edge_n);
+
+/* ---------------------------------------------------------------------------------------- */
+/* Queries -- batched TriMeshSDF::signedDistance / getClosestTriangle                         */
+/* (Source/EBGeometry_MeshDistanceFunctionsImplem.hpp:480-549)                                */
+/* ---------------------------------------------------------------------------------------- */
+
+typedef struct ebg_query_opts
+{
+  int32_t  presort;        /* 1: sort queries by Morton code before traversal (default)     */
+  int32_t  fp64_recheck;   /* 1: redo |d| < recheck_eps*diag in FP64 (default)              */
+  double   recheck_eps;    /* default 1e-4                                                  */
+  int32_t  learned_bound;  /* 1: seed each query's cull radius with the mesh's trained MLP */
+  int32_t  fp64_all;       /* 1: FP64 polish of every query (exact mode, slower)            */
+  float    bound_scale;    /* radius = bound_scale*pred + bound_offset*diag (1.25, 1e-3)    */
+  float    bound_offset;
+  void*    stream;         /* hipStream_t                                                   */
+  uint64_t* work;          /* optional device array of 4 uint64 accumulated per call:
+                              [0] interior nodes whose children each query tested,
+                              [1] real triangles tested, [2] queries rechecked in FP64,
+                              [3] queries that fell back to the full traversal              */
+} ebg_query_opts;
+
+/* Fill *opts with the defaults above. */
+void ebg_query_opts_default(ebg_query_opts* opts);
+
+/* d_pts: n float32 xyz triples; d_dist: n float32; d_tri (nullable): n uint32 face indices
+ * (closest face, up to exact ties). Device pointers, asynchronous on opts->stream. */
+ebg_status ebg_query_sdf(const ebg_mesh* mesh, const float* d_pts, uint64_t n, float* d_dist,
+                         uint32_t* d_tri, const ebg_query_opts* opts);
+/* Host-pointer convenience form; synchronous. */
+ebg_status ebg_query_sdf_host(const ebg_mesh* mesh, const float* pts, uint64_t n, float* dist,
+                              uint32_t* tri, const ebg_query_opts* opts);
+
+/* Time the dominant traversal kernel of the last query on `mesh` (HIP events recorded on the
+ * launch stream). Returns milliseconds of the FP32 traversal launch(es), or <0 if none. */
+double ebg_mesh_last_traversal_ms(const ebg_mesh* mesh);
+
+/* ---------------------------------------------------------------------------------------- */
+/* Space-filling curve -- SFC::computeBins<T>, Morton::encode, SFC::order                      */
+/* (Source/EBGeometry_SFCImplem.hpp:31-58, 228-300). Device pointers, async on stream.        */
+/* Bit-exact with the reference; equal codes are ordered by increasing index (stable).        */
+/* ---------------------------------------------------------------------------------------- */
+
+ebg_status ebg_morton_codes_f32(const float* d_pts, uint64_t n, uint64_t* d_codes, void* stream);
+ebg_status ebg_morton_codes_f64(const double* d_pts, uint64_t n, uint64_t* d_codes, void* stream);
+/* Stable LSD radix sort of (code, index) pairs; d_perm receives the sorted indices. Scratch is
+ * allocated internally. d_sorted_codes is nullable. */
+ebg_status ebg_morton_sort(const uint64_t* d_codes, uint64_t n, uint32_t* d_perm,
+                           uint64_t* d_sorted_codes, void* stream);
+
+/* ---------------------------------------------------------------------------------------- */
+/* Learned cull bound (no reference counterpart)                                              */
+/* ---------------------------------------------------------------------------------------- */
+
+typedef struct ebg_mlp_opts
+{
+  uint64_t num_samples;  /* training pairs generated by the mesh's own BVH (default 4M)      */
+  int32_t  epochs;       /* default 8                                                       */
+  int32_t  batch;        /* default 4096                                                    */
+  float    lr;           /* Adam learning rate (default 2e-3)                               */
+  uint64_t seed;
+  float    near_sigma;   /* surface offset sigma for half the samples (default 0.02)         */
+  float    box_scale;    /* uniform half sampled in box_scale x bbox (default 1.2)            */
+  void*    stream;
+} ebg_mlp_opts;
+
+void       ebg_mlp_opts_default(ebg_mlp_opts* opts);
+/* Train the mesh's bound network on the GPU (BF16 MFMA forward/backward, Adam). Writes the
+ * final mean training loss to *loss (nullable). */
+ebg_status ebg_mlp_train(ebg_mesh* mesh, const ebg_mlp_opts* opts, float* loss);
+/* Forward pass only: predicted unsigned distance for n device points. */
+ebg_status ebg_mlp_predict(const ebg_mesh* mesh, const float* d_pts, uint64_t n, float* d_pred,
+                           void* stream);
+/* Parameters as float32 in a fixed order (see ebg_mlp_param_count); host pointers. */
+uint64_t   ebg_mlp_param_count(void);
+ebg_status ebg_mlp_get_params(const ebg_mesh* mesh, float* params);
+ebg_status ebg_mlp_set_params(ebg_mesh* mesh, const float* params);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,188 @@
+"""ctypes front-end of the compiled reference (oracle/_ref/libebgref*.so) -- test infrastructure.
+
+The libraries are built by oracle/Makefile from the reference headers (never copied into
+this repository). ``available()`` is False where they were not built.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_REF = os.path.join(_HERE, "_ref")
+
+_d = C.POINTER(C.c_double)
+_f = C.POINTER(C.c_float)
+_u32 = C.POINTER(C.c_uint32)
+_u64 = C.POINTER(C.c_uint64)
+
+_LIBS: dict = {}
+
+
+def _cpu_flags() -> set:
+    try:
+        with open("/proc/cpuinfo") as fh:
+            for line in fh:
+                if line.startswith("flags"):
+                    return set(line.split(":", 1)[1].split())
+    except OSError:
+        pass
+    return set()
+
+
+def variants():
+    """Reference builds usable on this host, fastest first."""
+    fl = _cpu_flags()
+    out = []
+    if {"avx512f", "avx512dq", "avx512bw", "avx512vl", "avx2", "fma", "bmi2"} <= fl:
+        out.append("avx512")
+    if {"avx2", "fma"} <= fl:
+        out.append("avx2")
+    out.append("portable")
+    return [v for v in out if os.path.exists(_path(v))]
+
+
+def _path(variant: str) -> str:
+    return os.path.join(_REF, "libebgref.so" if variant == "portable" else f"libebgref_{variant}.so")
+
+
+def available(variant: str = "portable") -> bool:
+    return os.path.exists(_path(variant))
+
+
+def lib(variant: str = "portable"):
+    if variant not in _LIBS:
+        L = C.CDLL(_path(variant))
+        L.ebgref_info.restype = C.c_int
+        L.ebgref_info.argtypes = [C.POINTER(C.c_int), C.POINTER(C.c_int)]
+        L.ebgref_mesh_create.restype = C.c_void_p
+        L.ebgref_mesh_create.argtypes = [_d, C.c_int64, _u32, C.c_int64, C.c_int]
+        L.ebgref_mesh_destroy.argtypes = [C.c_void_p]
+        L.ebgref_mesh_sdf.argtypes = [C.c_void_p, _d, C.c_int64, _d, C.c_int]
+        L.ebgref_mesh_closest.argtypes = [C.c_void_p, _d, C.c_int64, _d, _u32, C.c_int]
+        L.ebgref_mesh_normals.argtypes = [C.c_void_p, _d, _d, _d]
+        L.ebgref_bins_f32.argtypes = [_f, C.c_int64, _u32]
+        L.ebgref_bins_f64.argtypes = [_d, C.c_int64, _u32]
+        L.ebgref_morton_encode.argtypes = [_u32, C.c_int64, _u64]
+        L.ebgref_morton_order_f32.argtypes = [_f, C.c_int64, _u32]
+        L.ebgref_scene_create.restype = C.c_void_p
+        L.ebgref_scene_create.argtypes = [C.c_int64, _d, _d, C.c_int64, _d, _d, C.c_int64,
+                                          C.POINTER(C.c_void_p), _d, _d, _d, _d, _d]
+        L.ebgref_scene_destroy.argtypes = [C.c_void_p]
+        L.ebgref_scene_value.argtypes = [C.c_void_p, _d, C.c_int64, _d, C.c_int]
+        _LIBS[variant] = L
+    return _LIBS[variant]
+
+
+def _p(a, t):
+    return a.ctypes.data_as(t)
+
+
+def _f64(a, shape=None):
+    a = np.ascontiguousarray(a, dtype=np.float64)
+    return a if shape is None else a.reshape(shape)
+
+
+def info(variant: str = "portable"):
+    k, w = C.c_int(), C.c_int()
+    isa = lib(variant).ebgref_info(C.byref(k), C.byref(w))
+    return {"K": k.value, "W": w.value, "simd_bits": isa, "variant": variant}
+
+
+class Mesh:
+    """Reference TriMeshSDF<double, uint32_t, K, W>(mesh, SAH, maxLeafGroups) via soupToDCEL."""
+
+    def __init__(self, verts, faces, max_leaf_groups: int = 4, variant: str = "portable"):
+        self.L = lib(variant)
+        v = _f64(verts, (-1, 3))
+        f = np.ascontiguousarray(faces, dtype=np.uint32).reshape(-1, 3)
+        self.nt = f.shape[0]
+        self._h = self.L.ebgref_mesh_create(_p(v, _d), v.shape[0], _p(f, _u32), f.shape[0], max_leaf_groups)
+
+    def __del__(self):
+        if getattr(self, "_h", None):
+            self.L.ebgref_mesh_destroy(self._h)
+            self._h = None
+
+    @property
+    def handle(self):
+        return self._h
+
+    def sdf(self, pts, nthreads: int | None = None):
+        p = _f64(pts, (-1, 3))
+        d = np.zeros(p.shape[0])
+        self.L.ebgref_mesh_sdf(self._h, _p(p, _d), p.shape[0], _p(d, _d), nthreads or os.cpu_count() or 1)
+        return d
+
+    def closest(self, pts, nthreads: int | None = None):
+        p = _f64(pts, (-1, 3))
+        d = np.zeros(p.shape[0])
+        t = np.zeros(p.shape[0], dtype=np.uint32)
+        self.L.ebgref_mesh_closest(self._h, _p(p, _d), p.shape[0], _p(d, _d), _p(t, _u32),
+                                   nthreads or os.cpu_count() or 1)
+        return d, t
+
+    def normals(self):
+        fn = np.zeros((self.nt, 3))
+        vn = np.zeros((self.nt, 3, 3))
+        en = np.zeros((self.nt, 3, 3))
+        self.L.ebgref_mesh_normals(self._h, _p(fn, _d), _p(vn, _d), _p(en, _d))
+        return fn, vn, en
+
+
+def bins_f32(pts, variant="portable"):
+    p = np.ascontiguousarray(pts, dtype=np.float32).reshape(-1, 3)
+    b = np.zeros(p.shape, dtype=np.uint32)
+    lib(variant).ebgref_bins_f32(_p(p, _f), p.shape[0], _p(b, _u32))
+    return b
+
+
+def bins_f64(pts, variant="portable"):
+    p = _f64(pts, (-1, 3))
+    b = np.zeros(p.shape, dtype=np.uint32)
+    lib(variant).ebgref_bins_f64(_p(p, _d), p.shape[0], _p(b, _u32))
+    return b
+
+
+def morton_encode(bins, variant="portable"):
+    b = np.ascontiguousarray(bins, dtype=np.uint32).reshape(-1, 3)
+    c = np.zeros(b.shape[0], dtype=np.uint64)
+    lib(variant).ebgref_morton_encode(_p(b, _u32), b.shape[0], _p(c, _u64))
+    return c
+
+
+def morton_order_f32(pts, variant="portable"):
+    p = np.ascontiguousarray(pts, dtype=np.float32).reshape(-1, 3)
+    o = np.zeros(p.shape[0], dtype=np.uint32)
+    lib(variant).ebgref_morton_order_f32(_p(p, _f), p.shape[0], _p(o, _u32))
+    return o
+
+
+class Scene:
+    """Reference BVHUnionIF over spheres, boxes and transformed TriMeshSDF instances."""
+
+    def __init__(self, sphere_c, sphere_r, box_lo, box_hi, meshes, mt, mrot, ms, mbv_lo, mbv_hi,
+                 variant="portable"):
+        self.L = lib(variant)
+        self._keep = [_f64(sphere_c, (-1, 3)), _f64(sphere_r).ravel(), _f64(box_lo, (-1, 3)),
+                      _f64(box_hi, (-1, 3)), _f64(mt, (-1, 3)), _f64(mrot, (-1, 3)), _f64(ms).ravel(),
+                      _f64(mbv_lo, (-1, 3)), _f64(mbv_hi, (-1, 3))]
+        sc, sr, blo, bhi, t, r, s, bl, bh = self._keep
+        self._meshes = list(meshes)
+        arr = (C.c_void_p * max(1, len(meshes)))(*[m.handle for m in meshes])
+        self._h = self.L.ebgref_scene_create(sc.shape[0], _p(sc, _d), _p(sr, _d), blo.shape[0], _p(blo, _d),
+                                             _p(bhi, _d), len(meshes), arr, _p(t, _d), _p(r, _d), _p(s, _d),
+                                             _p(bl, _d), _p(bh, _d))
+
+    def __del__(self):
+        if getattr(self, "_h", None):
+            self.L.ebgref_scene_destroy(self._h)
+            self._h = None
+
+    def value(self, pts, nthreads=None):
+        p = _f64(pts, (-1, 3))
+        v = np.zeros(p.shape[0])
+        self.L.ebgref_scene_value(self._h, _p(p, _d), p.shape[0], _p(v, _d), nthreads or os.cpu_count() or 1)
+        return v
