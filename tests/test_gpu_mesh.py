@@ -28,10 +28,13 @@ def face_dist(v, f, fn, vn, en, tri, p):
     return abs(port.triangle_sdf(x, fn[tri], vn[tri], en[tri], p))
 
 
-def check_parity(v, f, q, d_ref, tri_ref, d, tri, label=""):
+def check_parity(v, f, q, d_ref, tri_ref, d, tri, label="", rel=0.0):
+    """rel: extra tolerance relative to |d_ref| -- only for points far outside the workload box,
+    where one float32 ulp of the output distance alone exceeds D_TOL * diag."""
     diag = diag_of(v)
     err = np.abs(d.astype(np.float64) - d_ref)
-    assert err.max() <= D_TOL * diag, f"{label}: max |d-d_ref| = {err.max():.3e} (diag {diag:.3f})"
+    lim = D_TOL * diag + rel * np.abs(d_ref)
+    assert np.all(err <= lim), f"{label}: max |d-d_ref| = {err.max():.3e} (diag {diag:.3f})"
     nz = d_ref != 0
     sgn_bad = np.nonzero(nz & (np.sign(d) != np.sign(d_ref)))[0]
     assert sgn_bad.size == 0, f"{label}: {sgn_bad.size} sign mismatches, e.g. q={q[sgn_bad[0]]} d={d[sgn_bad[0]]} ref={d_ref[sgn_bad[0]]}"
@@ -42,7 +45,7 @@ def check_parity(v, f, q, d_ref, tri_ref, d, tri, label=""):
         for i in np.nonzero(tri != tri_ref)[0]:
             a = face_dist(v, f, fn, vn, en, tri[i], qq[i])
             b = abs(d_ref[i])
-            assert a - b <= TIE_TOL * diag, f"{label}: query {i} face {tri[i]} at {a} vs closest {b}"
+            assert a - b <= TIE_TOL * diag + rel * b, f"{label}: query {i} face {tri[i]} at {a} vs closest {b}"
 
 
 FIXTURES = ["dodecahedron.npz", "tetrahedron.npz", "ico3_displaced.npz"]
@@ -155,7 +158,8 @@ def test_points_on_vertices_and_far_away(cuda):
     m = TriMeshSDF(v, f)
     d, t = m.closest_triangle(q)
     assert np.all(np.abs(d[:500]) <= 1e-6)
-    check_parity(v, f, q, d_ref, t_ref, d, t, "vertices+far")
+    # |d| ~ 1e3: float32 resolution is 6e-5, so the bar is 1e-5*diag + 4 ulp(|d|)
+    check_parity(v, f, q, d_ref, t_ref, d, t, "vertices+far", rel=4 * 2.0 ** -24)
 
 
 def test_signed_distance_single_point_api(cuda):
