@@ -30,12 +30,13 @@ class MeshInfo(C.Structure):
 class QueryOpts(C.Structure):
     _fields_ = [("presort", _i32), ("fp64_recheck", _i32), ("recheck_eps", C.c_double),
                 ("learned_bound", _i32), ("fp64_all", _i32), ("bound_scale", C.c_float),
-                ("bound_offset", C.c_float), ("stream", _vp), ("work", _vp)]
+                ("bound_offset", C.c_float), ("stream", _vp), ("work", _vp), ("seed_radius", _vp)]
 
 
 class MlpOpts(C.Structure):
     _fields_ = [("num_samples", _u64), ("epochs", _i32), ("batch", _i32), ("lr", C.c_float), ("seed", _u64),
-                ("near_sigma", C.c_float), ("box_scale", C.c_float), ("stream", _vp)]
+                ("near_sigma", C.c_float), ("box_scale", C.c_float), ("stream", _vp),
+                ("under_weight", C.c_float), ("reserved", _i32)]
 
 
 # (name, restype, argtypes) of every exported entry point
@@ -58,6 +59,7 @@ SIGNATURES = [
     ("ebg_mlp_param_count", _u64, []),
     ("ebg_mlp_get_params", C.c_int, [_vp, _vp]),
     ("ebg_mlp_set_params", C.c_int, [_vp, _vp]),
+    ("ebg_mlp_loss_grad", C.c_int, [_vp, _vp, _vp, _u64, _vp, _vp, _vp]),
 ]
 
 STATUS = {0: "OK", 1: "INVALID_ARG", 2: "OOM", 3: "HIP_ERROR", 4: "STACK_OVERFLOW", 5: "NOT_READY",

@@ -114,9 +114,11 @@ struct ebg_mesh
   double      lo[3]{}, hi[3]{}, diag = 0;
   double      build_seconds = 0;
   DevBuf      nodes, tris, tripos, feat, vn32, en32, vn64, en64, fn64;
+  DevBuf      area_cdf; // normalised cumulative triangle area, leaf order (training samples)
   ebgpu::MeshDev dev{};
+  ebgpu::MlpTrainBuffers train;
   std::size_t device_bytes = 0;
-  std::mutex  mu;
+  std::recursive_mutex mu; // training runs queries while holding it
   Workspace   ws;
   hipEvent_t  ev0 = nullptr, ev1 = nullptr;
   bool        timed = false;
@@ -168,6 +170,23 @@ static ebg_status upload_mesh(ebg_mesh* m, ebgpu::PackedMesh& pm)
   EBG_HIP(upload(m->vn64, pm.vn64));
   EBG_HIP(upload(m->en64, pm.en64));
   EBG_HIP(upload(m->fn64, pm.fn64));
+  {
+    const std::size_t  nt = pm.tris.size();
+    std::vector<float> cdf(nt);
+    double             acc = 0.0;
+    std::vector<double> run(nt);
+    for (std::size_t i = 0; i < nt; i++) {
+      const float* x  = &pm.tripos[12 * i];
+      const double ax = double(x[3]) - x[0], ay = double(x[4]) - x[1], az = double(x[5]) - x[2];
+      const double bx = double(x[6]) - x[0], by = double(x[7]) - x[1], bz = double(x[8]) - x[2];
+      const double cx = ay * bz - az * by, cy = az * bx - ax * bz, cz = ax * by - ay * bx;
+      acc += 0.5 * std::sqrt(cx * cx + cy * cy + cz * cz);
+      run[i] = acc;
+    }
+    for (std::size_t i = 0; i < nt; i++) cdf[i] = float(run[i] / acc);
+    if (nt) cdf[nt - 1] = 1.0f;
+    EBG_HIP(upload(m->area_cdf, cdf));
+  }
   m->K          = pm.K;
   m->leaf       = pm.leaf_size;
   m->num_nodes  = pm.num_nodes;
@@ -307,7 +326,7 @@ ebg_status ebg_query_sdf(const ebg_mesh* cm, const float* d_pts, uint64_t n, flo
   if (o.learned_bound && !m->mlp.trained) return fail(EBG_NOT_READY, "learned bound requested but no network trained");
   hipStream_t s = static_cast<hipStream_t>(o.stream);
 
-  std::lock_guard<std::mutex> lock(m->mu);
+  std::lock_guard<std::recursive_mutex> lock(m->mu);
   Workspace&                  ws = m->ws;
   EBG_HIP(ws.ensure(n));
   if (!m->ev0) {
@@ -326,10 +345,10 @@ ebg_status ebg_query_sdf(const ebg_mesh* cm, const float* d_pts, uint64_t n, flo
     idx = ws.perm.as<uint32_t>();
   }
 
-  const float* seeds = nullptr;
+  const float* seeds = o.seed_radius;
   if (o.learned_bound) {
-    EBG_HIP(ebgpu::mlp_seed_radius(m->mlp, m->dev, d_pts, idx, n, o.bound_scale, float(o.bound_offset * m->diag),
-                                   ws.seeds.as<float>(), s));
+    EBG_HIP(ebgpu::mlp_forward(m->mlp, d_pts, idx, n, true, o.bound_scale, float(o.bound_offset * m->diag),
+                               ws.seeds.as<float>(), s));
     seeds = ws.seeds.as<float>();
   }
 
@@ -479,18 +498,128 @@ void ebg_mlp_opts_default(ebg_mlp_opts* o)
   o->seed        = 12345;
   o->near_sigma  = 0.02f;
   o->box_scale   = 1.2f;
+  o->under_weight = 8.0f;
 }
 
 uint64_t ebg_mlp_param_count(void) { return uint64_t(ebgpu::kMlpParams); }
 
-ebg_status ebg_mlp_train(ebg_mesh*, const ebg_mlp_opts*, float*) { return fail(EBG_UNSUPPORTED, "not yet"); }
-
-ebg_status ebg_mlp_predict(const ebg_mesh*, const float*, uint64_t, float*, void*)
+// Input normalisation of the network: the box_scale-scaled mesh bbox maps to [-1, 1]^3.
+static void mlp_set_box(ebg_mesh* m, float box_scale)
 {
-  return fail(EBG_UNSUPPORTED, "not yet");
+  ebgpu::Mlp& p = m->mlp;
+  for (int c = 0; c < 3; c++) {
+    const double ctr = 0.5 * (m->lo[c] + m->hi[c]);
+    double       h   = 0.5 * box_scale * (m->hi[c] - m->lo[c]);
+    if (!(h > 0.0)) h = 1.0;
+    p.lo[c]    = float(ctr - h);
+    p.scale[c] = float(1.0 / h);
+  }
+  p.diag      = float(m->diag);
+  p.box_scale = box_scale;
 }
 
-ebg_status ebg_mlp_get_params(const ebg_mesh*, float*) { return fail(EBG_UNSUPPORTED, "not yet"); }
-ebg_status ebg_mlp_set_params(ebg_mesh*, const float*) { return fail(EBG_UNSUPPORTED, "not yet"); }
+ebg_status ebg_mlp_train(ebg_mesh* m, const ebg_mlp_opts* opts, float* loss)
+{
+  if (!m) return fail(EBG_INVALID_ARG, "mesh is null");
+  ebg_mlp_opts o;
+  ebg_mlp_opts_default(&o);
+  if (opts) o = *opts;
+  if (o.num_samples < 64 || o.num_samples > 0xFFFFFFFFull) return fail(EBG_INVALID_ARG, "num_samples out of range");
+  if (o.epochs < 1 || o.batch < 32 || o.batch > 65536) return fail(EBG_INVALID_ARG, "epochs/batch out of range");
+  if (!(o.lr > 0.0f) || !(o.box_scale >= 1.0f) || !(o.near_sigma >= 0.0f) || !(o.under_weight >= 1.0f))
+    return fail(EBG_INVALID_ARG, "lr, box_scale, near_sigma or under_weight out of range");
+  hipStream_t s = static_cast<hipStream_t>(o.stream);
+  std::lock_guard<std::recursive_mutex> lock(m->mu);
+  ebgpu::Mlp& mlp = m->mlp;
+  EBG_HIP(ebgpu::mlp_alloc(mlp));
+  mlp_set_box(m, o.box_scale);
+  EBG_HIP(ebgpu::mlp_init(mlp, o.seed));
+  mlp.trained      = false;
+  mlp.under_weight = o.under_weight;
+
+  // (point, |d|) pairs from the mesh's own BVH
+  const uint64_t n = o.num_samples;
+  DevBuf         pts, dist;
+  EBG_HIP(pts.reserve(12 * n));
+  EBG_HIP(dist.reserve(4 * n));
+  const float lo[3] = {float(m->lo[0]), float(m->lo[1]), float(m->lo[2])};
+  const float hi[3] = {float(m->hi[0]), float(m->hi[1]), float(m->hi[2])};
+  EBG_HIP(ebgpu::mlp_samples(m->dev, m->area_cdf.as<float>(), uint32_t(m->nt), lo, hi, o.box_scale, o.near_sigma,
+                             o.seed, n, pts.as<float>(), s));
+  ebg_query_opts q;
+  ebg_query_opts_default(&q);
+  q.stream           = o.stream;
+  const ebg_status st = ebg_query_sdf(m, pts.as<float>(), n, dist.as<float>(), nullptr, &q);
+  if (st != EBG_OK) return st;
+
+  // Adam with a cosine learning-rate decay to 5% over all steps
+  EBG_HIP(m->train.ensure(o.batch));
+  const uint64_t spe   = (n + o.batch - 1) / o.batch;
+  const double   total = double(spe) * o.epochs;
+  float          last  = 0.0f;
+  uint64_t       k     = 0;
+  for (int ep = 0; ep < o.epochs; ep++) {
+    EBG_HIP(hipMemsetAsync(m->train.loss, 0, sizeof(float), s));
+    for (uint64_t b = 0; b < n; b += o.batch, k++) {
+      const int   nb = int(std::min<uint64_t>(o.batch, n - b));
+      const float lr = float(o.lr * (0.05 + 0.95 * 0.5 * (1.0 + std::cos(M_PI * double(k) / total))));
+      EBG_HIP(ebgpu::mlp_loss_grad(mlp, m->train, pts.as<float>() + 3 * b, dist.as<float>() + b, nb, s));
+      EBG_HIP(ebgpu::mlp_adam(mlp, lr, s));
+    }
+    float sum = 0.0f;
+    EBG_HIP(hipMemcpyAsync(&sum, m->train.loss, sizeof(float), hipMemcpyDeviceToHost, s));
+    EBG_HIP(hipStreamSynchronize(s));
+    last = sum / float(n);
+  }
+  mlp.trained = true;
+  if (loss) *loss = last;
+  return EBG_OK;
+}
+
+ebg_status ebg_mlp_predict(const ebg_mesh* cm, const float* d_pts, uint64_t n, float* d_pred, void* stream)
+{
+  if (!cm) return fail(EBG_INVALID_ARG, "mesh is null");
+  if (n == 0) return EBG_OK;
+  if (!d_pts || !d_pred) return fail(EBG_INVALID_ARG, "null pointer");
+  if (!cm->mlp.master) return fail(EBG_NOT_READY, "no network");
+  EBG_HIP(ebgpu::mlp_forward(cm->mlp, d_pts, nullptr, n, false, 1.0f, 0.0f, d_pred, static_cast<hipStream_t>(stream)));
+  return EBG_OK;
+}
+
+ebg_status ebg_mlp_get_params(const ebg_mesh* m, float* params)
+{
+  if (!m || !params) return fail(EBG_INVALID_ARG, "null pointer");
+  if (!m->mlp.master) return fail(EBG_NOT_READY, "no network");
+  EBG_HIP(ebgpu::mlp_get_params_host(m->mlp, params));
+  return EBG_OK;
+}
+
+ebg_status ebg_mlp_set_params(ebg_mesh* m, const float* params)
+{
+  if (!m || !params) return fail(EBG_INVALID_ARG, "null pointer");
+  for (int i = 0; i < ebgpu::kMlpParams; i++)
+    if (!std::isfinite(params[i])) return fail(EBG_INVALID_ARG, "non-finite parameter");
+  std::lock_guard<std::recursive_mutex> lock(m->mu);
+  mlp_set_box(m, m->mlp.box_scale > 0.0f ? m->mlp.box_scale : 1.2f);
+  EBG_HIP(ebgpu::mlp_set_params_host(m->mlp, params));
+  m->mlp.trained = true;
+  return EBG_OK;
+}
+
+ebg_status ebg_mlp_loss_grad(ebg_mesh* m, const float* d_pts, const float* d_target, uint64_t n, float* d_grad,
+                             float* d_loss, void* stream)
+{
+  if (!m || !d_pts || !d_target || !d_grad) return fail(EBG_INVALID_ARG, "null pointer");
+  if (n < 1 || n > 65536) return fail(EBG_INVALID_ARG, "1 <= n <= 65536");
+  if (!m->mlp.master) return fail(EBG_NOT_READY, "no network");
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  std::lock_guard<std::recursive_mutex> lock(m->mu);
+  EBG_HIP(m->train.ensure(int(n)));
+  EBG_HIP(hipMemsetAsync(m->train.loss, 0, sizeof(float), s));
+  EBG_HIP(ebgpu::mlp_loss_grad(m->mlp, m->train, d_pts, d_target, int(n), s));
+  EBG_HIP(ebgpu::mlp_grad_public(m->mlp, d_grad, s));
+  if (d_loss) EBG_HIP(hipMemcpyAsync(d_loss, m->train.loss, sizeof(float), hipMemcpyDeviceToDevice, s));
+  return EBG_OK;
+}
 
 } // extern "C"

@@ -1,40 +1,654 @@
-// Learned cull bound: device memory management and the forward pass.
+// Learned cull bound: fused MFMA inference, MFMA training GEMMs, Adam, sample generation.
+//
+// MFMA: v_mfma_f32_16x16x32_bf16 (gfx950). Operand layout per lane l (n = l % 16, g = l / 16):
+//   A (16 x 32, M x K): row n, k = 8g .. 8g+7      B (32 x 16, K x N): column n, k = 8g .. 8g+7
+//   D (16 x 16):        column n, rows 4g .. 4g+3
+// The fused kernel computes h^T = W^T x^T per layer so that queries run along N. Layer L's
+// accumulators hold, for query n, features 16t + 4g + e of output tile t; the next layer reads
+// its B operand straight from them by taking the K order
+//   phi(b, g, e) = 32b + (e < 4 ? 4g + e : 16 + 4g + e - 4)
+// and storing the next weight matrix's A fragments in that same K order (no data movement).
 #include "mlp.hpp"
+
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstring>
+#include <vector>
 
 namespace ebgpu {
 
+namespace {
+
+typedef __bf16   bf16x8 __attribute__((ext_vector_type(8)));
+typedef float    f32x4 __attribute__((ext_vector_type(4)));
+typedef uint16_t u16x8 __attribute__((ext_vector_type(8)));
+
+__device__ __forceinline__ uint16_t f2bf(float x)
+{
+  uint32_t u = __float_as_uint(x);
+  u += 0x7FFFu + ((u >> 16) & 1u); // round to nearest even (inputs are finite)
+  return (uint16_t)(u >> 16);
+}
+__device__ __forceinline__ float bf2f(uint16_t h) { return __uint_as_float((uint32_t)h << 16); }
+__device__ __forceinline__ f32x4 mfma(u16x8 a, u16x8 b, f32x4 c)
+{
+  return __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(bf16x8, a), __builtin_bit_cast(bf16x8, b), c, 0,
+                                                 0, 0);
+}
+
+// Fourier frequencies w_k = pi * 2^(0.4 k), k = 0..15 (float32-rounded, same table on the host)
+__constant__ float c_freq[kMlpFreqs];
+
+struct MlpDev
+{
+  float           lo[3], sc[3], diag;
+  const uint16_t* winfer;
+  const float*    master;
+};
+
+__device__ __forceinline__ float feature(int f, float x0, float x1, float x2)
+{
+  if (f >= kMlpIn) return 0.0f;
+  const int   c = f < 3 ? f : (f - 3) % 3;
+  const float x = c == 0 ? x0 : (c == 1 ? x1 : x2);
+  if (f < 3) return x;
+  return sinf(c_freq[(f - 3) / 3] * x);
+}
+
+// ------------------------------------------------------------------------------------------
+// fused inference: one wavefront = 64 queries (4 tiles of 16); 4 waves per workgroup
+// ------------------------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void infer_kernel(const float* __restrict__ pts, const uint32_t* __restrict__ idx,
+                                                     uint64_t n, MlpDev p, float scale, float offset, int seed_mode,
+                                                     float* __restrict__ out)
+{
+  const int      lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int      nn = lane & 15, g = lane >> 4;
+  const uint64_t base = ((uint64_t)blockIdx.x * 4 + wave) * 64;
+  if (base >= n) return;
+
+  u16x8    bq[4][4];
+  uint32_t qid[4];
+  bool     valid[4];
+#pragma unroll
+  for (int q = 0; q < 4; q++) {
+    const uint64_t i = base + q * 16 + nn;
+    valid[q]         = i < n;
+    qid[q]           = valid[q] ? (idx ? idx[i] : (uint32_t)i) : 0u;
+    float x[3]       = {0.0f, 0.0f, 0.0f};
+    if (valid[q])
+      for (int c = 0; c < 3; c++) x[c] = (pts[3 * (size_t)qid[q] + c] - p.lo[c]) * p.sc[c] - 1.0f;
+#pragma unroll
+    for (int b = 0; b < 2; b++)
+#pragma unroll
+      for (int e = 0; e < 8; e++) bq[q][b][e] = f2bf(feature(32 * b + 8 * g + e, x[0], x[1], x[2]));
+  }
+
+  f32x4 acc[4][8];
+#pragma unroll
+  for (int q = 0; q < 4; q++)
+#pragma unroll
+    for (int t = 0; t < 8; t++) acc[q][t] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
+  // layer 1 (K = 64)
+#pragma unroll
+  for (int t = 0; t < 8; t++)
+#pragma unroll
+    for (int b = 0; b < 2; b++) {
+      const u16x8 a = *reinterpret_cast<const u16x8*>(p.winfer + kBfW1 + (16 * t + nn) * 64 + 32 * b + 8 * g);
+#pragma unroll
+      for (int q = 0; q < 4; q++) acc[q][t] = mfma(a, bq[q][b], acc[q][t]);
+    }
+  // layers 2..4 (K = 128)
+  const int wofs[3] = {kBfW2, kBfW3, kBfW4};
+  const int bofs[3] = {kOffB1, kOffB2, kOffB3};
+#pragma unroll
+  for (int L = 0; L < 3; L++) {
+    const float* bias = p.master + bofs[L];
+#pragma unroll
+    for (int b = 0; b < 4; b++) {
+      float bl[8];
+#pragma unroll
+      for (int e = 0; e < 4; e++) {
+        bl[e]     = bias[32 * b + 4 * g + e];
+        bl[e + 4] = bias[32 * b + 16 + 4 * g + e];
+      }
+#pragma unroll
+      for (int q = 0; q < 4; q++)
+#pragma unroll
+        for (int e = 0; e < 8; e++) {
+          const float z = (e < 4 ? acc[q][2 * b][e] : acc[q][2 * b + 1][e - 4]) + bl[e];
+          bq[q][b][e]   = f2bf(fmaxf(z, 0.0f));
+        }
+    }
+#pragma unroll
+    for (int q = 0; q < 4; q++)
+#pragma unroll
+      for (int t = 0; t < 8; t++) acc[q][t] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll
+    for (int t = 0; t < 8; t++)
+#pragma unroll
+      for (int b = 0; b < 4; b++) {
+        const u16x8 a = *reinterpret_cast<const u16x8*>(p.winfer + wofs[L] + (16 * t + nn) * 128 + 32 * b + 8 * g);
+#pragma unroll
+        for (int q = 0; q < 4; q++) acc[q][t] = mfma(a, bq[q][b], acc[q][t]);
+      }
+  }
+  // output layer: y = relu(h4 + b4) . w5 + b5 (h4 rounded to bf16 as in training)
+  const float* b4 = p.master + kOffB4;
+  const float* w5 = p.master + kOffW5;
+  const float  b5 = p.master[kOffB5];
+#pragma unroll
+  for (int q = 0; q < 4; q++) {
+    float s = 0.0f;
+#pragma unroll
+    for (int t = 0; t < 8; t++)
+#pragma unroll
+      for (int e = 0; e < 4; e++) {
+        const int f = 16 * t + 4 * g + e;
+        s += bf2f(f2bf(fmaxf(acc[q][t][e] + b4[f], 0.0f))) * w5[f];
+      }
+    s += __shfl_xor(s, 16, 64);
+    s += __shfl_xor(s, 32, 64);
+    if (g == 0 && valid[q]) {
+      const float y = (expf(s + b5) - kMlpLogC) * p.diag;
+      out[qid[q]]   = seed_mode ? scale * fmaxf(y, 0.0f) + offset : y;
+    }
+  }
+}
+
+// ------------------------------------------------------------------------------------------
+// training: generic MFMA GEMM with strided operands and a fused epilogue
+// ------------------------------------------------------------------------------------------
+struct GemmArgs
+{
+  const uint16_t* A;
+  int64_t         as0, as1; // A(i,k) = A[i*as0 + k*as1]
+  const uint16_t* B;
+  int64_t         bs0, bs1; // B(k,j) = B[k*bs0 + j*bs1]
+  int             M, N, K;
+  const float*    bias;  // per column (nullable)
+  const uint16_t* mask;  // C(i,j) kept only where mask[i*ms0 + j] > 0 (nullable)
+  int64_t         ms0;
+  int             relu;
+  float*          Cf;    // nullable
+  uint16_t*       Cb;    // nullable
+  int64_t         ldc;
+};
+
+__global__ __launch_bounds__(64) void gemm_kernel(GemmArgs a)
+{
+  const int lane = threadIdx.x, nn = lane & 15, g = lane >> 4;
+  const int i0 = blockIdx.x * 16, j0 = blockIdx.y * 16;
+  f32x4     acc{0.0f, 0.0f, 0.0f, 0.0f};
+  for (int k0 = 0; k0 < a.K; k0 += 32) {
+    u16x8 va, vb;
+#pragma unroll
+    for (int e = 0; e < 8; e++) {
+      const int64_t k = k0 + 8 * g + e;
+      va[e]           = a.A[(int64_t)(i0 + nn) * a.as0 + k * a.as1];
+      vb[e]           = a.B[k * a.bs0 + (int64_t)(j0 + nn) * a.bs1];
+    }
+    acc = mfma(va, vb, acc);
+  }
+#pragma unroll
+  for (int e = 0; e < 4; e++) {
+    const int i = i0 + 4 * g + e, j = j0 + nn;
+    float     v = acc[e] + (a.bias ? a.bias[j] : 0.0f);
+    if (a.mask) {
+      const uint16_t h = a.mask[(int64_t)i * a.ms0 + j];
+      if (h == 0 || (h & 0x8000u)) v = 0.0f;
+    }
+    if (a.relu) v = fmaxf(v, 0.0f);
+    if (a.Cf) a.Cf[(int64_t)i * a.ldc + j] = v;
+    if (a.Cb) a.Cb[(int64_t)i * a.ldc + j] = f2bf(v);
+  }
+}
+
+hipError_t gemm(const GemmArgs& a, hipStream_t s)
+{
+  hipLaunchKernelGGL(gemm_kernel, dim3(a.M / 16, a.N / 16), dim3(64), 0, s, a);
+  return hipGetLastError();
+}
+
+__global__ __launch_bounds__(256) void features_kernel(const float* __restrict__ pts, int n, int rows, MlpDev p,
+                                                       uint16_t* __restrict__ x0)
+{
+  const int i = blockIdx.x * 256 + threadIdx.x;
+  if (i >= rows) return;
+  float x[3] = {0.0f, 0.0f, 0.0f};
+  if (i < n)
+    for (int c = 0; c < 3; c++) x[c] = (pts[3 * (size_t)i + c] - p.lo[c]) * p.sc[c] - 1.0f;
+  for (int f = 0; f < kMlpInPad; f++) x0[(size_t)i * kMlpInPad + f] = f2bf(feature(f, x[0], x[1], x[2]));
+}
+
+// Output layer, loss and its gradient. The network predicts y ~ log(|d|/diag + c); the loss is
+// an asymmetric squared error in that log space, e = y - log(|d|/diag + c), loss = w e^2 with
+// w = under_weight where e < 0 (a too-small cull radius forces the exact fallback) and 1
+// otherwise. The prediction is (exp(y) - c) * diag.
+__global__ __launch_bounds__(256) void out_fwd_bwd_kernel(const uint16_t* __restrict__ h4, const float* __restrict__ master,
+                                                          const float* __restrict__ target, int n, int rows,
+                                                          float inv_diag, float inv_n, float under_weight,
+                                                          float* __restrict__ gout, float* __restrict__ loss_sum,
+                                                          uint16_t* __restrict__ d4)
+{
+  const int i = blockIdx.x * 256 + threadIdx.x;
+  if (i >= rows) return;
+  const float* w5 = master + kOffW5;
+  float        gi = 0.0f, li = 0.0f;
+  if (i < n) {
+    float y = master[kOffB5];
+    for (int k = 0; k < kMlpHidden; k++) y += bf2f(h4[(size_t)i * kMlpHidden + k]) * w5[k];
+    const float lt = logf(fabsf(target[i]) * inv_diag + kMlpLogC);
+    const float e  = y - lt;
+    const float w  = e < 0.0f ? under_weight : 1.0f;
+    li             = w * e * e;
+    gi             = 2.0f * w * e * inv_n;
+  }
+  gout[i] = gi;
+  for (int k = 0; k < kMlpHidden; k++) {
+    const uint16_t h = h4[(size_t)i * kMlpHidden + k];
+    d4[(size_t)i * kMlpHidden + k] = f2bf((h == 0 || (h & 0x8000u)) ? 0.0f : gi * w5[k]);
+  }
+  // wave-reduce the loss then one atomic per wave
+  for (int o = 32; o > 0; o >>= 1) li += __shfl_xor(li, o, 64);
+  if ((threadIdx.x & 63) == 0) atomicAdd(loss_sum, li);
+}
+
+// dw5[k] = sum_i h4[i,k] g[i] (block k < 128), db5 = sum_i g[i] (block 128)
+__global__ __launch_bounds__(256) void out_wgrad_kernel(const uint16_t* __restrict__ h4, const float* __restrict__ gin,
+                                                        int rows, float* __restrict__ grad)
+{
+  __shared__ float sh[256];
+  const int        k = blockIdx.x;
+  float            s = 0.0f;
+  for (int i = threadIdx.x; i < rows; i += 256)
+    s += (k < kMlpHidden ? bf2f(h4[(size_t)i * kMlpHidden + k]) : 1.0f) * gin[i];
+  sh[threadIdx.x] = s;
+  __syncthreads();
+  for (int o = 128; o > 0; o >>= 1) {
+    if (threadIdx.x < (unsigned)o) sh[threadIdx.x] += sh[threadIdx.x + o];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) grad[k < kMlpHidden ? kOffW5 + k : kOffB5] = sh[0];
+}
+
+// db[j] = sum_i D[i, j] (one block per column)
+__global__ __launch_bounds__(256) void colsum_kernel(const uint16_t* __restrict__ d, int rows, int ld,
+                                                     float* __restrict__ out)
+{
+  __shared__ float sh[256];
+  const int        j = blockIdx.x;
+  float            s = 0.0f;
+  for (int i = threadIdx.x; i < rows; i += 256) s += bf2f(d[(size_t)i * ld + j]);
+  sh[threadIdx.x] = s;
+  __syncthreads();
+  for (int o = 128; o > 0; o >>= 1) {
+    if (threadIdx.x < (unsigned)o) sh[threadIdx.x] += sh[threadIdx.x + o];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) out[j] = sh[0];
+}
+
+__global__ __launch_bounds__(256) void adam_kernel(float* __restrict__ w, const float* __restrict__ g,
+                                                   float* __restrict__ m, float* __restrict__ v, int n, float lr,
+                                                   float bc1, float bc2)
+{
+  const int i = blockIdx.x * 256 + threadIdx.x;
+  if (i >= n) return;
+  const float gi = g[i];
+  const float mi = 0.9f * m[i] + 0.1f * gi;
+  const float vi = 0.999f * v[i] + 0.001f * gi * gi;
+  m[i]           = mi;
+  v[i]           = vi;
+  w[i] -= lr * (mi * bc1) / (sqrtf(vi * bc2) + 1e-8f);
+}
+
+// BF16 copies: row-major training copy and the permuted inference fragments.
+__global__ __launch_bounds__(256) void sync_bf16_kernel(const float* __restrict__ master, uint16_t* __restrict__ wtrain,
+                                                        uint16_t* __restrict__ winfer)
+{
+  const int i = blockIdx.x * 256 + threadIdx.x;
+  if (i >= kBfTotal) return;
+  // training copy: same [in][out] layout as the master
+  int src;
+  if (i < kBfW2) src = kOffW1 + i;
+  else if (i < kBfW3) src = kOffW2 + (i - kBfW2);
+  else if (i < kBfW4) src = kOffW3 + (i - kBfW3);
+  else src = kOffW4 + (i - kBfW4);
+  wtrain[i] = f2bf(master[src]);
+  // inference copy
+  if (i < kBfW2) {
+    const int r = i / kMlpInPad, c = i % kMlpInPad; // W1^T[out r][in c]
+    winfer[i] = f2bf(master[kOffW1 + c * kMlpHidden + r]);
+  }
+  else {
+    const int base = i < kBfW3 ? kBfW2 : (i < kBfW4 ? kBfW3 : kBfW4);
+    const int moff = i < kBfW3 ? kOffW2 : (i < kBfW4 ? kOffW3 : kOffW4);
+    const int j = i - base, r = j / kMlpHidden, c = j % kMlpHidden;
+    const int b = c / 32, g = (c % 32) / 8, e = c % 8;
+    const int k = 32 * b + (e < 4 ? 4 * g + e : 16 + 4 * g + (e - 4));
+    winfer[i] = f2bf(master[moff + k * kMlpHidden + r]);
+  }
+}
+
+__global__ void scale_kernel(float* p, int n, float s)
+{
+  const int i = blockIdx.x * 256 + threadIdx.x;
+  if (i < n) p[i] *= s;
+}
+
+// public order <-> master layout (W1 rows 51..63 are padding)
+__global__ void grad_public_kernel(const float* __restrict__ g, float* __restrict__ out)
+{
+  const int i = blockIdx.x * 256 + threadIdx.x;
+  if (i >= kMlpParams) return;
+  const int w1 = kMlpIn * kMlpHidden;
+  out[i]       = i < w1 ? g[i] : g[i + (kMlpInPad - kMlpIn) * kMlpHidden];
+}
+
+// ------------------------------------------------------------------------------------------
+// training samples
+// ------------------------------------------------------------------------------------------
+__device__ __forceinline__ uint64_t mix64(uint64_t z)
+{
+  z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+  z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+  return z ^ (z >> 31);
+}
+__device__ __forceinline__ double u01(uint64_t key, uint64_t ctr)
+{
+  return (double)(mix64(key + ctr * 0x9E3779B97F4A7C15ull) >> 11) * (1.0 / 9007199254740992.0);
+}
+
+__global__ __launch_bounds__(256) void samples_kernel(MeshDev m, const float* __restrict__ cdf, uint32_t nt, float lo0,
+                                                      float lo1, float lo2, float hi0, float hi1, float hi2,
+                                                      float box_scale, float sigma, uint64_t key, uint64_t n,
+                                                      float* __restrict__ out)
+{
+  const uint64_t i = (uint64_t)blockIdx.x * 256 + threadIdx.x;
+  if (i >= n) return;
+  const double lo[3] = {lo0, lo1, lo2}, hi[3] = {hi0, hi1, hi2};
+  double       p[3];
+  if ((i & 1u) == 0) { // even: uniform in the box; odd: near the surface (interleaved batches)
+    for (int c = 0; c < 3; c++) {
+      const double ctr = 0.5 * (lo[c] + hi[c]), h = 0.5 * box_scale * (hi[c] - lo[c]);
+      p[c]             = ctr + h * (2.0 * u01(key, 8 * i + c) - 1.0);
+    }
+  }
+  else {
+    const float u = (float)u01(key, 8 * i);
+    uint32_t    a = 0, b = nt - 1; // first t with cdf[t] > u
+    while (a < b) {
+      const uint32_t mid = (a + b) / 2;
+      if (cdf[mid] > u) b = mid;
+      else a = mid + 1;
+    }
+    const float*  x  = m.tripos + 12 * (size_t)a;
+    const double* fn = m.fn64 + 4 * (size_t)a;
+    const double  r1 = sqrt(u01(key, 8 * i + 1)), r2 = u01(key, 8 * i + 2);
+    const double  u1 = u01(key, 8 * i + 3) + 1e-300, u2 = u01(key, 8 * i + 4);
+    const double  z  = sqrt(-2.0 * log(u1)) * cos(6.283185307179586 * u2);
+    for (int c = 0; c < 3; c++)
+      p[c] = (1.0 - r1) * x[c] + r1 * (1.0 - r2) * x[3 + c] + r1 * r2 * x[6 + c] + sigma * z * fn[c];
+  }
+  for (int c = 0; c < 3; c++) out[3 * i + c] = (float)p[c];
+}
+
+MlpDev dev_of(const Mlp& m)
+{
+  MlpDev d;
+  for (int c = 0; c < 3; c++) {
+    d.lo[c] = m.lo[c];
+    d.sc[c] = m.scale[c];
+  }
+  d.diag   = m.diag;
+  d.winfer = m.winfer;
+  d.master = m.master;
+  return d;
+}
+
+hipError_t upload_freqs()
+{
+  static bool done = false;
+  if (done) return hipSuccess;
+  float f[kMlpFreqs];
+  for (int k = 0; k < kMlpFreqs; k++) f[k] = (float)(M_PI * std::pow(2.0, 0.4 * k));
+  hipError_t e = hipMemcpyToSymbol(HIP_SYMBOL(c_freq), f, sizeof(f));
+  if (e == hipSuccess) done = true;
+  return e;
+}
+
+} // namespace
+
 Mlp::~Mlp()
 {
-  if (master) (void)hipFree(master);
-  if (wbf16) (void)hipFree(wbf16);
-  if (adam_m) (void)hipFree(adam_m);
-  if (adam_v) (void)hipFree(adam_v);
-  if (grad) (void)hipFree(grad);
+  for (void* p : {(void*)master, (void*)grad, (void*)adam_m, (void*)adam_v, (void*)wtrain, (void*)winfer})
+    if (p) (void)hipFree(p);
+}
+
+MlpTrainBuffers::~MlpTrainBuffers()
+{
+  for (void* p : {(void*)x0, (void*)h[0], (void*)h[1], (void*)h[2], (void*)h[3], (void*)dl[0], (void*)dl[1],
+                  (void*)dl[2], (void*)dl[3], (void*)g, (void*)loss})
+    if (p) (void)hipFree(p);
+}
+
+hipError_t MlpTrainBuffers::ensure(int b)
+{
+  b = (b + 31) / 32 * 32;
+  if (b <= batch) return hipSuccess;
+  this->~MlpTrainBuffers();
+  std::memset(static_cast<void*>(this), 0, sizeof(*this));
+  hipError_t e;
+  if ((e = hipMalloc(&x0, sizeof(uint16_t) * b * kMlpInPad)) != hipSuccess) return e;
+  for (int i = 0; i < 4; i++) {
+    if ((e = hipMalloc(&h[i], sizeof(uint16_t) * b * kMlpHidden)) != hipSuccess) return e;
+    if ((e = hipMalloc(&dl[i], sizeof(uint16_t) * b * kMlpHidden)) != hipSuccess) return e;
+    if ((e = hipMemset(h[i], 0, sizeof(uint16_t) * b * kMlpHidden)) != hipSuccess) return e;
+  }
+  if ((e = hipMemset(x0, 0, sizeof(uint16_t) * b * kMlpInPad)) != hipSuccess) return e;
+  if ((e = hipMalloc(&g, sizeof(float) * b)) != hipSuccess) return e;
+  if ((e = hipMalloc(&loss, sizeof(float) * 4)) != hipSuccess) return e;
+  batch = b;
+  return hipSuccess;
 }
 
 hipError_t mlp_alloc(Mlp& m)
 {
+  hipError_t e = upload_freqs();
+  if (e != hipSuccess) return e;
   if (m.master) return hipSuccess;
-  hipError_t e;
-  if ((e = hipMalloc(&m.master, sizeof(float) * kMlpParams)) != hipSuccess) return e;
-  if ((e = hipMalloc(&m.wbf16, sizeof(uint16_t) * kMlpParams)) != hipSuccess) return e;
-  if ((e = hipMalloc(&m.adam_m, sizeof(float) * kMlpParams)) != hipSuccess) return e;
-  if ((e = hipMalloc(&m.adam_v, sizeof(float) * kMlpParams)) != hipSuccess) return e;
-  if ((e = hipMalloc(&m.grad, sizeof(float) * kMlpParams)) != hipSuccess) return e;
+  if ((e = hipMalloc(&m.master, sizeof(float) * kMaster)) != hipSuccess) return e;
+  if ((e = hipMalloc(&m.grad, sizeof(float) * kMaster)) != hipSuccess) return e;
+  if ((e = hipMalloc(&m.adam_m, sizeof(float) * kMaster)) != hipSuccess) return e;
+  if ((e = hipMalloc(&m.adam_v, sizeof(float) * kMaster)) != hipSuccess) return e;
+  if ((e = hipMalloc(&m.wtrain, sizeof(uint16_t) * kBfTotal)) != hipSuccess) return e;
+  if ((e = hipMalloc(&m.winfer, sizeof(uint16_t) * kBfTotal)) != hipSuccess) return e;
+  if ((e = hipMemset(m.grad, 0, sizeof(float) * kMaster)) != hipSuccess) return e;
   return hipSuccess;
 }
 
-hipError_t mlp_sync_bf16(Mlp&, hipStream_t) { return hipErrorNotSupported; }
-
-hipError_t mlp_forward(const Mlp&, const float*, const uint32_t*, uint64_t, float, float, float*, hipStream_t)
+static void public_to_master(const float* pub, std::vector<float>& mst)
 {
-  return hipErrorNotSupported;
+  mst.assign(kMaster, 0.0f);
+  const int w1 = kMlpIn * kMlpHidden;
+  for (int i = 0; i < kMlpParams; i++) mst[i < w1 ? i : i + (kMlpInPad - kMlpIn) * kMlpHidden] = pub[i];
 }
 
-hipError_t mlp_seed_radius(const Mlp&, const MeshDev&, const float*, const uint32_t*, uint64_t, float, float, float*,
-                           hipStream_t)
+hipError_t mlp_set_params_host(Mlp& m, const float* params)
 {
-  return hipErrorNotSupported;
+  hipError_t e = mlp_alloc(m);
+  if (e != hipSuccess) return e;
+  std::vector<float> mst;
+  public_to_master(params, mst);
+  if ((e = hipMemcpy(m.master, mst.data(), sizeof(float) * kMaster, hipMemcpyHostToDevice)) != hipSuccess) return e;
+  if ((e = hipMemset(m.adam_m, 0, sizeof(float) * kMaster)) != hipSuccess) return e;
+  if ((e = hipMemset(m.adam_v, 0, sizeof(float) * kMaster)) != hipSuccess) return e;
+  m.step = 0;
+  if ((e = mlp_sync_bf16(m, nullptr)) != hipSuccess) return e;
+  return hipDeviceSynchronize();
+}
+
+hipError_t mlp_get_params_host(const Mlp& m, float* params)
+{
+  if (!m.master) return hipErrorInvalidValue;
+  std::vector<float> mst(kMaster);
+  hipError_t         e = hipMemcpy(mst.data(), m.master, sizeof(float) * kMaster, hipMemcpyDeviceToHost);
+  if (e != hipSuccess) return e;
+  const int w1 = kMlpIn * kMlpHidden;
+  for (int i = 0; i < kMlpParams; i++) params[i] = mst[i < w1 ? i : i + (kMlpInPad - kMlpIn) * kMlpHidden];
+  return hipSuccess;
+}
+
+hipError_t mlp_init(Mlp& m, uint64_t seed)
+{
+  // He-uniform weights from SplitMix64, zero biases
+  std::vector<float> pub(kMlpParams, 0.0f);
+  uint64_t           s    = seed ? seed : 1;
+  auto               next = [&s]() {
+    s += 0x9E3779B97F4A7C15ull;
+    uint64_t z = s;
+    z          = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z          = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    z ^= z >> 31;
+    return (double)(z >> 11) * (1.0 / 9007199254740992.0);
+  };
+  int o = 0;
+  auto fill = [&](int fan_in, int count) {
+    const double a = std::sqrt(6.0 / fan_in);
+    for (int i = 0; i < count; i++) pub[o + i] = float((2.0 * next() - 1.0) * a);
+    o += count;
+  };
+  fill(kMlpIn, kMlpIn * kMlpHidden);
+  o += kMlpHidden;
+  for (int l = 0; l < 3; l++) {
+    fill(kMlpHidden, kMlpHidden * kMlpHidden);
+    o += kMlpHidden;
+  }
+  // small output layer: the first predictions sit near log(0.05) for every point
+  const double a5 = 0.01 * std::sqrt(6.0 / kMlpHidden);
+  for (int i = 0; i < kMlpHidden; i++) pub[o + i] = float((2.0 * next() - 1.0) * a5);
+  o += kMlpHidden;
+  pub[o] = float(std::log(0.05));
+  return mlp_set_params_host(m, pub.data());
+}
+
+hipError_t mlp_sync_bf16(Mlp& m, hipStream_t s)
+{
+  hipLaunchKernelGGL(sync_bf16_kernel, dim3((kBfTotal + 255) / 256), dim3(256), 0, s, m.master, m.wtrain, m.winfer);
+  return hipGetLastError();
+}
+
+hipError_t mlp_forward(const Mlp& m, const float* pts, const uint32_t* idx, uint64_t n, bool seed_mode, float scale,
+                       float offset, float* out, hipStream_t s)
+{
+  if (n == 0) return hipSuccess;
+  const uint64_t blocks = (n + 255) / 256;
+  hipLaunchKernelGGL(infer_kernel, dim3((uint32_t)blocks), dim3(256), 0, s, pts, idx, n, dev_of(m), scale, offset,
+                     seed_mode ? 1 : 0, out);
+  return hipGetLastError();
+}
+
+hipError_t mlp_loss_grad(Mlp& m, MlpTrainBuffers& tb, const float* pts, const float* target, int n, hipStream_t s)
+{
+  hipError_t e = tb.ensure(n);
+  if (e != hipSuccess) return e;
+  const int    rows = (n + 31) / 32 * 32;
+  const MlpDev d    = dev_of(m);
+  hipLaunchKernelGGL(features_kernel, dim3((rows + 255) / 256), dim3(256), 0, s, pts, n, rows, d, tb.x0);
+  // forward
+  const uint16_t* in    = tb.x0;
+  int             in_ld = kMlpInPad;
+  const int       wb[4] = {kBfW1, kBfW2, kBfW3, kBfW4};
+  const int       bo[4] = {kOffB1, kOffB2, kOffB3, kOffB4};
+  const int       wo[4] = {kOffW1, kOffW2, kOffW3, kOffW4};
+  for (int L = 0; L < 4; L++) {
+    GemmArgs a{};
+    a.A    = in;
+    a.as0  = in_ld;
+    a.as1  = 1;
+    a.B    = m.wtrain + wb[L];
+    a.bs0  = kMlpHidden;
+    a.bs1  = 1;
+    a.M    = rows;
+    a.N    = kMlpHidden;
+    a.K    = in_ld;
+    a.bias = m.master + bo[L];
+    a.relu = 1;
+    a.Cb   = tb.h[L];
+    a.ldc  = kMlpHidden;
+    if ((e = gemm(a, s)) != hipSuccess) return e;
+    in    = tb.h[L];
+    in_ld = kMlpHidden;
+  }
+  // loss (accumulated into tb.loss[0]; the caller clears it) + output layer backward
+  hipLaunchKernelGGL(out_fwd_bwd_kernel, dim3((rows + 255) / 256), dim3(256), 0, s, tb.h[3], m.master, target, n,
+                     rows, 1.0f / m.diag, 1.0f / (float)n, m.under_weight, tb.g, tb.loss, tb.dl[3]);
+  hipLaunchKernelGGL(out_wgrad_kernel, dim3(kMlpHidden + 1), dim3(256), 0, s, tb.h[3], tb.g, rows, m.grad);
+  // hidden layers, last to first
+  for (int L = 3; L >= 0; L--) {
+    const uint16_t* hin    = L > 0 ? tb.h[L - 1] : tb.x0;
+    const int       hin_ld = L > 0 ? kMlpHidden : kMlpInPad;
+    GemmArgs        w{};
+    w.A   = hin;
+    w.as0 = 1;
+    w.as1 = hin_ld;
+    w.B   = tb.dl[L];
+    w.bs0 = kMlpHidden;
+    w.bs1 = 1;
+    w.M   = hin_ld;
+    w.N   = kMlpHidden;
+    w.K   = rows;
+    w.Cf  = m.grad + wo[L];
+    w.ldc = kMlpHidden;
+    if ((e = gemm(w, s)) != hipSuccess) return e;
+    hipLaunchKernelGGL(colsum_kernel, dim3(kMlpHidden), dim3(256), 0, s, tb.dl[L], rows, kMlpHidden, m.grad + bo[L]);
+    if (L > 0) {
+      GemmArgs p{};
+      p.A    = tb.dl[L];
+      p.as0  = kMlpHidden;
+      p.as1  = 1;
+      p.B    = m.wtrain + wb[L];
+      p.bs0  = 1;
+      p.bs1  = kMlpHidden;
+      p.M    = rows;
+      p.N    = kMlpHidden;
+      p.K    = kMlpHidden;
+      p.mask = tb.h[L - 1];
+      p.ms0  = kMlpHidden;
+      p.Cb   = tb.dl[L - 1];
+      p.ldc  = kMlpHidden;
+      if ((e = gemm(p, s)) != hipSuccess) return e;
+    }
+  }
+  return hipGetLastError();
+}
+
+hipError_t mlp_adam(Mlp& m, float lr, hipStream_t s)
+{
+  m.step++;
+  const float bc1 = 1.0f / (1.0f - std::pow(0.9f, (float)m.step));
+  const float bc2 = 1.0f / (1.0f - std::pow(0.999f, (float)m.step));
+  hipLaunchKernelGGL(adam_kernel, dim3((kMaster + 255) / 256), dim3(256), 0, s, m.master, m.grad, m.adam_m, m.adam_v,
+                     kMaster, lr, bc1, bc2);
+  return mlp_sync_bf16(m, s);
+}
+
+hipError_t mlp_grad_public(const Mlp& m, float* out, hipStream_t s)
+{
+  hipLaunchKernelGGL(grad_public_kernel, dim3((kMlpParams + 255) / 256), dim3(256), 0, s, m.grad, out);
+  return hipGetLastError();
+}
+
+hipError_t mlp_samples(const MeshDev& dev, const float* cdf, uint32_t nt, const float* lo, const float* hi,
+                       float box_scale, float sigma, uint64_t seed, uint64_t n, float* pts, hipStream_t s)
+{
+  if (n == 0) return hipSuccess;
+  uint64_t key = seed * 0x9E3779B97F4A7C15ull + 0x632BE59BD9B4E019ull;
+  hipLaunchKernelGGL(samples_kernel, dim3((uint32_t)((n + 255) / 256)), dim3(256), 0, s, dev, cdf, nt, lo[0], lo[1],
+                     lo[2], hi[0], hi[1], hi[2], box_scale, sigma, key, n, pts);
+  return hipGetLastError();
 }
 
 } // namespace ebgpu

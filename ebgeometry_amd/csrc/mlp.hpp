@@ -1,10 +1,17 @@
 // Learned cull bound: a Fourier-feature MLP that predicts the unsigned distance to the mesh.
 //
-// Network: x_hat = 2 (p - lo) / (hi - lo) - 1 (bbox-normalised); features
-// [x_hat, sin(2^k pi x_hat_j) for k = 0..15, j = 0..2] (51 values, zero-padded to 64);
-// 4 hidden layers of 128 with ReLU; scalar output (softplus-free: ReLU on the output so the
-// prediction is a non-negative distance). BF16 weights and activations on MFMA with FP32
-// accumulation; FP32 master weights and Adam moments.
+// Network (56,321 parameters):
+//   x_hat = (p - lo) * scale - 1                 (lo/scale from the training box)
+//   features[51] = [x_hat_0..2, sin(w_k x_hat_c) for k = 0..15, c = 0..2],
+//                  w_k = pi * 2^(0.4 k)  (pi .. 64 pi), zero-padded to 64
+//   h1 = relu(features W1 + b1)   W1: 51 x 128
+//   h2..h4 = relu(h W + b)        128 x 128 each
+//   y = h4 . w5 + b5              prediction of |d| / diag
+// Weights and activations are BF16 on MFMA (v_mfma_f32_16x16x32_bf16) with FP32
+// accumulation; master weights and Adam moments are FP32.
+//
+// Public parameter order (ebg_mlp_get_params): W1 (51x128 row-major, [in][out]), b1, W2, b2,
+// W3, b3, W4, b4, w5 (128), b5 (1).
 #pragma once
 
 #include "ebgpu_internal.hpp"
@@ -18,35 +25,89 @@ namespace ebgpu {
 constexpr int kMlpIn     = 51; // 3 + 3*16
 constexpr int kMlpInPad  = 64;
 constexpr int kMlpHidden = 128;
-constexpr int kMlpLayers = 4; // hidden layers
+constexpr int kMlpFreqs  = 16;
+constexpr float kMlpLogC = 0.005f; // y ~ log(|d|/diag + kMlpLogC)
 
-// Parameter counts: W1 64x128 (rows >= 51 are zero) + b1, W2..W4 128x128 + b, w5 128 + b5.
-constexpr int kMlpW1     = kMlpInPad * kMlpHidden;
-constexpr int kMlpWh     = kMlpHidden * kMlpHidden;
-constexpr int kMlpParams = kMlpW1 + kMlpHidden + 3 * (kMlpWh + kMlpHidden) + kMlpHidden + 1;
+constexpr int kMlpParams = kMlpIn * kMlpHidden + kMlpHidden + 3 * (kMlpHidden * kMlpHidden + kMlpHidden) +
+                           kMlpHidden + 1; // 56,321
+
+// Device-side FP32 master layout (W1 padded to 64 input rows).
+constexpr int kOffW1 = 0;
+constexpr int kOffB1 = kOffW1 + kMlpInPad * kMlpHidden;
+constexpr int kOffW2 = kOffB1 + kMlpHidden;
+constexpr int kOffB2 = kOffW2 + kMlpHidden * kMlpHidden;
+constexpr int kOffW3 = kOffB2 + kMlpHidden;
+constexpr int kOffB3 = kOffW3 + kMlpHidden * kMlpHidden;
+constexpr int kOffW4 = kOffB3 + kMlpHidden;
+constexpr int kOffB4 = kOffW4 + kMlpHidden * kMlpHidden;
+constexpr int kOffW5 = kOffB4 + kMlpHidden;
+constexpr int kOffB5 = kOffW5 + kMlpHidden;
+constexpr int kMaster = kOffB5 + 1; // padded parameter count
+
+// BF16 copies: row-major [in][out] for the training GEMMs (W1..W4), and the A-operand layout
+// of the fused inference kernel (W^T with the K order permuted to match the accumulator layout).
+constexpr int kBfW1 = 0;
+constexpr int kBfW2 = kBfW1 + kMlpInPad * kMlpHidden;
+constexpr int kBfW3 = kBfW2 + kMlpHidden * kMlpHidden;
+constexpr int kBfW4 = kBfW3 + kMlpHidden * kMlpHidden;
+constexpr int kBfTotal = kBfW4 + kMlpHidden * kMlpHidden;
 
 struct Mlp
 {
-  bool     trained = false;
-  float    lo[3]{}, scale[3]{}; // x_hat = (p - lo) * scale - 1
-  float*   master  = nullptr;   // kMlpParams FP32 (layout: W1,b1,W2,b2,W3,b3,W4,b4,w5,b5; W row-major [in][out])
-  uint16_t* wbf16  = nullptr;   // BF16 copy of the weight matrices in the MFMA fragment layout
-  float*   adam_m  = nullptr;
-  float*   adam_v  = nullptr;
-  float*   grad    = nullptr;
+  bool      trained = false;
+  float     lo[3]{}, scale[3]{}; // x_hat = (p - lo) * scale - 1
+  float     diag    = 1.0f;      // output unit
+  float     box_scale = 0.0f;    // training box = box_scale x mesh bbox (0: not set)
+  float     under_weight = 8.0f; // loss weight of under-predictions
+  float*    master  = nullptr;   // kMaster FP32
+  float*    grad    = nullptr;   // kMaster FP32
+  float*    adam_m  = nullptr;
+  float*    adam_v  = nullptr;
+  uint16_t* wtrain  = nullptr;   // kBfTotal BF16, row-major [in][out]
+  uint16_t* winfer  = nullptr;   // kBfTotal BF16, inference fragment layout
+  int       step    = 0;
   ~Mlp();
 };
 
-// Allocate (if needed) device memory of an Mlp.
 hipError_t mlp_alloc(Mlp& m);
-// Refresh the BF16 weight copy from the FP32 master weights.
+// Upload parameters given in the public order (kMlpParams floats, host pointer).
+hipError_t mlp_set_params_host(Mlp& m, const float* params);
+hipError_t mlp_get_params_host(const Mlp& m, float* params);
+// Deterministic He initialisation from a seed.
+hipError_t mlp_init(Mlp& m, uint64_t seed);
+// Refresh both BF16 copies from the FP32 master weights.
 hipError_t mlp_sync_bf16(Mlp& m, hipStream_t s);
-// Forward pass: pred[i] = network(pts[idx ? idx[i] : i]) for i < n, written at position
-// (idx ? idx[i] : i). With seed mode, writes scale*pred + offset instead.
-hipError_t mlp_forward(const Mlp& m, const float* pts, const uint32_t* idx, uint64_t n, float scale, float offset,
-                       float* out, hipStream_t s);
-// Seeded cull radius for the traversal: scale * prediction + offset, indexed by query.
-hipError_t mlp_seed_radius(const Mlp& m, const MeshDev& dev, const float* pts, const uint32_t* idx, uint64_t n,
-                           float scale, float offset, float* seeds, hipStream_t s);
+
+// Fused forward pass over queries q = idx ? idx[i] : i (i < n), written at out[q]:
+// the prediction y(q) * diag, or with seed_mode the cull radius scale * max(y, 0) * diag + offset.
+hipError_t mlp_forward(const Mlp& m, const float* pts, const uint32_t* idx, uint64_t n, bool seed_mode, float scale,
+                       float offset, float* out, hipStream_t s);
+
+// Training state for one batch size.
+struct MlpTrainBuffers
+{
+  int       batch = 0;
+  uint16_t* x0    = nullptr; // B x 64
+  uint16_t* h[4]  = {};      // B x 128 each
+  uint16_t* dl[4] = {};      // B x 128 each (deltas)
+  float*    g     = nullptr; // B
+  float*    loss  = nullptr; // 1
+  ~MlpTrainBuffers();
+  hipError_t ensure(int b);
+};
+
+// Loss and full parameter gradient (into m.grad, master layout) for n <= buffers.batch points
+// with targets |d| (physical units). The summed loss is added to tb.loss[0].
+hipError_t mlp_loss_grad(Mlp& m, MlpTrainBuffers& tb, const float* pts, const float* target, int n,
+                         hipStream_t s);
+// One Adam step on m (uses m.grad), then refresh the BF16 copies.
+hipError_t mlp_adam(Mlp& m, float lr, hipStream_t s);
+// Copy the gradient out in the public parameter order (device pointer, kMlpParams floats).
+hipError_t mlp_grad_public(const Mlp& m, float* out, hipStream_t s);
+
+// Training samples: n points, first half uniform in the box (center +- box_scale * half
+// extent), second half area-uniform on the surface pushed along the face normal by N(0, sigma).
+hipError_t mlp_samples(const MeshDev& dev, const float* area_cdf, uint32_t num_tris, const float* lo, const float* hi,
+                       float box_scale, float sigma, uint64_t seed, uint64_t n, float* pts, hipStream_t s);
 
 } // namespace ebgpu

@@ -336,7 +336,7 @@ __device__ __forceinline__ void wave_append(bool pred, uint32_t idx, uint32_t* l
 // R = float: FP32 pass on TriRec records. R = double: FP64 recheck on raw positions.
 template <int K, int NB, class R, bool COUNT>
 __device__ __forceinline__ void traverse(const MeshDev& m, R px, R py, R pz, R bound, R& best, uint32_t& btri,
-                                         uint32_t& bfeat, uint32_t& n_inner, uint32_t& n_tris)
+                                         uint32_t& bfeat, uint32_t& n_inner, uint32_t& n_tris, uint32_t& w_inner, uint32_t& w_tris)
 {
   const Node<K>* __restrict__ nodes = static_cast<const Node<K>*>(m.nodes);
   LaneStack<NB>               st;
@@ -356,7 +356,7 @@ __device__ __forceinline__ void traverse(const MeshDev& m, R px, R py, R pz, R b
       // interior node: test the K children
       const uint32_t ni = uni(e.ref);
       const Node<K>& nd = nodes[ni];
-      if (COUNT) n_inner += need ? 1u : 0u;
+      if (COUNT) { n_inner += need ? 1u : 0u; w_inner++; }
       uint32_t key[K];
       uint64_t msk[K];
 #pragma unroll
@@ -390,7 +390,7 @@ __device__ __forceinline__ void traverse(const MeshDev& m, R px, R py, R pz, R b
     else {
       // leaf: every lane tests every triangle (keeping the minimum is exact for all lanes)
       const uint32_t first = uni(e.ref), cnt = uni(e.cnt);
-      if (COUNT) n_tris += need ? cnt : 0u;
+      if (COUNT) { n_tris += need ? cnt : 0u; w_tris += cnt; }
       for (uint32_t t = first; t < first + cnt; t++) {
         R        d2;
         uint32_t f;
@@ -435,9 +435,9 @@ __global__ __launch_bounds__(256) void sdf32_kernel(const MeshDev m, const Query
     }
   }
   float    best = INFINITY;
-  uint32_t btri = kInvalid, bfeat = 0, n_inner = 0, n_tris = 0;
+  uint32_t btri = kInvalid, bfeat = 0, n_inner = 0, n_tris = 0, w_inner = 0, w_tris = 0;
   const float seed2 = bound;
-  traverse<K, NB, float, COUNT>(m, px, py, pz, bound, best, btri, bfeat, n_inner, n_tris);
+  traverse<K, NB, float, COUNT>(m, px, py, pz, bound, best, btri, bfeat, n_inner, n_tris, w_inner, w_tris);
 
   // Exact only if some triangle lies within the seeded radius: otherwise nodes between the
   // seeded radius and the true distance were pruned, and the query must be redone unseeded.
@@ -456,6 +456,8 @@ __global__ __launch_bounds__(256) void sdf32_kernel(const MeshDev m, const Query
     if (__lane_id() == 0) {
       atomicAdd(a.work, (unsigned long long)si);
       atomicAdd(a.work + 1, (unsigned long long)st);
+      atomicAdd(a.work + 4, (unsigned long long)w_inner);
+      atomicAdd(a.work + 5, (unsigned long long)w_tris);
     }
   }
 }
@@ -483,9 +485,9 @@ __global__ __launch_bounds__(256) void sdf64_kernel(const MeshDev m, const Query
     }
   }
   double   best = INFINITY;
-  uint32_t btri = kInvalid, bfeat = 0, n_inner = 0, n_tris = 0;
+  uint32_t btri = kInvalid, bfeat = 0, n_inner = 0, n_tris = 0, w_inner = 0, w_tris = 0;
   const double seed2 = bound;
-  traverse<K, NB, double, false>(m, px, py, pz, bound, best, btri, bfeat, n_inner, n_tris);
+  traverse<K, NB, double, false>(m, px, py, pz, bound, best, btri, bfeat, n_inner, n_tris, w_inner, w_tris);
   const bool found = active && btri != kInvalid && best <= seed2;
   if (found) {
     a.out_d[qi] = (float)resolve64(m, btri, bfeat, best, px, py, pz);

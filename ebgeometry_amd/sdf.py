@@ -98,7 +98,7 @@ class TriMeshSDF(SignedDistanceFunction):
 
     @staticmethod
     def query_opts(presort=True, fp64_recheck=True, recheck_eps=1e-4, learned_bound=False, fp64_all=False,
-                   bound_scale=1.25, bound_offset=1e-3, stream=None, work=None) -> QueryOpts:
+                   bound_scale=1.25, bound_offset=1e-3, stream=None, work=None, seed_radius=None) -> QueryOpts:
         o = QueryOpts()
         lib().ebg_query_opts_default(C.byref(o))
         o.presort = int(presort)
@@ -110,6 +110,7 @@ class TriMeshSDF(SignedDistanceFunction):
         o.bound_offset = float(bound_offset)
         o.stream = stream
         o.work = work
+        o.seed_radius = seed_radius
         return o
 
     # -- queries --------------------------------------------------------------------------
@@ -163,12 +164,13 @@ class TriMeshSDF(SignedDistanceFunction):
     # -- learned bound --------------------------------------------------------------------
 
     def train_bound(self, num_samples: int = 4_000_000, epochs: int = 8, batch: int = 4096, lr: float = 2e-3,
-                    seed: int = 12345, near_sigma: float = 0.02, box_scale: float = 1.2, stream=None) -> float:
+                    seed: int = 12345, near_sigma: float = 0.02, box_scale: float = 1.2, under_weight: float = 8.0,
+                    stream=None) -> float:
         """Train the mesh's learned cull bound on the GPU; returns the final mean loss."""
         o = _lib.MlpOpts()
         lib().ebg_mlp_opts_default(C.byref(o))
         o.num_samples, o.epochs, o.batch, o.lr, o.seed = num_samples, epochs, batch, lr, seed
-        o.near_sigma, o.box_scale, o.stream = near_sigma, box_scale, stream
+        o.near_sigma, o.box_scale, o.stream, o.under_weight = near_sigma, box_scale, stream, under_weight
         loss = C.c_float(0.0)
         check(lib().ebg_mlp_train(self._h, C.byref(o), C.byref(loss)))
         return float(loss.value)

@@ -113,10 +113,17 @@ typedef struct ebg_query_opts
   float    bound_scale;    /* radius = bound_scale*pred + bound_offset*diag (1.25, 1e-3)    */
   float    bound_offset;
   void*    stream;         /* hipStream_t                                                   */
-  uint64_t* work;          /* optional device array of 4 uint64 accumulated per call:
+  uint64_t* work;          /* optional device array of 8 uint64 accumulated per call:
                               [0] interior nodes whose children each query tested,
-                              [1] real triangles tested, [2] queries rechecked in FP64,
-                              [3] queries that fell back to the full traversal              */
+                              [1] real triangles tested by each query,
+                              [2] queries rechecked in FP64,
+                              [3] queries that fell back to the full traversal,
+                              [4] interior node records fetched per wavefront (all waves),
+                              [5] triangle records fetched per wavefront, [6..7] reserved.
+                              Counting uses a slower kernel variant; time without it.       */
+  const float* seed_radius; /* optional device array (indexed by query): a caller-supplied
+                              cull radius per query, used like the learned bound (queries
+                              with no triangle inside it are redone unseeded). NULL = none. */
 } ebg_query_opts;
 
 /* Fill *opts with the defaults above. */
@@ -161,6 +168,9 @@ typedef struct ebg_mlp_opts
   float    near_sigma;   /* surface offset sigma for half the samples (default 0.02)         */
   float    box_scale;    /* uniform half sampled in box_scale x bbox (default 1.2)            */
   void*    stream;
+  float    under_weight; /* loss weight of under-predictions (default 8): higher = fewer
+                            fallbacks, looser radii                                          */
+  int32_t  reserved;
 } ebg_mlp_opts;
 
 void       ebg_mlp_opts_default(ebg_mlp_opts* opts);
@@ -170,10 +180,20 @@ ebg_status ebg_mlp_train(ebg_mesh* mesh, const ebg_mlp_opts* opts, float* loss);
 /* Forward pass only: predicted unsigned distance for n device points. */
 ebg_status ebg_mlp_predict(const ebg_mesh* mesh, const float* d_pts, uint64_t n, float* d_pred,
                            void* stream);
-/* Parameters as float32 in a fixed order (see ebg_mlp_param_count); host pointers. */
+/* Parameters as float32 in a fixed order (see ebg_mlp_param_count); host pointers:
+ * W1 (51x128, [in][out] row-major), b1 (128), W2, b2, W3, b3, W4, b4 (128x128 / 128 each),
+ * w5 (128), b5 (1). set_params also arms the learned bound (it counts as trained). */
 uint64_t   ebg_mlp_param_count(void);
 ebg_status ebg_mlp_get_params(const ebg_mesh* mesh, float* params);
 ebg_status ebg_mlp_set_params(ebg_mesh* mesh, const float* params);
+/* Training loss summed over n <= 65536 device points with device targets |d| (d_loss, one
+ * float, nullable) and its gradient (d_grad, ebg_mlp_param_count floats, parameter order
+ * above). The loss is the one the trainer minimises: the network output y estimates
+ * log(|d|/diag + 0.005); e = y - log(|d|/diag + 0.005), loss = w e^2 with w = under_weight
+ * (last trained value, default 8) where e < 0 and 1 elsewhere; the gradient is that of the mean
+ * over the batch. For checking the backward kernels. */
+ebg_status ebg_mlp_loss_grad(ebg_mesh* mesh, const float* d_pts, const float* d_target, uint64_t n,
+                             float* d_grad, float* d_loss, void* stream);
 
 #ifdef __cplusplus
 }

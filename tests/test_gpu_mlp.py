@@ -1,0 +1,151 @@
+"""GPU learned bound: MFMA forward/backward against the CPU oracle, training, exactness."""
+import ctypes as C
+
+import numpy as np
+import pytest
+
+from ebgeometry_amd import TriMeshSDF
+from ebgeometry_amd import workloads as wl
+from ebgeometry_amd._lib import check, lib
+from oracle import mlp_ref, port
+
+pytestmark = pytest.mark.gpu
+
+
+def norm_box(mesh, box_scale=1.2):
+    info = mesh.info()
+    lo, hi = np.array(info["bbox_lo"]), np.array(info["bbox_hi"])
+    ctr, h = 0.5 * (lo + hi), 0.5 * box_scale * (hi - lo)
+    return (ctr - h).astype(np.float32), (1.0 / h).astype(np.float32), np.float32(info["diag"])
+
+
+def small_theta(seed):
+    s = wl.Stream(seed, "theta-gpu")
+    parts = []
+    for shape in mlp_ref.SHAPES:
+        fan_in = shape[0] if len(shape) == 2 else 128
+        a = np.sqrt(6.0 / fan_in)
+        parts.append(((s.uniform(int(np.prod(shape))) * 2 - 1) * a).reshape(shape))
+    parts[1] += 0.01
+    parts[8] *= 0.1  # keep y = log-distance estimate O(1) so exp() stays well conditioned
+    return mlp_ref.pack(parts).astype(np.float32)
+
+
+@pytest.fixture(scope="module")
+def mesh():
+    v, f = wl.displaced_icosphere(4, 3)
+    return TriMeshSDF(v, f), v, f
+
+
+def test_forward_matches_oracle(cuda, mesh):
+    import torch
+    m, v, f = mesh
+    theta = small_theta(1)
+    m.set_bound_params(theta)
+    assert np.array_equal(m.get_bound_params(), theta)
+    q = wl.mixed_queries(3000, v, f, 5)
+    pred = m.predict_bound(torch.from_numpy(q).to(cuda)).cpu().numpy()
+    lo, sc, diag = norm_box(m)
+    ref = mlp_ref.predict(theta.astype(np.float64), q, lo, sc, float(diag), bf16_mode=True)
+    scale = np.abs(ref).max()
+    err = np.abs(pred - ref) / scale
+    # Same rounding points as the oracle: most outputs agree to FP32 accumulation order; a few
+    # move by one BF16 step where an activation sits on a rounding boundary (sin() and the FP32
+    # sums differ in the last bits), which a layout or permutation mistake could never mimic.
+    # Measured on MI355X: ~90% of queries agree to 1e-7, ~9% move by 1e-4..2e-2 of the scale.
+    assert np.median(err) <= 1e-5, np.median(err)
+    assert np.quantile(err, 0.8) <= 1e-5, np.quantile(err, 0.8)
+    assert np.quantile(err, 0.98) <= 5e-3, np.quantile(err, 0.98)
+    assert err.max() <= 3e-2, err.max()
+    # and against exact FP32 math the BF16 network stays within BF16 precision everywhere
+    ref32 = mlp_ref.predict(theta.astype(np.float64), q, lo, sc, float(diag), bf16_mode=False)
+    assert np.abs(pred - ref32).max() <= 5e-2 * scale
+
+
+def test_gradient_matches_oracle(cuda, mesh):
+    import torch
+    m, v, f = mesh
+    theta = small_theta(2)
+    m.set_bound_params(theta)
+    q = wl.mixed_queries(1000, v, f, 6)
+    d_ref, _ = port.Mesh(v.astype(np.float64), f).sdf(q.astype(np.float64))
+    qt = torch.from_numpy(q).to(cuda)
+    tt = torch.from_numpy(np.abs(d_ref).astype(np.float32)).to(cuda)
+    n = int(lib().ebg_mlp_param_count())
+    g = torch.zeros(n, dtype=torch.float32, device=cuda)
+    loss = torch.zeros(1, dtype=torch.float32, device=cuda)
+    check(lib().ebg_mlp_loss_grad(m.handle, qt.data_ptr(), tt.data_ptr(), q.shape[0], g.data_ptr(),
+                                  loss.data_ptr(), None))
+    torch.cuda.synchronize()
+    lo, sc, diag = norm_box(m)
+    L, gref = mlp_ref.loss_and_grad(theta.astype(np.float64), q, np.abs(d_ref).astype(np.float32), lo, sc,
+                                    float(diag), bf16_mode=True)
+    assert loss.item() / q.shape[0] == pytest.approx(L, rel=2e-3)
+    gg = g.cpu().numpy().astype(np.float64)
+    o = 0
+    for shape in mlp_ref.SHAPES:
+        k = int(np.prod(shape))
+        a, b = gg[o:o + k], gref[o:o + k]
+        err = np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-12)
+        assert err < 2e-2, (shape, o, err)
+        o += k
+
+
+def test_training_learns_and_bound_is_exact(cuda):
+    import torch
+    v, f = wl.displaced_icosphere(5, 21)
+    m = TriMeshSDF(v, f)
+    loss = m.train_bound(num_samples=400_000, epochs=4, batch=2048, seed=7)
+    assert np.isfinite(loss) and loss < 1.0
+    q = wl.mixed_queries(200_000, v, f, 22)
+    qt = torch.from_numpy(q).to(cuda)
+    pred = m.predict_bound(qt).cpu().numpy()
+    d0, t0 = m.closest_triangle(qt)
+    w = torch.zeros(8, dtype=torch.int64, device=cuda)
+    d1, t1 = m.closest_triangle(qt, learned_bound=True, work=w.data_ptr())
+    torch.cuda.synchronize()
+    d0, d1 = d0.cpu().numpy(), d1.cpu().numpy()
+    diag = float(np.linalg.norm(v.max(0).astype(np.float64) - v.min(0)))
+    # the bound never changes the answer: only exact ties may resolve to another face
+    assert np.abs(d0 - d1).max() <= 1e-6 * diag
+    assert np.array_equal(np.sign(d0), np.sign(d1))
+    # predictions are close to |d| for most queries
+    rel = np.abs(pred - np.abs(d0)) / (np.abs(d0) + 0.01 * diag)
+    assert np.median(rel) < 0.5, np.median(rel)
+    fallbacks = int(w[3].item())
+    assert fallbacks < 0.10 * q.shape[0], fallbacks
+
+
+def test_poisoned_bound_falls_back_everywhere(cuda):
+    """A network predicting 0 gives radius 1e-3*diag: every query beyond it must fall back and
+    still return the exact answer."""
+    import torch
+    v, f = wl.displaced_icosphere(4, 31)
+    m = TriMeshSDF(v, f)
+    m.set_bound_params(np.zeros(int(lib().ebg_mlp_param_count()), np.float32))
+    q = wl.uniform_box(20_000, [-1.5] * 3, [1.5] * 3, 32)
+    qt = torch.from_numpy(q).to(cuda)
+    d0 = m.query(qt)
+    w = torch.zeros(8, dtype=torch.int64, device=cuda)
+    d1 = m.query(qt, learned_bound=True, work=w.data_ptr())
+    torch.cuda.synchronize()
+    diag = float(np.linalg.norm(v.max(0).astype(np.float64) - v.min(0)))
+    far = (torch.abs(d0) > 2e-3 * diag).sum().item()
+    assert w[3].item() >= far
+    assert torch.equal(torch.sign(d0), torch.sign(d1))
+    assert (d0 - d1).abs().max().item() <= 1e-6 * diag
+
+
+def test_caller_seeded_radius_is_exact(cuda):
+    import torch
+    v, f = wl.displaced_icosphere(4, 41)
+    m = TriMeshSDF(v, f)
+    q = wl.mixed_queries(50_000, v, f, 42)
+    qt = torch.from_numpy(q).to(cuda)
+    d0 = m.query(qt)
+    for fac in [0.5, 1.0001, 3.0]:
+        seed = (d0.abs() * fac + 1e-6).contiguous()
+        d1 = m.query(qt, seed_radius=seed.data_ptr())
+        torch.cuda.synchronize()
+        assert torch.equal(torch.sign(d0), torch.sign(d1))
+        assert (d0 - d1).abs().max().item() <= 1e-6 * 4
