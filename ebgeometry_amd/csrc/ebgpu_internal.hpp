@@ -56,7 +56,8 @@ struct alignas(64) TriRec
   float    c2[3];  // e2 x n
   float    inv[3]; // 1/|e_i|^2
   uint32_t face;   // original face index
-  float    pad[4];
+  float    sc[3];  // bounding sphere centre (float) ...
+  float    sr;     // ... and radius, rounded outward: |p - sc| - sr <= distance(p, triangle)
 };
 static_assert(sizeof(TriRec) == 128, "TriRec layout");
 

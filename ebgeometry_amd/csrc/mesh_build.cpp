@@ -145,6 +145,13 @@ void pack_triangles(const float* xyz, const uint32_t* tri, const std::vector<uin
     put(r.c2, cross(e[2], n));
     for (int c = 0; c < 3; c++) r.inv[c] = float(1.0 / dot(e[c], e[c]));
     r.face = f;
+    // bounding sphere about the (float-rounded) centroid; the radius is measured in double from
+    // that float centre and inflated so that FP32 culling tests stay conservative
+    const D3 cen = {double(float((x0.x + x1.x + x2.x) / 3.0)), double(float((x0.y + x1.y + x2.y) / 3.0)),
+                    double(float((x0.z + x1.z + x2.z) / 3.0))};
+    const double rad = std::max(length(x0 - cen), std::max(length(x1 - cen), length(x2 - cen)));
+    put(r.sc, cen);
+    r.sr = float(rad * (1.0 + 1e-5) + 1e-7 * (length(cen) + rad));
     for (int v = 0; v < 3; v++)
       for (int c = 0; c < 3; c++) out.tripos[12 * i + 3 * v + c] = xyz[3 * tri[3 * f + v] + c];
     for (int v = 0; v < 3; v++) {

@@ -25,6 +25,8 @@
 
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
+
 namespace ebgpu {
 
 namespace {
@@ -388,10 +390,21 @@ __device__ __forceinline__ void traverse(const MeshDev& m, R px, R py, R pz, R b
       top += pushed;
     }
     else {
-      // leaf: every lane tests every triangle (keeping the minimum is exact for all lanes)
+      // leaf: a triangle is evaluated by the whole wavefront (keeping the minimum is exact for
+      // every lane) unless its bounding sphere lies beyond every lane's bound. Inactive lanes
+      // have bound < 0, so rb is NaN and they never ask for a triangle.
       const uint32_t first = uni(e.ref), cnt = uni(e.cnt);
-      if (COUNT) { n_tris += need ? cnt : 0u; w_tris += cnt; }
+      const R        rb    = sqrt(bound);
       for (uint32_t t = first; t < first + cnt; t++) {
+        const TriRec& rec = m.tris[t];
+        const R       sx = px - R(rec.sc[0]), sy = py - R(rec.sc[1]), sz = pz - R(rec.sc[2]);
+        const R       lim = rb + R(rec.sr);
+        const bool    tn  = sx * sx + sy * sy + sz * sz <= lim * lim;
+        if (__ballot(tn) == 0) continue;
+        if (COUNT) {
+          n_tris += tn ? 1u : 0u;
+          w_tris++;
+        }
         R        d2;
         uint32_t f;
         if constexpr (sizeof(R) == 4) {
@@ -411,14 +424,12 @@ __device__ __forceinline__ void traverse(const MeshDev& m, R px, R py, R pz, R b
   }
 }
 
+// One packet: the 64 queries pk*64 .. pk*64+63 of the (presorted) query order, one per lane.
 template <int K, int NB, bool COUNT>
-__global__ __launch_bounds__(256) void sdf32_kernel(const MeshDev m, const QueryArgs a)
+__device__ __forceinline__ void sdf32_packet(const MeshDev& m, const QueryArgs& a, uint64_t n, uint64_t pk)
 {
-  const uint32_t blk = xcd_remap(blockIdx.x, gridDim.x);
-  const uint64_t tid = (uint64_t)blk * 256u + threadIdx.x;
-  const uint64_t n   = a.count_ptr ? (uint64_t)*a.count_ptr : a.n;
-  if ((uint64_t)blk * 256u >= n) return; // whole wavefront-group past the end (uniform per block)
-  const bool active = tid < n;
+  const uint64_t tid    = pk * 64u + (uint64_t)__lane_id();
+  const bool     active = tid < n;
   uint32_t   qi     = 0;
   float      px = 0.0f, py = 0.0f, pz = 0.0f, bound = -1.0f;
   if (active) {
@@ -460,6 +471,17 @@ __global__ __launch_bounds__(256) void sdf32_kernel(const MeshDev m, const Query
       atomicAdd(a.work + 5, (unsigned long long)w_tris);
     }
   }
+}
+
+// Grid launch: one wavefront per 64-query packet, 4 packets per workgroup; workgroups are
+// mapped XCD-aware so that each XCD works through one contiguous (Morton) range of queries.
+template <int K, int NB, bool COUNT>
+__global__ __launch_bounds__(256) void sdf32_kernel(const MeshDev m, const QueryArgs a)
+{
+  const uint32_t blk = xcd_remap(blockIdx.x, gridDim.x);
+  const uint64_t n   = a.count_ptr ? (uint64_t)*a.count_ptr : a.n;
+  if ((uint64_t)blk * 256u >= n) return; // whole workgroup past the end (uniform per block)
+  sdf32_packet<K, NB, COUNT>(m, a, n, (uint64_t)blk * 4u + (threadIdx.x >> 6));
 }
 
 template <int K, int NB>
