@@ -38,7 +38,7 @@ SOURCES = [
     ("util.hip", DEVICE),
     ("mlp.hip", DEVICE),
 ]
-HEADERS = ["ebgpu_internal.hpp", "kernels.hpp", "mesh_build.hpp", "mlp.hpp"]
+HEADERS = ["ebgpu_internal.hpp", "kernels.hpp", "mesh_build.hpp", "mlp.hpp", "traverse_dev.hpp"]
 
 
 def _mtime(p):

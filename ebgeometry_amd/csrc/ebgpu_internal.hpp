@@ -78,4 +78,46 @@ struct MeshDev
   uint32_t        num_tris;
 };
 
+// ---------------------------------------------------------------------------------------------
+// CSG union scenes: primitives in BVH leaf order
+// ---------------------------------------------------------------------------------------------
+enum PrimType : uint32_t
+{
+  kPrimSphere = 0,
+  kPrimBox    = 1,
+  kPrimMesh   = 2
+};
+
+struct alignas(64) PrimRec
+{
+  uint32_t type, mesh, pad0, pad1;
+  float    a[6];  // sphere: centre xyz, radius; box: lo xyz, hi xyz
+  float    A[9];  // mesh instance: local point = A p + b, value = s * mesh(local)
+  float    b[3];
+  float    s;
+  float    bv[6]; // world bounding box (lo, hi)
+  float    pad2[3];
+};
+static_assert(sizeof(PrimRec) == 128, "PrimRec layout");
+
+struct alignas(64) PrimRec64 // FP64 copy of the same parameters (near-surface recheck)
+{
+  double a[6];
+  double A[9];
+  double b[3];
+  double s;
+  double pad[5];
+};
+static_assert(sizeof(PrimRec64) == 192, "PrimRec64 layout");
+
+struct SceneDev
+{
+  const void*      nodes;  // Node<4>
+  const PrimRec*   prims;
+  const PrimRec64* prims64;
+  const MeshDev*   meshes; // device array, indexed by PrimRec::mesh
+  float            root_lo[3], root_hi[3];
+  float            diag;
+};
+
 } // namespace ebgpu

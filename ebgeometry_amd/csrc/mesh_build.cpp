@@ -413,28 +413,55 @@ bool build_sah_mesh(const float* xyz, std::size_t nv, const uint32_t* tri, std::
     err = "leaf_size must be in 1..16";
     return false;
   }
-  std::vector<float> blo(3 * nt), bhi(3 * nt), cen(3 * nt);
+  std::vector<float> blo(3 * nt), bhi(3 * nt);
   for (std::size_t f = 0; f < nt; f++)
     for (int c = 0; c < 3; c++) {
       const float a = xyz[3 * tri[3 * f] + c], b = xyz[3 * tri[3 * f + 1] + c], d = xyz[3 * tri[3 * f + 2] + c];
       blo[3 * f + c] = std::min(a, std::min(b, d));
       bhi[3 * f + c] = std::max(a, std::max(b, d));
-      cen[3 * f + c] = 0.5f * (blo[3 * f + c] + bhi[3 * f + c]);
     }
+  WideBvh w;
+  if (!build_wide_bvh(blo.data(), bhi.data(), nt, K, leaf_size, w, err)) return false;
+  out.K          = K;
+  out.leaf_size  = leaf_size;
+  out.num_nodes  = w.num_nodes;
+  out.num_leaves = w.num_leaves;
+  out.nodes      = std::move(w.nodes);
+  for (int c = 0; c < 3; c++) {
+    out.lo[c] = w.lo[c];
+    out.hi[c] = w.hi[c];
+  }
+  out.diag = w.diag;
+  pack_triangles(xyz, tri, w.order, pn, nv, out);
+  out.stack_need = w.stack_need;
+  return true;
+}
+
+bool build_wide_bvh(const float* blo, const float* bhi, std::size_t n, int K, int leaf_size, WideBvh& out,
+                    std::string& err)
+{
+  if (K != 4 && K != 8) {
+    err = "branching must be 4 or 8";
+    return false;
+  }
+  if (leaf_size < 1 || leaf_size > kMaxLeaf || n < 1) {
+    err = "leaf_size must be in 1..16 and there must be at least one primitive";
+    return false;
+  }
+  std::vector<float> cen(3 * n);
+  for (std::size_t i = 0; i < 3 * n; i++) cen[i] = 0.5f * (blo[i] + bhi[i]);
   SahBuilder sb;
-  sb.blo       = blo.data();
-  sb.bhi       = bhi.data();
+  sb.blo       = blo;
+  sb.bhi       = bhi;
   sb.cen       = cen.data();
   sb.leaf_size = leaf_size;
-  sb.idx.resize(nt);
-  for (std::size_t i = 0; i < nt; i++) sb.idx[i] = uint32_t(i);
-  sb.nodes.resize(std::max<std::size_t>(1, 2 * nt));
+  sb.idx.resize(n);
+  for (std::size_t i = 0; i < n; i++) sb.idx[i] = uint32_t(i);
+  sb.nodes.resize(std::max<std::size_t>(1, 2 * n));
   const int32_t root = sb.alloc();
-  sb.build(root, 0, uint32_t(nt), 0);
+  sb.build(root, 0, uint32_t(n), 0);
   sb.nodes.resize(std::size_t(sb.next.load()));
 
-  out.K         = K;
-  out.leaf_size = leaf_size;
   const BNode& r = sb.nodes[root];
   for (int c = 0; c < 3; c++) {
     out.lo[c] = r.lo[c];
@@ -443,6 +470,8 @@ bool build_sah_mesh(const float* xyz, std::size_t nv, const uint32_t* tri, std::
   const double dx = double(r.hi[0]) - r.lo[0], dy = double(r.hi[1]) - r.lo[1], dz = double(r.hi[2]) - r.lo[2];
   out.diag = std::sqrt(dx * dx + dy * dy + dz * dz);
 
+  PackedMesh tmp; // only to reuse compute_stack_need
+  tmp.K = K;
   auto collapse = [&](auto tag) {
     constexpr int KK = decltype(tag)::value;
     Collapser<KK> col(sb.nodes);
@@ -455,9 +484,10 @@ bool build_sah_mesh(const float* xyz, std::size_t nv, const uint32_t* tri, std::
   };
   if (K == 4) collapse(std::integral_constant<int, 4>{});
   else collapse(std::integral_constant<int, 8>{});
-
-  pack_triangles(xyz, tri, sb.idx, pn, nv, out);
-  out.stack_need = compute_stack_need(out);
+  tmp.nodes      = out.nodes;
+  tmp.num_nodes  = out.num_nodes;
+  out.stack_need = compute_stack_need(tmp);
+  out.order      = std::move(sb.idx);
   return true;
 }
 

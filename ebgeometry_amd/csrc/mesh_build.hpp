@@ -53,4 +53,18 @@ void pack_triangles(const float* xyz, const uint32_t* tri, const std::vector<uin
 // Maximum traversal stack depth any query can need in a packed K-wide tree.
 int compute_stack_need(const PackedMesh& m);
 
+// A K-wide SAH BVH over arbitrary boxes (Node<K> records, DFS pre-order, root at 0); leaf
+// children reference ranges of `order` (primitive indices in leaf order).
+struct WideBvh
+{
+  std::vector<uint8_t>  nodes;
+  std::size_t           num_nodes = 0, num_leaves = 0;
+  int                   stack_need = 0;
+  std::vector<uint32_t> order;
+  float                 lo[3], hi[3];
+  double                diag = 0;
+};
+bool build_wide_bvh(const float* blo, const float* bhi, std::size_t n, int K, int leaf_size, WideBvh& out,
+                    std::string& err);
+
 } // namespace ebgpu

@@ -1,0 +1,406 @@
+// Device-side building blocks of the packet traversal, shared by the mesh, scene and batched
+// kernels (included by their .hip translation units; everything here is inline or a template).
+#pragma once
+
+#include "ebgpu_internal.hpp"
+
+#include <hip/hip_runtime.h>
+
+namespace ebgpu {
+namespace dev {
+
+// ---------------------------------------------------------------------------------------
+// wavefront helpers
+// ---------------------------------------------------------------------------------------
+
+__device__ __forceinline__ uint32_t rdlane(uint32_t v, int l) { return (uint32_t)__builtin_amdgcn_readlane((int)v, l); }
+__device__ __forceinline__ float rdlanef(float v, int l)
+{
+  return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), l));
+}
+// Lane write: `me` is true on exactly one lane; one v_cndmask per field.
+__device__ __forceinline__ uint32_t wrlane(uint32_t v, bool me, uint32_t old) { return me ? v : old; }
+__device__ __forceinline__ float    wrlanef(float v, bool me, float old) { return me ? v : old; }
+__device__ __forceinline__ uint32_t uni(uint32_t v) { return (uint32_t)__builtin_amdgcn_readfirstlane((int)v); }
+
+// Minimum over the 64 lanes of an unsigned value (all lanes active): DPP within rows of 16,
+// then the four row results combined in scalar registers.
+__device__ __forceinline__ uint32_t wave_min_u32(uint32_t v)
+{
+  v = min(v, (uint32_t)__builtin_amdgcn_mov_dpp((int)v, 0xB1, 0xF, 0xF, false));  // quad_perm 1,0,3,2
+  v = min(v, (uint32_t)__builtin_amdgcn_mov_dpp((int)v, 0x4E, 0xF, 0xF, false));  // quad_perm 2,3,0,1
+  v = min(v, (uint32_t)__builtin_amdgcn_mov_dpp((int)v, 0x141, 0xF, 0xF, false)); // row_half_mirror
+  v = min(v, (uint32_t)__builtin_amdgcn_mov_dpp((int)v, 0x140, 0xF, 0xF, false)); // row_mirror
+  const uint32_t a = rdlane(v, 0), b = rdlane(v, 16), c = rdlane(v, 32), d = rdlane(v, 48);
+  return min(min(a, b), min(c, d));
+}
+
+__device__ __forceinline__ uint64_t wave_sum_u64(uint64_t v)
+{
+  // few calls per wave: plain butterfly through ds_swizzle-free shuffles
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+  return v;
+}
+
+// Hardware dispatches workgroup b to XCD b % 8. Give each XCD one contiguous range of the
+// (Morton-sorted) queries so that its L2 holds one region of the tree.
+__device__ __forceinline__ uint32_t xcd_remap(uint32_t b, uint32_t nb)
+{
+  const uint32_t x = b & 7u, i = b >> 3, per = nb >> 3, rem = nb & 7u;
+  return x * per + min(x, rem) + i;
+}
+
+// ---------------------------------------------------------------------------------------
+// VGPR-lane stack
+// ---------------------------------------------------------------------------------------
+
+struct Entry
+{
+  uint32_t ref, cnt;
+  float    lx, ly, lz, hx, hy, hz;
+};
+
+template <int NB>
+struct LaneStack
+{
+  uint32_t ref[NB], cnt[NB];
+  float    lx[NB], ly[NB], lz[NB], hx[NB], hy[NB], hz[NB];
+
+  __device__ __forceinline__ void init()
+  {
+#pragma unroll
+    for (int i = 0; i < NB; i++) {
+      ref[i] = cnt[i] = 0u;
+      lx[i] = ly[i] = lz[i] = hx[i] = hy[i] = hz[i] = 0.0f;
+    }
+  }
+
+  // Write entry e at stack position pos (wave-uniform); lane = this lane's id.
+  __device__ __forceinline__ void put(int pos, const Entry& e, int lane)
+  {
+    const bool me = lane == (pos & 63);
+#pragma unroll
+    for (int i = 0; i < NB; i++) {
+      if ((pos >> 6) == i) {
+        ref[i] = wrlane(e.ref, me, ref[i]);
+        cnt[i] = wrlane(e.cnt, me, cnt[i]);
+        lx[i]  = wrlanef(e.lx, me, lx[i]);
+        ly[i]  = wrlanef(e.ly, me, ly[i]);
+        lz[i]  = wrlanef(e.lz, me, lz[i]);
+        hx[i]  = wrlanef(e.hx, me, hx[i]);
+        hy[i]  = wrlanef(e.hy, me, hy[i]);
+        hz[i]  = wrlanef(e.hz, me, hz[i]);
+      }
+    }
+  }
+
+  __device__ __forceinline__ Entry get(int pos) const
+  {
+    const int l = pos & 63;
+    Entry     e{};
+#pragma unroll
+    for (int i = 0; i < NB; i++) {
+      if ((pos >> 6) == i) {
+        e.ref = rdlane(ref[i], l);
+        e.cnt = rdlane(cnt[i], l);
+        e.lx  = rdlanef(lx[i], l);
+        e.ly  = rdlanef(ly[i], l);
+        e.lz  = rdlanef(lz[i], l);
+        e.hx  = rdlanef(hx[i], l);
+        e.hy  = rdlanef(hy[i], l);
+        e.hz  = rdlanef(hz[i], l);
+      }
+    }
+    return e;
+  }
+};
+
+// ---------------------------------------------------------------------------------------
+// distance kernels
+// ---------------------------------------------------------------------------------------
+
+template <class R>
+__device__ __forceinline__ R box_d2(R px, R py, R pz, float lx, float ly, float lz, float hx, float hy, float hz)
+{
+  const R dx = max(R(0), max(R(lx) - px, px - R(hx)));
+  const R dy = max(R(0), max(R(ly) - py, py - R(hy)));
+  const R dz = max(R(0), max(R(lz) - pz, pz - R(hz)));
+  return dx * dx + dy * dy + dz * dz;
+}
+
+// Squared distance from p to one triangle and the closest feature (FP32, precomputed record).
+__device__ __forceinline__ void tri_eval32(const TriRec& r, float px, float py, float pz, float& d2, uint32_t& feat)
+{
+  const float p1x = px - r.v0[0], p1y = py - r.v0[1], p1z = pz - r.v0[2];
+  const float p2x = p1x - r.e0[0], p2y = p1y - r.e0[1], p2z = p1z - r.e0[2];
+  const float p3x = p1x + r.e2[0], p3y = p1y + r.e2[1], p3z = p1z + r.e2[2];
+
+  const bool s0 = (r.c0[0] * p1x + r.c0[1] * p1y + r.c0[2] * p1z) > 0.0f;
+  const bool s1 = (r.c1[0] * p2x + r.c1[1] * p2y + r.c1[2] * p2z) > 0.0f;
+  const bool s2 = (r.c2[0] * p3x + r.c2[1] * p3y + r.c2[2] * p3z) > 0.0f;
+  const bool inface = (s0 == s1) && (s1 == s2);
+  const float fd    = r.n[0] * p1x + r.n[1] * p1y + r.n[2] * p1z;
+
+  float    r2 = p1x * p1x + p1y * p1y + p1z * p1z;
+  uint32_t f  = kVert0;
+  float    q  = p2x * p2x + p2y * p2y + p2z * p2z;
+  if (q < r2) { r2 = q; f = kVert1; }
+  q = p3x * p3x + p3y * p3y + p3z * p3z;
+  if (q < r2) { r2 = q; f = kVert2; }
+
+  float t = (p1x * r.e0[0] + p1y * r.e0[1] + p1z * r.e0[2]) * r.inv[0];
+  if (t > 0.0f && t < 1.0f) {
+    const float yx = p1x - t * r.e0[0], yy = p1y - t * r.e0[1], yz = p1z - t * r.e0[2];
+    q              = yx * yx + yy * yy + yz * yz;
+    if (q < r2) { r2 = q; f = kEdge0; }
+  }
+  t = (p2x * r.e1[0] + p2y * r.e1[1] + p2z * r.e1[2]) * r.inv[1];
+  if (t > 0.0f && t < 1.0f) {
+    const float yx = p2x - t * r.e1[0], yy = p2y - t * r.e1[1], yz = p2z - t * r.e1[2];
+    q              = yx * yx + yy * yy + yz * yz;
+    if (q < r2) { r2 = q; f = kEdge1; }
+  }
+  t = (p3x * r.e2[0] + p3y * r.e2[1] + p3z * r.e2[2]) * r.inv[2];
+  if (t > 0.0f && t < 1.0f) {
+    const float yx = p3x - t * r.e2[0], yy = p3y - t * r.e2[1], yz = p3z - t * r.e2[2];
+    q              = yx * yx + yy * yy + yz * yz;
+    if (q < r2) { r2 = q; f = kEdge2; }
+  }
+  d2   = inface ? fd * fd : r2;
+  feat = inface ? (uint32_t)kFace : f;
+}
+
+// Signed distance of the chosen feature (FP32): face -> n.(p-v0); vertex/edge ->
+// sqrt(d2) * sgn(pseudo-normal . (p - closest point)), sgn(x) = x > 0 ? 1 : -1.
+__device__ __forceinline__ float resolve32(const MeshDev& m, uint32_t t, uint32_t feat, float d2, float px, float py, float pz)
+{
+  const TriRec& r   = m.tris[t];
+  const float   p1x = px - r.v0[0], p1y = py - r.v0[1], p1z = pz - r.v0[2];
+  if (feat == kFace) return r.n[0] * p1x + r.n[1] * p1y + r.n[2] * p1z;
+  float vx, vy, vz;
+  const float* nrm;
+  if (feat <= kVert2) {
+    const uint32_t vid = m.feat[8 * t + (feat - kVert0)];
+    nrm                = m.vn32 + 4 * (size_t)vid;
+    if (feat == kVert0) { vx = p1x; vy = p1y; vz = p1z; }
+    else if (feat == kVert1) { vx = p1x - r.e0[0]; vy = p1y - r.e0[1]; vz = p1z - r.e0[2]; }
+    else { vx = p1x + r.e2[0]; vy = p1y + r.e2[1]; vz = p1z + r.e2[2]; }
+  }
+  else {
+    const uint32_t e  = feat - kEdge0;
+    const uint32_t he = m.feat[8 * t + 3 + e];
+    nrm               = m.en32 + 4 * (size_t)he;
+    float ax, ay, az, ex, ey, ez, inv;
+    if (e == 0) { ax = p1x; ay = p1y; az = p1z; ex = r.e0[0]; ey = r.e0[1]; ez = r.e0[2]; inv = r.inv[0]; }
+    else if (e == 1) {
+      ax = p1x - r.e0[0]; ay = p1y - r.e0[1]; az = p1z - r.e0[2];
+      ex = r.e1[0]; ey = r.e1[1]; ez = r.e1[2]; inv = r.inv[1];
+    }
+    else {
+      ax = p1x + r.e2[0]; ay = p1y + r.e2[1]; az = p1z + r.e2[2];
+      ex = r.e2[0]; ey = r.e2[1]; ez = r.e2[2]; inv = r.inv[2];
+    }
+    const float tt = (ax * ex + ay * ey + az * ez) * inv;
+    vx             = ax - tt * ex;
+    vy             = ay - tt * ey;
+    vz             = az - tt * ez;
+  }
+  const float s = (nrm[0] * vx + nrm[1] * vy + nrm[2] * vz) > 0.0f ? 1.0f : -1.0f;
+  return s * sqrtf(d2);
+}
+
+// FP64: one triangle with the reference lane formulas on the exact (float) vertex positions
+// and the FP64 pseudo-normals. Returns d^2 (face: (n.p1)^2) and the feature.
+__device__ __forceinline__ void tri_eval64(const float* pos, const double* fn, double px, double py, double pz,
+                                           double& d2, uint32_t& feat)
+{
+  const double v0x = pos[0], v0y = pos[1], v0z = pos[2];
+  const double v1x = pos[3], v1y = pos[4], v1z = pos[5];
+  const double v2x = pos[6], v2y = pos[7], v2z = pos[8];
+  const double nx = fn[0], ny = fn[1], nz = fn[2];
+  const double ax = v1x - v0x, ay = v1y - v0y, az = v1z - v0z; // v21
+  const double bx = v2x - v1x, by = v2y - v1y, bz = v2z - v1z; // v32
+  const double cx = v0x - v2x, cy = v0y - v2y, cz = v0z - v2z; // v13
+  const double p1x = px - v0x, p1y = py - v0y, p1z = pz - v0z;
+  const double p2x = px - v1x, p2y = py - v1y, p2z = pz - v1z;
+  const double p3x = px - v2x, p3y = py - v2y, p3z = pz - v2z;
+  // (v x n) . p
+  const double s0 = (ay * nz - az * ny) * p1x + (az * nx - ax * nz) * p1y + (ax * ny - ay * nx) * p1z;
+  const double s1 = (by * nz - bz * ny) * p2x + (bz * nx - bx * nz) * p2y + (bx * ny - by * nx) * p2z;
+  const double s2 = (cy * nz - cz * ny) * p3x + (cz * nx - cx * nz) * p3y + (cx * ny - cy * nx) * p3z;
+  const int    ss = (s0 > 0.0 ? 1 : -1) + (s1 > 0.0 ? 1 : -1) + (s2 > 0.0 ? 1 : -1);
+  if (ss >= 2 || ss <= -2) {
+    const double fd = nx * p1x + ny * p1y + nz * p1z;
+    d2              = fd * fd;
+    feat            = kFace;
+    return;
+  }
+  double   r2 = p1x * p1x + p1y * p1y + p1z * p1z;
+  uint32_t f  = kVert0;
+  double   q  = p2x * p2x + p2y * p2y + p2z * p2z;
+  if (q < r2) { r2 = q; f = kVert1; }
+  q = p3x * p3x + p3y * p3y + p3z * p3z;
+  if (q < r2) { r2 = q; f = kVert2; }
+  double t = (p1x * ax + p1y * ay + p1z * az) / (ax * ax + ay * ay + az * az);
+  if (t > 0.0 && t < 1.0) {
+    const double yx = p1x - t * ax, yy = p1y - t * ay, yz = p1z - t * az;
+    q               = yx * yx + yy * yy + yz * yz;
+    if (q < r2) { r2 = q; f = kEdge0; }
+  }
+  t = (p2x * bx + p2y * by + p2z * bz) / (bx * bx + by * by + bz * bz);
+  if (t > 0.0 && t < 1.0) {
+    const double yx = p2x - t * bx, yy = p2y - t * by, yz = p2z - t * bz;
+    q               = yx * yx + yy * yy + yz * yz;
+    if (q < r2) { r2 = q; f = kEdge1; }
+  }
+  t = (p3x * cx + p3y * cy + p3z * cz) / (cx * cx + cy * cy + cz * cz);
+  if (t > 0.0 && t < 1.0) {
+    const double yx = p3x - t * cx, yy = p3y - t * cy, yz = p3z - t * cz;
+    q               = yx * yx + yy * yy + yz * yz;
+    if (q < r2) { r2 = q; f = kEdge2; }
+  }
+  d2   = r2;
+  feat = f;
+}
+
+__device__ __forceinline__ double resolve64(const MeshDev& m, uint32_t t, uint32_t feat, double d2, double px,
+                                            double py, double pz)
+{
+  const float*  pos = m.tripos + 12 * (size_t)t;
+  const double* fn  = m.fn64 + 4 * (size_t)t;
+  const double  v0x = pos[0], v0y = pos[1], v0z = pos[2];
+  const double  p1x = px - v0x, p1y = py - v0y, p1z = pz - v0z;
+  if (feat == kFace) return fn[0] * p1x + fn[1] * p1y + fn[2] * p1z;
+  double        vx, vy, vz;
+  const double* nrm;
+  if (feat <= kVert2) {
+    const uint32_t k = feat - kVert0;
+    nrm              = m.vn64 + 4 * (size_t)m.feat[8 * t + k];
+    vx               = px - double(pos[3 * k]);
+    vy               = py - double(pos[3 * k + 1]);
+    vz               = pz - double(pos[3 * k + 2]);
+  }
+  else {
+    const uint32_t e = feat - kEdge0, a = e, b = (e + 1) % 3;
+    nrm              = m.en64 + 4 * (size_t)m.feat[8 * t + 3 + e];
+    const double ax = double(pos[3 * b]) - double(pos[3 * a]), ay = double(pos[3 * b + 1]) - double(pos[3 * a + 1]),
+                 az = double(pos[3 * b + 2]) - double(pos[3 * a + 2]);
+    const double qx = px - double(pos[3 * a]), qy = py - double(pos[3 * a + 1]), qz = pz - double(pos[3 * a + 2]);
+    const double tt = (qx * ax + qy * ay + qz * az) / (ax * ax + ay * ay + az * az);
+    vx              = qx - tt * ax;
+    vy              = qy - tt * ay;
+    vz              = qz - tt * az;
+  }
+  const double s = (nrm[0] * vx + nrm[1] * vy + nrm[2] * vz) > 0.0 ? 1.0 : -1.0;
+  return sqrt(d2) * s;
+}
+
+// Wave-aggregated append of `idx` for the lanes where pred holds.
+__device__ __forceinline__ void wave_append(bool pred, uint32_t idx, uint32_t* list, uint32_t* count)
+{
+  const uint64_t mask = __ballot(pred);
+  if (mask == 0) return;
+  const int      lane   = __lane_id();
+  const uint32_t n      = (uint32_t)__popcll(mask);
+  const int      leader = __ffsll((unsigned long long)mask) - 1;
+  uint32_t       base   = 0;
+  if (lane == leader) base = atomicAdd(count, n);
+  base = (uint32_t)__builtin_amdgcn_readlane((int)base, leader);
+  if (pred) list[base + (uint32_t)__popcll(mask & ((1ull << lane) - 1ull))] = idx;
+}
+
+// ---------------------------------------------------------------------------------------
+// the packet traversal
+// ---------------------------------------------------------------------------------------
+
+// R = float: FP32 pass on TriRec records. R = double: FP64 recheck on raw positions.
+template <int K, int NB, class R, bool COUNT>
+__device__ __forceinline__ void traverse(const MeshDev& m, R px, R py, R pz, R bound, R& best, uint32_t& btri,
+                                         uint32_t& bfeat, uint32_t& n_inner, uint32_t& n_tris, uint32_t& w_inner, uint32_t& w_tris)
+{
+  const Node<K>* __restrict__ nodes = static_cast<const Node<K>*>(m.nodes);
+  LaneStack<NB>               st;
+  st.init();
+  const int lane = __lane_id();
+  int       top  = 0;
+  Entry root{0u, 0u, m.root_lo[0], m.root_lo[1], m.root_lo[2], m.root_hi[0], m.root_hi[1], m.root_hi[2]};
+  st.put(top++, root, lane);
+
+  while (top > 0) {
+    const Entry e = st.get(--top);
+    const R     d2e  = box_d2<R>(px, py, pz, e.lx, e.ly, e.lz, e.hx, e.hy, e.hz);
+    const bool  need = d2e <= bound;
+    if (__ballot(need) == 0) continue;
+
+    if (e.cnt == 0u) {
+      // interior node: test the K children
+      const uint32_t ni = uni(e.ref);
+      const Node<K>& nd = nodes[ni];
+      if (COUNT) { n_inner += need ? 1u : 0u; w_inner++; }
+      uint32_t key[K];
+      uint64_t msk[K];
+#pragma unroll
+      for (int k = 0; k < K; k++) {
+        msk[k] = 0;
+        key[k] = 0xFFFFFFFFu;
+        if (nd.cnt[k] != kInvalid) {
+          const R    d2 = box_d2<R>(px, py, pz, nd.lox[k], nd.loy[k], nd.loz[k], nd.hix[k], nd.hiy[k], nd.hiz[k]);
+          const bool nk = d2 <= bound;
+          msk[k]        = __ballot(nk);
+          if (msk[k]) key[k] = wave_min_u32(nk ? __float_as_uint(float(d2)) : 0xFFFFFFFFu);
+        }
+      }
+      // rank the needed children: farthest deepest, nearest on top
+      int pushed = 0;
+#pragma unroll
+      for (int k = 0; k < K; k++) {
+        if (!msk[k]) continue;
+        int rank = 0;
+#pragma unroll
+        for (int j = 0; j < K; j++) {
+          if (j == k || !msk[j]) continue;
+          rank += (key[j] > key[k] || (key[j] == key[k] && j < k)) ? 1 : 0;
+        }
+        Entry c{nd.ref[k], nd.cnt[k], nd.lox[k], nd.loy[k], nd.loz[k], nd.hix[k], nd.hiy[k], nd.hiz[k]};
+        st.put(top + rank, c, lane);
+        pushed++;
+      }
+      top += pushed;
+    }
+    else {
+      // leaf: a triangle is evaluated by the whole wavefront (keeping the minimum is exact for
+      // every lane) unless its bounding sphere lies beyond every lane's bound. Inactive lanes
+      // have bound < 0, so rb is NaN and they never ask for a triangle.
+      const uint32_t first = uni(e.ref), cnt = uni(e.cnt);
+      const R        rb    = sqrt(bound);
+      for (uint32_t t = first; t < first + cnt; t++) {
+        const TriRec& rec = m.tris[t];
+        const R       sx = px - R(rec.sc[0]), sy = py - R(rec.sc[1]), sz = pz - R(rec.sc[2]);
+        const R       lim = rb + R(rec.sr);
+        const bool    tn  = sx * sx + sy * sy + sz * sz <= lim * lim;
+        if (__ballot(tn) == 0) continue;
+        if (COUNT) {
+          n_tris += tn ? 1u : 0u;
+          w_tris++;
+        }
+        R        d2;
+        uint32_t f;
+        if constexpr (sizeof(R) == 4) {
+          tri_eval32(m.tris[t], px, py, pz, d2, f);
+        }
+        else {
+          tri_eval64(m.tripos + 12 * (size_t)t, m.fn64 + 4 * (size_t)t, px, py, pz, d2, f);
+        }
+        if (d2 < best) {
+          best  = d2;
+          btri  = t;
+          bfeat = f;
+        }
+      }
+      bound = min(bound, best);
+    }
+  }
+}
+
+} // namespace dev
+} // namespace ebgpu
