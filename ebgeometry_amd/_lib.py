@@ -39,6 +39,12 @@ class MlpOpts(C.Structure):
                 ("under_weight", C.c_float), ("reserved", _i32)]
 
 
+class Prim(C.Structure):
+    _fields_ = [("type", _i32), ("mesh", _i32), ("p", C.c_double * 6), ("translate", C.c_double * 3),
+                ("matrix", C.c_double * 9), ("scale", C.c_double), ("bv_lo", C.c_double * 3),
+                ("bv_hi", C.c_double * 3)]
+
+
 # (name, restype, argtypes) of every exported entry point
 SIGNATURES = [
     ("ebg_last_error", C.c_char_p, []),
@@ -53,6 +59,12 @@ SIGNATURES = [
     ("ebg_morton_codes_f64", C.c_int, [_vp, _u64, _vp, _vp]),
     ("ebg_morton_sort", C.c_int, [_vp, _u64, _vp, _vp, _vp]),
     ("ebg_pseudo_normals_host", C.c_int, [_vp, _u64, _vp, _u64, _vp, _vp, _vp]),
+    ("ebg_scene_create", C.c_int, [C.POINTER(Prim), _u64, _vp, _u64, C.POINTER(BuildOpts), C.POINTER(_vp)]),
+    ("ebg_scene_destroy", C.c_int, [_vp]),
+    ("ebg_scene_bounds", C.c_int, [_vp, _vp, _vp, _vp]),
+    ("ebg_scene_query", C.c_int, [_vp, _vp, _u64, _vp, C.POINTER(QueryOpts)]),
+    ("ebg_scene_query_host", C.c_int, [_vp, _vp, _u64, _vp, C.POINTER(QueryOpts)]),
+    ("ebg_scene_last_traversal_ms", C.c_double, [_vp]),
     ("ebg_mlp_opts_default", None, [C.POINTER(MlpOpts)]),
     ("ebg_mlp_train", C.c_int, [_vp, C.POINTER(MlpOpts), C.POINTER(C.c_float)]),
     ("ebg_mlp_predict", C.c_int, [_vp, _vp, _u64, _vp, _vp]),

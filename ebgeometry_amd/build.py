@@ -33,12 +33,14 @@ HOST = ["-pthread"]
 SOURCES = [
     ("mesh_build.cpp", HOST),
     ("api.cpp", HOST),
+    ("scene_api.cpp", HOST),
     ("traverse.hip", DEVICE),
+    ("scene.hip", DEVICE),
     ("morton.hip", DEVICE),
     ("util.hip", DEVICE),
     ("mlp.hip", DEVICE),
 ]
-HEADERS = ["ebgpu_internal.hpp", "kernels.hpp", "mesh_build.hpp", "mlp.hpp", "traverse_dev.hpp"]
+HEADERS = ["ebgpu_internal.hpp", "kernels.hpp", "mesh_build.hpp", "mlp.hpp", "traverse_dev.hpp", "host_common.hpp"]
 
 
 def _mtime(p):

@@ -8,10 +8,8 @@
 //      no triangle (exactness fallback)
 //   5. FP64 traversal for queries with |d| < recheck_eps * diag (sign is ill-conditioned in
 //      FP32 there), seeded with |d32| + slack
-#include "../../include/ebgpu.h"
-#include "kernels.hpp"
+#include "host_common.hpp"
 #include "mesh_build.hpp"
-#include "mlp.hpp"
 
 #include <hip/hip_runtime.h>
 
@@ -24,124 +22,16 @@
 #include <vector>
 
 namespace {
-
 thread_local std::string g_last_error;
-
-ebg_status fail(ebg_status s, const std::string& msg)
-{
-  g_last_error = msg;
-  return s;
-}
-
-#define EBG_HIP(call)                                                                                        \
-  do {                                                                                                       \
-    const hipError_t e_ = (call);                                                                            \
-    if (e_ != hipSuccess)                                                                                    \
-      return fail(e_ == hipErrorOutOfMemory ? EBG_OOM : EBG_HIP_ERROR,                                      \
-                  std::string(#call) + ": " + hipGetErrorString(e_));                                        \
-  } while (0)
-
-struct DevBuf
-{
-  void*       p     = nullptr;
-  std::size_t bytes = 0;
-  ~DevBuf() { release(); }
-  void release()
-  {
-    if (p) (void)hipFree(p);
-    p     = nullptr;
-    bytes = 0;
-  }
-  hipError_t reserve(std::size_t b)
-  {
-    if (b <= bytes) return hipSuccess;
-    release();
-    hipError_t e = hipMalloc(&p, b);
-    if (e == hipSuccess) bytes = b;
-    else p = nullptr;
-    return e;
-  }
-  template <class T>
-  T* as() const
-  {
-    return static_cast<T*>(p);
-  }
-};
-
-template <class T>
-hipError_t upload(DevBuf& b, const std::vector<T>& v)
-{
-  const std::size_t bytes = std::max<std::size_t>(64, v.size() * sizeof(T));
-  hipError_t        e     = b.reserve(bytes);
-  if (e != hipSuccess) return e;
-  if (!v.empty()) e = hipMemcpy(b.p, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice);
-  return e;
-}
-
-// Query workspace, grown on demand.
-struct Workspace
-{
-  DevBuf codes, codes_sorted, perm, sort_scratch, box, lists, seeds;
-  uint64_t cap = 0;
-
-  hipError_t ensure(uint64_t n)
-  {
-    if (n <= cap && cap) return hipSuccess;
-    const uint64_t m = std::max<uint64_t>(n, 1024);
-    hipError_t     e;
-    if ((e = codes.reserve(8 * m)) != hipSuccess) return e;
-    if ((e = codes_sorted.reserve(8 * m)) != hipSuccess) return e;
-    if ((e = perm.reserve(4 * m)) != hipSuccess) return e;
-    if ((e = sort_scratch.reserve(ebgpu::radix_sort_scratch_bytes(m))) != hipSuccess) return e;
-    if ((e = box.reserve(sizeof(double) * 6 * (ebgpu::kBboxParts + 1))) != hipSuccess) return e;
-    // recheck list, fallback list, 2 counters (+pad)
-    if ((e = lists.reserve(8 * m + 256)) != hipSuccess) return e;
-    if ((e = seeds.reserve(4 * m)) != hipSuccess) return e;
-    cap = m;
-    return hipSuccess;
-  }
-  uint32_t* recheck_list() const { return lists.as<uint32_t>(); }
-  uint32_t* fallback_list() const { return lists.as<uint32_t>() + cap; }
-  uint32_t* counters() const { return lists.as<uint32_t>() + 2 * cap; }
-};
-
 } // namespace
 
-struct ebg_mesh
-{
-  int         K = 4, leaf = 4, banks = 1, stack_need = 0;
-  std::size_t nv = 0, nt = 0, num_nodes = 0, num_leaves = 0;
-  double      lo[3]{}, hi[3]{}, diag = 0;
-  double      build_seconds = 0;
-  DevBuf      nodes, tris, tripos, feat, vn32, en32, vn64, en64, fn64;
-  DevBuf      area_cdf; // normalised cumulative triangle area, leaf order (training samples)
-  ebgpu::MeshDev dev{};
-  ebgpu::MlpTrainBuffers train;
-  std::size_t device_bytes = 0;
-  std::recursive_mutex mu; // training runs queries while holding it
-  Workspace   ws;
-  hipEvent_t  ev0 = nullptr, ev1 = nullptr;
-  bool        timed = false;
-  ebgpu::Mlp  mlp;
-
-  ~ebg_mesh()
-  {
-    if (ev0) (void)hipEventDestroy(ev0);
-    if (ev1) (void)hipEventDestroy(ev1);
-  }
-};
+using ebgpu::DevBuf;
+using ebgpu::fail;
+using ebgpu::upload;
+using ebgpu::Workspace;
 
 namespace ebgpu {
-// Exposed to the other translation units of the library (MLP training, batch, scene).
-const MeshDev& mesh_dev(const ebg_mesh* m) { return m->dev; }
-int            mesh_K(const ebg_mesh* m) { return m->K; }
-int            mesh_banks(const ebg_mesh* m) { return m->banks; }
-double         mesh_diag(const ebg_mesh* m) { return m->diag; }
-const double*  mesh_lo(const ebg_mesh* m) { return m->lo; }
-const double*  mesh_hi(const ebg_mesh* m) { return m->hi; }
-Mlp&           mesh_mlp(ebg_mesh* m) { return m->mlp; }
-const Mlp&     mesh_mlp(const ebg_mesh* m) { return m->mlp; }
-void           set_last_error(const std::string& s) { g_last_error = s; }
+void set_last_error(const std::string& s) { g_last_error = s; }
 } // namespace ebgpu
 
 extern "C" {
@@ -254,6 +144,7 @@ ebg_status ebg_mesh_create(const float* xyz, uint64_t nv, const uint32_t* tri, u
   auto* m = new ebg_mesh;
   m->nv   = nv;
   m->nt   = nt;
+  m->host_xyz.assign(xyz, xyz + 3 * nv);
   const ebg_status st = upload_mesh(m, pm);
   if (st != EBG_OK) {
     delete m;
@@ -266,7 +157,7 @@ ebg_status ebg_mesh_create(const float* xyz, uint64_t nv, const uint32_t* tri, u
 
 ebg_status ebg_mesh_destroy(ebg_mesh* mesh)
 {
-  delete mesh;
+  if (mesh) ebgpu::mesh_release(mesh);
   return EBG_OK;
 }
 
@@ -338,10 +229,7 @@ ebg_status ebg_query_sdf(const ebg_mesh* cm, const float* d_pts, uint64_t n, flo
 
   const uint32_t* idx = nullptr;
   if (o.presort && n > 1) {
-    EBG_HIP(ebgpu::launch_bbox_f32(d_pts, n, ws.box.as<float>(), ws.box.as<float>() + 8, s));
-    EBG_HIP(ebgpu::launch_codes_f32(d_pts, n, ws.box.as<float>(), ws.codes.as<uint64_t>(), s));
-    EBG_HIP(ebgpu::radix_sort_pairs(ws.codes.as<uint64_t>(), nullptr, ws.codes_sorted.as<uint64_t>(),
-                                    ws.perm.as<uint32_t>(), n, 63, ws.sort_scratch.p, s));
+    EBG_HIP(ws.presort(d_pts, n, s));
     idx = ws.perm.as<uint32_t>();
   }
 

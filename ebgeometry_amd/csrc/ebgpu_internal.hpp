@@ -92,8 +92,8 @@ struct alignas(64) PrimRec
 {
   uint32_t type, mesh, pad0, pad1;
   float    a[6];  // sphere: centre xyz, radius; box: lo xyz, hi xyz
-  float    A[9];  // mesh instance: local point = A p + b, value = s * mesh(local)
-  float    b[3];
+  float    A[9];  // mesh instance: local point = A (p - t) with A = M / s, value = s * mesh(local)
+  float    t[3];
   float    s;
   float    bv[6]; // world bounding box (lo, hi)
   float    pad2[3];
@@ -103,8 +103,8 @@ static_assert(sizeof(PrimRec) == 128, "PrimRec layout");
 struct alignas(64) PrimRec64 // FP64 copy of the same parameters (near-surface recheck)
 {
   double a[6];
-  double A[9];
-  double b[3];
+  double A[9]; // M itself: local point = M (p - t) / s
+  double t[3];
   double s;
   double pad[5];
 };

@@ -1,0 +1,149 @@
+// Host-side plumbing shared by the C-ABI translation units: error reporting, device buffers,
+// the per-handle query workspace and the mesh handle itself.
+#pragma once
+
+#include "../../include/ebgpu.h"
+#include "kernels.hpp"
+#include "mlp.hpp"
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <atomic>
+#include <cstddef>
+#include <mutex>
+#include <string>
+#include <vector>
+
+namespace ebgpu {
+
+void set_last_error(const std::string& s);
+
+inline ebg_status fail(ebg_status s, const std::string& msg)
+{
+  set_last_error(msg);
+  return s;
+}
+
+#define EBG_HIP(call)                                                                                        \
+  do {                                                                                                       \
+    const hipError_t e_ = (call);                                                                            \
+    if (e_ != hipSuccess)                                                                                    \
+      return ::ebgpu::fail(e_ == hipErrorOutOfMemory ? EBG_OOM : EBG_HIP_ERROR,                             \
+                           std::string(#call) + ": " + hipGetErrorString(e_));                               \
+  } while (0)
+
+struct DevBuf
+{
+  void*       p     = nullptr;
+  std::size_t bytes = 0;
+  DevBuf()              = default;
+  DevBuf(const DevBuf&) = delete;
+  DevBuf& operator=(const DevBuf&) = delete;
+  ~DevBuf() { release(); }
+  void release()
+  {
+    if (p) (void)hipFree(p);
+    p     = nullptr;
+    bytes = 0;
+  }
+  hipError_t reserve(std::size_t b)
+  {
+    if (b <= bytes) return hipSuccess;
+    release();
+    hipError_t e = hipMalloc(&p, b);
+    if (e == hipSuccess) bytes = b;
+    else p = nullptr;
+    return e;
+  }
+  template <class T>
+  T* as() const
+  {
+    return static_cast<T*>(p);
+  }
+};
+
+template <class T>
+hipError_t upload(DevBuf& b, const std::vector<T>& v)
+{
+  const std::size_t bytes = std::max<std::size_t>(64, v.size() * sizeof(T));
+  hipError_t        e     = b.reserve(bytes);
+  if (e != hipSuccess) return e;
+  if (!v.empty()) e = hipMemcpy(b.p, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice);
+  return e;
+}
+
+// Query workspace, grown on demand.
+struct Workspace
+{
+  DevBuf   codes, codes_sorted, perm, sort_scratch, box, lists, seeds;
+  uint64_t cap = 0;
+
+  hipError_t ensure(uint64_t n)
+  {
+    if (n <= cap && cap) return hipSuccess;
+    const uint64_t m = std::max<uint64_t>(n, 1024);
+    hipError_t     e;
+    if ((e = codes.reserve(8 * m)) != hipSuccess) return e;
+    if ((e = codes_sorted.reserve(8 * m)) != hipSuccess) return e;
+    if ((e = perm.reserve(4 * m)) != hipSuccess) return e;
+    if ((e = sort_scratch.reserve(radix_sort_scratch_bytes(m))) != hipSuccess) return e;
+    if ((e = box.reserve(sizeof(double) * 6 * (kBboxParts + 1))) != hipSuccess) return e;
+    // recheck list, fallback list, 2 counters (+pad)
+    if ((e = lists.reserve(8 * m + 256)) != hipSuccess) return e;
+    if ((e = seeds.reserve(4 * m)) != hipSuccess) return e;
+    cap = m;
+    return hipSuccess;
+  }
+  uint32_t* recheck_list() const { return lists.as<uint32_t>(); }
+  uint32_t* fallback_list() const { return lists.as<uint32_t>() + cap; }
+  uint32_t* counters() const { return lists.as<uint32_t>() + 2 * cap; }
+
+  // Morton presort of n device points: returns the permutation (valid until the next call).
+  hipError_t presort(const float* d_pts, uint64_t n, hipStream_t s)
+  {
+    hipError_t e;
+    if ((e = launch_bbox_f32(d_pts, n, box.as<float>(), box.as<float>() + 8, s)) != hipSuccess) return e;
+    if ((e = launch_codes_f32(d_pts, n, box.as<float>(), codes.as<uint64_t>(), s)) != hipSuccess) return e;
+    return radix_sort_pairs(codes.as<uint64_t>(), nullptr, codes_sorted.as<uint64_t>(), perm.as<uint32_t>(), n,
+                            63, sort_scratch.p, s);
+  }
+};
+
+} // namespace ebgpu
+
+// The mesh handle. Scenes that instance a mesh hold a reference: the handle (and its device
+// memory) lives until ebg_mesh_destroy and every such scene have released it.
+struct ebg_mesh
+{
+  int                    K = 4, leaf = 4, banks = 1, stack_need = 0;
+  std::size_t            nv = 0, nt = 0, num_nodes = 0, num_leaves = 0;
+  double                 lo[3]{}, hi[3]{}, diag = 0;
+  double                 build_seconds = 0;
+  ebgpu::DevBuf          nodes, tris, tripos, feat, vn32, en32, vn64, en64, fn64;
+  ebgpu::DevBuf          area_cdf; // normalised cumulative triangle area, leaf order (training samples)
+  std::vector<float>     host_xyz; // vertex positions (scene instance bounds)
+  ebgpu::MeshDev         dev{};
+  ebgpu::MlpTrainBuffers train;
+  std::size_t            device_bytes = 0;
+  std::recursive_mutex   mu; // training runs queries while holding it
+  ebgpu::Workspace       ws;
+  hipEvent_t             ev0 = nullptr, ev1 = nullptr;
+  bool                   timed = false;
+  ebgpu::Mlp             mlp;
+  std::atomic<int>       refs{1};
+
+  ~ebg_mesh()
+  {
+    if (ev0) (void)hipEventDestroy(ev0);
+    if (ev1) (void)hipEventDestroy(ev1);
+  }
+};
+
+namespace ebgpu {
+inline void mesh_retain(ebg_mesh* m) { m->refs.fetch_add(1); }
+inline void mesh_release(ebg_mesh* m)
+{
+  if (m->refs.fetch_sub(1) == 1) delete m;
+}
+} // namespace ebgpu

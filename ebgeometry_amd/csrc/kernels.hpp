@@ -33,6 +33,13 @@ hipError_t launch_sdf32(const MeshDev& m, int K, int banks, const QueryArgs& a, 
 // double-precision lane kernel; seeded from the FP32 result (bound |d| + slack).
 hipError_t launch_sdf64(const MeshDev& m, int K, int banks, const QueryArgs& a, hipStream_t s);
 
+// CSG union scenes (scene.hip). KM = branching of the instanced meshes (all equal); banks =
+// stack capacity / 64 for both the scene tree and the mesh trees. Work counters: [0] interior
+// nodes, [1] triangles, [4]/[5] their wave-level fetches, [6] primitives, [7] wave-level.
+hipError_t launch_scene32(const SceneDev& sc, int KM, int banks, const QueryArgs& a, bool count, hipStream_t s);
+// FP64 pass over a.idx / a.count_ptr, seeded with |v32| (1 + slack_rel) + slack_abs.
+hipError_t launch_scene64(const SceneDev& sc, int KM, int banks, const QueryArgs& a, hipStream_t s);
+
 // util.hip
 hipError_t launch_fill_u32(uint32_t* p, uint64_t n, uint32_t v, hipStream_t s);
 hipError_t launch_add_counts(unsigned long long* work, const uint32_t* recheck, const uint32_t* fallback, hipStream_t s);

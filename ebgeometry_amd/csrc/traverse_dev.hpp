@@ -313,6 +313,42 @@ __device__ __forceinline__ void wave_append(bool pred, uint32_t idx, uint32_t* l
 // the packet traversal
 // ---------------------------------------------------------------------------------------
 
+// Interior node: test the K child boxes and push the ones some lane needs, ranked so that the
+// farthest lies deepest and the nearest (by wave-minimum box distance) is popped next.
+template <int K, int NB, class R>
+__device__ __forceinline__ void push_children(const Node<K>& nd, R px, R py, R pz, R bound, LaneStack<NB>& st,
+                                              int& top, int lane)
+{
+  uint32_t key[K];
+  uint64_t msk[K];
+#pragma unroll
+  for (int k = 0; k < K; k++) {
+    msk[k] = 0;
+    key[k] = 0xFFFFFFFFu;
+    if (nd.cnt[k] != kInvalid) {
+      const R    d2 = box_d2<R>(px, py, pz, nd.lox[k], nd.loy[k], nd.loz[k], nd.hix[k], nd.hiy[k], nd.hiz[k]);
+      const bool nk = d2 <= bound;
+      msk[k]        = __ballot(nk);
+      if (msk[k]) key[k] = wave_min_u32(nk ? __float_as_uint(float(d2)) : 0xFFFFFFFFu);
+    }
+  }
+  int pushed = 0;
+#pragma unroll
+  for (int k = 0; k < K; k++) {
+    if (!msk[k]) continue;
+    int rank = 0;
+#pragma unroll
+    for (int j = 0; j < K; j++) {
+      if (j == k || !msk[j]) continue;
+      rank += (key[j] > key[k] || (key[j] == key[k] && j < k)) ? 1 : 0;
+    }
+    Entry c{nd.ref[k], nd.cnt[k], nd.lox[k], nd.loy[k], nd.loz[k], nd.hix[k], nd.hiy[k], nd.hiz[k]};
+    st.put(top + rank, c, lane);
+    pushed++;
+  }
+  top += pushed;
+}
+
 // R = float: FP32 pass on TriRec records. R = double: FP64 recheck on raw positions.
 template <int K, int NB, class R, bool COUNT>
 __device__ __forceinline__ void traverse(const MeshDev& m, R px, R py, R pz, R bound, R& best, uint32_t& btri,
@@ -333,39 +369,8 @@ __device__ __forceinline__ void traverse(const MeshDev& m, R px, R py, R pz, R b
     if (__ballot(need) == 0) continue;
 
     if (e.cnt == 0u) {
-      // interior node: test the K children
-      const uint32_t ni = uni(e.ref);
-      const Node<K>& nd = nodes[ni];
       if (COUNT) { n_inner += need ? 1u : 0u; w_inner++; }
-      uint32_t key[K];
-      uint64_t msk[K];
-#pragma unroll
-      for (int k = 0; k < K; k++) {
-        msk[k] = 0;
-        key[k] = 0xFFFFFFFFu;
-        if (nd.cnt[k] != kInvalid) {
-          const R    d2 = box_d2<R>(px, py, pz, nd.lox[k], nd.loy[k], nd.loz[k], nd.hix[k], nd.hiy[k], nd.hiz[k]);
-          const bool nk = d2 <= bound;
-          msk[k]        = __ballot(nk);
-          if (msk[k]) key[k] = wave_min_u32(nk ? __float_as_uint(float(d2)) : 0xFFFFFFFFu);
-        }
-      }
-      // rank the needed children: farthest deepest, nearest on top
-      int pushed = 0;
-#pragma unroll
-      for (int k = 0; k < K; k++) {
-        if (!msk[k]) continue;
-        int rank = 0;
-#pragma unroll
-        for (int j = 0; j < K; j++) {
-          if (j == k || !msk[j]) continue;
-          rank += (key[j] > key[k] || (key[j] == key[k] && j < k)) ? 1 : 0;
-        }
-        Entry c{nd.ref[k], nd.cnt[k], nd.lox[k], nd.loy[k], nd.loz[k], nd.hix[k], nd.hiy[k], nd.hiz[k]};
-        st.put(top + rank, c, lane);
-        pushed++;
-      }
-      top += pushed;
+      push_children<K, NB, R>(nodes[uni(e.ref)], px, py, pz, bound, st, top, lane);
     }
     else {
       // leaf: a triangle is evaluated by the whole wavefront (keeping the minimum is exact for

@@ -142,6 +142,59 @@ ebg_status ebg_query_sdf_host(const ebg_mesh* mesh, const float* pts, uint64_t n
 double ebg_mesh_last_traversal_ms(const ebg_mesh* mesh);
 
 /* ---------------------------------------------------------------------------------------- */
+/* CSG union scenes -- BVHUnionIF(primitives, bounding volumes) and its value()               */
+/* (Source/EBGeometry_CSG.hpp:401-440, CSGImplem.hpp:51-95, 466-501) over SphereSDF and      */
+/* BoxSDF (AnalyticDistanceFunctions.hpp:137-423) and mesh instances                          */
+/* Translate(Rotate(Rotate(Rotate(Scale(mesh, s), ax, 0), ay, 1), az, 2), t)                  */
+/* (Transform.hpp:44-70, TransformImplem.hpp:140-270).                                        */
+/* ---------------------------------------------------------------------------------------- */
+
+typedef enum ebg_prim_type
+{
+  EBG_PRIM_SPHERE = 0, /* p[0..2] centre, p[3] radius                                         */
+  EBG_PRIM_BOX    = 1, /* p[0..2] low corner, p[3..5] high corner (axis-aligned)              */
+  EBG_PRIM_MESH   = 2  /* meshes[mesh] under the similarity map below                          */
+} ebg_prim_type;
+
+typedef struct ebg_prim
+{
+  int32_t type;          /* ebg_prim_type                                                    */
+  int32_t mesh;          /* EBG_PRIM_MESH: index into the scene's mesh array                 */
+  double  p[6];          /* sphere / box parameters                                          */
+  double  translate[3];  /* EBG_PRIM_MESH: value(x) = scale * mesh(M (x - t) / scale), the   */
+  double  matrix[9];     /*   form every chain of Translate / Rotate / Scale takes. M is     */
+                         /*   orthonormal, row-major (all zeros = identity); for            */
+                         /*   Translate(Rz(Ry(Rx(Scale(mesh))))) it is Rx Ry Rz with R_axis */
+                         /*   the point map of RotateIF::value.                             */
+  double  scale;         /*   uniform scale > 0                                               */
+  double  bv_lo[3];      /* bounding volume of the primitive, as passed to BVHUnionIF; it    */
+  double  bv_hi[3];      /*   must contain the primitive. bv_lo[0] > bv_hi[0] = compute it:  */
+                         /*   exact for spheres and boxes, the transformed mesh bbox corners */
+                         /*   for mesh instances.                                            */
+} ebg_prim;
+
+/* Build a scene over n primitives. meshes[0..num_meshes) are retained by the scene (a mesh
+ * handle may be destroyed by the caller at any time; its memory lives until the last scene
+ * using it is destroyed). All meshes must have the same branching factor. opts->leaf_size
+ * sets primitives per BVH leaf (default 4); builder must be EBG_BUILD_SAH. */
+ebg_status ebg_scene_create(const ebg_prim* prims, uint64_t n, ebg_mesh* const* meshes, uint64_t num_meshes,
+                            const ebg_build_opts* opts, ebg_scene** out);
+ebg_status ebg_scene_destroy(ebg_scene* scene);
+/* World bounding box (lo, hi) of the scene and its diagonal. */
+ebg_status ebg_scene_bounds(const ebg_scene* scene, double* lo3, double* hi3, double* diag);
+
+/* Batched BVHUnionIF::value: d_pts n float32 xyz, d_value n float32 (device pointers, async on
+ * opts->stream). Honours presort, fp64_recheck / recheck_eps (relative to the scene diagonal),
+ * fp64_all, stream and work ([0] interior nodes incl. mesh nodes, [1] triangles, [2] rechecked,
+ * [4]/[5] wave-level fetches, [6] primitives evaluated, [7] wave-level primitive fetches).
+ * learned_bound and seed_radius are not supported for scenes (EBG_INVALID_ARG). */
+ebg_status ebg_scene_query(const ebg_scene* scene, const float* d_pts, uint64_t n, float* d_value,
+                           const ebg_query_opts* opts);
+ebg_status ebg_scene_query_host(const ebg_scene* scene, const float* pts, uint64_t n, float* value,
+                                const ebg_query_opts* opts);
+double     ebg_scene_last_traversal_ms(const ebg_scene* scene);
+
+/* ---------------------------------------------------------------------------------------- */
 /* Space-filling curve -- SFC::computeBins<T>, Morton::encode, SFC::order                      */
 /* (Source/EBGeometry_SFCImplem.hpp:31-58, 228-300). Device pointers, async on stream.        */
 /* Bit-exact with the reference; equal codes are ordered by increasing index (stable).        */
