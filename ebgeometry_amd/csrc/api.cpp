@@ -54,6 +54,7 @@ static ebg_status upload_mesh(ebg_mesh* m, ebgpu::PackedMesh& pm)
   EBG_HIP(upload(m->nodes, pm.nodes));
   EBG_HIP(upload(m->tris, pm.tris));
   EBG_HIP(upload(m->tripos, pm.tripos));
+  EBG_HIP(upload(m->tsph, pm.tsph));
   EBG_HIP(upload(m->feat, pm.feat));
   EBG_HIP(upload(m->vn32, pm.vn32));
   EBG_HIP(upload(m->en32, pm.en32));
@@ -96,6 +97,7 @@ static ebg_status upload_mesh(ebg_mesh* m, ebgpu::PackedMesh& pm)
   d.nodes           = m->nodes.p;
   d.tris            = m->tris.as<ebgpu::TriRec>();
   d.tripos          = m->tripos.as<float>();
+  d.tsph            = m->tsph.as<float>();
   d.feat            = m->feat.as<uint32_t>();
   d.vn32            = m->vn32.as<float>();
   d.en32            = m->en32.as<float>();
@@ -108,7 +110,7 @@ static ebg_status upload_mesh(ebg_mesh* m, ebgpu::PackedMesh& pm)
   }
   d.diag          = float(pm.diag);
   d.num_tris      = uint32_t(m->nt);
-  m->device_bytes = m->nodes.bytes + m->tris.bytes + m->tripos.bytes + m->feat.bytes + m->vn32.bytes +
+  m->device_bytes = m->nodes.bytes + m->tris.bytes + m->tripos.bytes + m->tsph.bytes + m->feat.bytes + m->vn32.bytes +
                     m->en32.bytes + m->vn64.bytes + m->en64.bytes + m->fn64.bytes;
   return EBG_OK;
 }

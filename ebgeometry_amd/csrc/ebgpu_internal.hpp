@@ -8,6 +8,8 @@
 //                                point-triangle kernel needs precomputed (edges, face normal,
 //                                edge x normal, 1/|edge|^2). Also read with scalar loads.
 //   tripos  float[12*num_tris]   raw vertex positions (v0,v1,v2,pad) for the FP64 path.
+//   tsph    float4[num_tris+16]  per-triangle bounding spheres, contiguous per leaf so that a
+//                                leaf's culling data arrives in one scalar round trip.
 //   feat    uint32[8*num_tris]   vertex ids, half-edge ids, original face id: lets the sign
 //                                step fetch the one pseudo-normal a query needs.
 //   vn32/vn64, en32/en64         vertex / half-edge pseudo-normals (float4 / double4).
@@ -21,6 +23,7 @@ namespace ebgpu {
 
 constexpr uint32_t kInvalid = 0xFFFFFFFFu;
 constexpr int      kMaxLeaf = 16;
+constexpr int      kSphPad  = 16; // zero sphere entries after the last triangle (chunked leaf loads)
 
 // Feature codes of the closest point on a triangle.
 enum Feature : uint32_t
@@ -67,6 +70,7 @@ struct MeshDev
   const void*     nodes;
   const TriRec*   tris;
   const float*    tripos; // 12 per triangle
+  const float*    tsph;   // float4 per triangle: bounding sphere (centre, radius), leaf order
   const uint32_t* feat;   // 8 per triangle: vid0..2, he0..2, face, 0
   const float*    vn32;   // float4 per vertex
   const float*    en32;   // float4 per half-edge (3*face + i)
@@ -77,6 +81,7 @@ struct MeshDev
   float           diag;
   uint32_t        num_tris;
 };
+static_assert(sizeof(MeshDev) % 16 == 0, "MeshDev is read with 16-byte scalar loads");
 
 // ---------------------------------------------------------------------------------------------
 // CSG union scenes: primitives in BVH leaf order

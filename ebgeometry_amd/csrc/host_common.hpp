@@ -120,7 +120,7 @@ struct ebg_mesh
   std::size_t            nv = 0, nt = 0, num_nodes = 0, num_leaves = 0;
   double                 lo[3]{}, hi[3]{}, diag = 0;
   double                 build_seconds = 0;
-  ebgpu::DevBuf          nodes, tris, tripos, feat, vn32, en32, vn64, en64, fn64;
+  ebgpu::DevBuf          nodes, tris, tripos, tsph, feat, vn32, en32, vn64, en64, fn64;
   ebgpu::DevBuf          area_cdf; // normalised cumulative triangle area, leaf order (training samples)
   std::vector<float>     host_xyz; // vertex positions (scene instance bounds)
   ebgpu::MeshDev         dev{};

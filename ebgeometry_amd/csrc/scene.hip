@@ -70,12 +70,12 @@ __device__ __forceinline__ void scene_traverse(const SceneDev& sc, R px, R py, R
 
     if (e.cnt == 0u) {
       if (COUNT) { c.n_inner += need ? 1u : 0u; c.w_inner++; }
-      push_children<4, NB, R>(nodes[uni(e.ref)], px, py, pz, bound, st, top, lane);
+      push_children<4, NB, R>(sload_record(nodes + uni(e.ref)), px, py, pz, bound, st, top, lane);
       continue;
     }
     const uint32_t first = uni(e.ref), cnt = uni(e.cnt);
     for (uint32_t i = first; i < first + cnt; i++) {
-      const PrimRec& pr = sc.prims[i];
+      const PrimRec  pr = sload_record(sc.prims + i);
       const bool     pn = box_d2<R>(px, py, pz, pr.bv[0], pr.bv[1], pr.bv[2], pr.bv[3], pr.bv[4], pr.bv[5]) <= bound;
       if (__ballot(pn) == 0) continue;
       if (COUNT) { c.n_prims += pn ? 1u : 0u; c.w_prims++; }
@@ -84,20 +84,20 @@ __device__ __forceinline__ void scene_traverse(const SceneDev& sc, R px, R py, R
       if (type == kPrimSphere) {
         if constexpr (sizeof(R) == 4) val = sphere_value<R>(px, py, pz, pr.a[0], pr.a[1], pr.a[2], pr.a[3]);
         else {
-          const PrimRec64& q = sc.prims64[i];
-          val                = sphere_value<R>(px, py, pz, q.a[0], q.a[1], q.a[2], q.a[3]);
+          const PrimRec64 q = sload_record(sc.prims64 + i);
+          val               = sphere_value<R>(px, py, pz, q.a[0], q.a[1], q.a[2], q.a[3]);
         }
       }
       else if (type == kPrimBox) {
         if constexpr (sizeof(R) == 4)
           val = box_value<R>(px, py, pz, pr.a[0], pr.a[1], pr.a[2], pr.a[3], pr.a[4], pr.a[5]);
         else {
-          const PrimRec64& q = sc.prims64[i];
-          val                = box_value<R>(px, py, pz, q.a[0], q.a[1], q.a[2], q.a[3], q.a[4], q.a[5]);
+          const PrimRec64 q = sload_record(sc.prims64 + i);
+          val               = box_value<R>(px, py, pz, q.a[0], q.a[1], q.a[2], q.a[3], q.a[4], q.a[5]);
         }
       }
       else {
-        const MeshDev& md = sc.meshes[uni(pr.mesh)];
+        const MeshDev  md = sload_record(sc.meshes + uni(pr.mesh));
         R              qx, qy, qz, s;
         if constexpr (sizeof(R) == 4) {
           // pr.A = M / s
@@ -108,8 +108,8 @@ __device__ __forceinline__ void scene_traverse(const SceneDev& sc, R px, R py, R
           s  = pr.s;
         }
         else {
-          const PrimRec64& q  = sc.prims64[i];
-          const double     ux = px - q.t[0], uy = py - q.t[1], uz = pz - q.t[2];
+          const PrimRec64 q  = sload_record(sc.prims64 + i);
+          const double    ux = px - q.t[0], uy = py - q.t[1], uz = pz - q.t[2];
           s  = q.s;
           qx = (q.A[0] * ux + q.A[1] * uy + q.A[2] * uz) / s;
           qy = (q.A[3] * ux + q.A[4] * uy + q.A[5] * uz) / s;

@@ -51,6 +51,59 @@ __device__ __forceinline__ uint32_t xcd_remap(uint32_t b, uint32_t nb)
 }
 
 // ---------------------------------------------------------------------------------------
+// wave-uniform record loads
+// ---------------------------------------------------------------------------------------
+
+template <int N>
+struct SWords
+{
+  uint32_t w[N];
+};
+
+// Load N dwords (16-byte aligned, wave-uniform address) of read-only data into SGPRs: every
+// load is issued back to back and waited for once. The loads go through the constant address
+// space (the tree and triangle arrays are not written while a query kernel runs), so they are
+// scalar loads regardless of stores elsewhere in the kernel; the empty asm statements take each
+// value as an SGPR operand at this point, which keeps the compiler from sinking the loads into
+// the branches that use them (each sunk load would cost its own memory round trip).
+template <int N>
+__device__ __forceinline__ SWords<N> sload(const void* p)
+{
+  static_assert(N % 4 == 0, "sload: whole 16-byte units");
+  typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
+  typedef const __attribute__((address_space(4))) u32x4 ConstU4;
+  SWords<N>      r;
+  ConstU4*       q = (ConstU4*)p;
+#pragma unroll
+  for (int i = 0; i < N / 4; i++) {
+    const u32x4 v = q[i];
+    r.w[4 * i]     = v.x;
+    r.w[4 * i + 1] = v.y;
+    r.w[4 * i + 2] = v.z;
+    r.w[4 * i + 3] = v.w;
+  }
+#pragma unroll
+  for (int i = 0; i < N; i += 4)
+    asm volatile("" ::"s"(r.w[i]), "s"(r.w[i + 1]), "s"(r.w[i + 2]), "s"(r.w[i + 3]));
+  return r;
+}
+
+template <class T, int N>
+__device__ __forceinline__ T as_record(const SWords<N>& s)
+{
+  static_assert(sizeof(T) == 4 * N, "as_record: size mismatch");
+  T t;
+  __builtin_memcpy(&t, s.w, sizeof(T));
+  return t;
+}
+
+template <class T>
+__device__ __forceinline__ T sload_record(const T* p)
+{
+  return as_record<T>(sload<sizeof(T) / 4>(p));
+}
+
+// ---------------------------------------------------------------------------------------
 // VGPR-lane stack
 // ---------------------------------------------------------------------------------------
 
@@ -370,36 +423,53 @@ __device__ __forceinline__ void traverse(const MeshDev& m, R px, R py, R pz, R b
 
     if (e.cnt == 0u) {
       if (COUNT) { n_inner += need ? 1u : 0u; w_inner++; }
-      push_children<K, NB, R>(nodes[uni(e.ref)], px, py, pz, bound, st, top, lane);
+      push_children<K, NB, R>(sload_record(nodes + uni(e.ref)), px, py, pz, bound, st, top, lane);
     }
     else {
       // leaf: a triangle is evaluated by the whole wavefront (keeping the minimum is exact for
       // every lane) unless its bounding sphere lies beyond every lane's bound. Inactive lanes
-      // have bound < 0, so rb is NaN and they never ask for a triangle.
+      // have bound < 0, so rb is NaN and they never ask for a triangle. The spheres of up to 8
+      // triangles arrive in one scalar round trip (the array is padded past the last leaf),
+      // then each needed triangle record in one more.
       const uint32_t first = uni(e.ref), cnt = uni(e.cnt);
       const R        rb    = sqrt(bound);
-      for (uint32_t t = first; t < first + cnt; t++) {
-        const TriRec& rec = m.tris[t];
-        const R       sx = px - R(rec.sc[0]), sy = py - R(rec.sc[1]), sz = pz - R(rec.sc[2]);
-        const R       lim = rb + R(rec.sr);
-        const bool    tn  = sx * sx + sy * sy + sz * sz <= lim * lim;
-        if (__ballot(tn) == 0) continue;
-        if (COUNT) {
-          n_tris += tn ? 1u : 0u;
-          w_tris++;
+      for (uint32_t c0 = 0; c0 < cnt; c0 += 8) {
+        const SWords<32> sp   = sload<32>(m.tsph + 4 * (size_t)(first + c0));
+        uint32_t         need = 0;
+#pragma unroll
+        for (int i = 0; i < 8; i++) {
+          if (c0 + i < cnt) {
+            const R    sx  = px - R(__uint_as_float(sp.w[4 * i])), sy = py - R(__uint_as_float(sp.w[4 * i + 1])),
+                       sz  = pz - R(__uint_as_float(sp.w[4 * i + 2]));
+            const R    lim = rb + R(__uint_as_float(sp.w[4 * i + 3]));
+            const bool tn  = sx * sx + sy * sy + sz * sz <= lim * lim;
+            if (COUNT) n_tris += tn ? 1u : 0u;
+            need |= __ballot(tn) ? (1u << i) : 0u;
+          }
         }
-        R        d2;
-        uint32_t f;
-        if constexpr (sizeof(R) == 4) {
-          tri_eval32(m.tris[t], px, py, pz, d2, f);
-        }
-        else {
-          tri_eval64(m.tripos + 12 * (size_t)t, m.fn64 + 4 * (size_t)t, px, py, pz, d2, f);
-        }
-        if (d2 < best) {
-          best  = d2;
-          btri  = t;
-          bfeat = f;
+        while (need) {
+          const uint32_t t = first + c0 + (uint32_t)__builtin_ctz(need);
+          need &= need - 1u;
+          if (COUNT) w_tris++;
+          R        d2;
+          uint32_t f;
+          if constexpr (sizeof(R) == 4) {
+            tri_eval32(sload_record(m.tris + t), px, py, pz, d2, f);
+          }
+          else {
+            const SWords<12> pos = sload<12>(m.tripos + 12 * (size_t)t);
+            const SWords<8>  fn  = sload<8>(m.fn64 + 4 * (size_t)t);
+            double           fnd[4];
+            __builtin_memcpy(fnd, fn.w, sizeof(fnd));
+            float posf[12];
+            __builtin_memcpy(posf, pos.w, sizeof(posf));
+            tri_eval64(posf, fnd, px, py, pz, d2, f);
+          }
+          if (d2 < best) {
+            best  = d2;
+            btri  = t;
+            bfeat = f;
+          }
         }
       }
       bound = min(bound, best);
