@@ -9,9 +9,10 @@ mirror of the reference API over that ABI.
 """
 from .sdf import (Build, ImplicitFunction, SignedDistanceFunction, TriMeshSDF, morton_codes, morton_sort,
                   pseudo_normals)
+from .batch import BatchPlan, MeshBatch
 from .csg import (AABB, BoxSDF, BVHUnion, BVHUnionIF, Rotate, RotateIF, Scale, ScaleIF, SphereSDF, Translate,
                   TranslateIF)
 
 __all__ = ["Build", "ImplicitFunction", "SignedDistanceFunction", "TriMeshSDF", "morton_codes", "morton_sort",
            "pseudo_normals", "AABB", "BoxSDF", "BVHUnion", "BVHUnionIF", "Rotate", "RotateIF", "Scale", "ScaleIF",
-           "SphereSDF", "Translate", "TranslateIF"]
+           "SphereSDF", "Translate", "TranslateIF", "MeshBatch", "BatchPlan"]

@@ -45,6 +45,20 @@ class Prim(C.Structure):
                 ("bv_hi", C.c_double * 3)]
 
 
+class BatchOpts(C.Structure):
+    _fields_ = [("autotune", _i32), ("branching", _i32), ("leaf_size", _i32), ("threads", _i32)]
+
+
+class BatchInfo(C.Structure):
+    _fields_ = [("num_meshes", _u64), ("num_triangles", _u64), ("branching", _i32), ("leaf_size", _i32),
+                ("stack_capacity", _i32), ("tuned", _i32), ("tune_ms", C.c_double * 4), ("build_seconds", C.c_double)]
+
+
+class BatchItem(C.Structure):
+    _fields_ = [("mesh", C.c_uint32), ("reserved", C.c_uint32), ("n", _u64), ("d_pts", _vp), ("d_dist", _vp),
+                ("d_tri", _vp)]
+
+
 # (name, restype, argtypes) of every exported entry point
 SIGNATURES = [
     ("ebg_last_error", C.c_char_p, []),
@@ -65,6 +79,17 @@ SIGNATURES = [
     ("ebg_scene_query", C.c_int, [_vp, _vp, _u64, _vp, C.POINTER(QueryOpts)]),
     ("ebg_scene_query_host", C.c_int, [_vp, _vp, _u64, _vp, C.POINTER(QueryOpts)]),
     ("ebg_scene_last_traversal_ms", C.c_double, [_vp]),
+    ("ebg_batch_create", C.c_int, [_vp, _vp, _vp, _vp, _u64, C.POINTER(BatchOpts), C.POINTER(_vp)]),
+    ("ebg_batch_destroy", C.c_int, [_vp]),
+    ("ebg_batch_get_info", C.c_int, [_vp, C.POINTER(BatchInfo)]),
+    ("ebg_batch_mesh", C.c_int, [_vp, _u64, C.POINTER(_vp)]),
+    ("ebg_batch_plan_create", C.c_int, [_vp, C.POINTER(BatchItem), _u64, C.POINTER(QueryOpts), _i32, _i32,
+                                        C.POINTER(_vp)]),
+    ("ebg_batch_plan_launch", C.c_int, [_vp, _vp]),
+    ("ebg_batch_plan_destroy", C.c_int, [_vp]),
+    ("ebg_batch_plan_workgroup", _i32, [_vp]),
+    ("ebg_batch_plan_last_traversal_ms", C.c_double, [_vp]),
+    ("ebg_batch_query", C.c_int, [_vp, C.POINTER(BatchItem), _u64, C.POINTER(QueryOpts)]),
     ("ebg_mlp_opts_default", None, [C.POINTER(MlpOpts)]),
     ("ebg_mlp_train", C.c_int, [_vp, C.POINTER(MlpOpts), C.POINTER(C.c_float)]),
     ("ebg_mlp_predict", C.c_int, [_vp, _vp, _u64, _vp, _vp]),

@@ -34,8 +34,10 @@ SOURCES = [
     ("mesh_build.cpp", HOST),
     ("api.cpp", HOST),
     ("scene_api.cpp", HOST),
+    ("batch_api.cpp", HOST),
     ("traverse.hip", DEVICE),
     ("scene.hip", DEVICE),
+    ("batch.hip", DEVICE),
     ("morton.hip", DEVICE),
     ("util.hip", DEVICE),
     ("mlp.hip", DEVICE),

@@ -1,0 +1,207 @@
+// Many meshes, one traversal launch (gfx950).
+//
+// The queries of all items (item = one query array against one mesh) form one flat index
+// space. The pipeline:
+//   1. keys: item id in the high bits, the query's Morton code (relative to a box around the
+//      item's mesh) in the low bits; one stable radix sort then orders every item's queries along
+//      the curve and keeps the items apart;
+//   2. batch32: a packet = up to 64 consecutive (sorted) queries of one item, one per lane of a
+//      wavefront; a packet table (built on the host, most expensive meshes first) maps each
+//      wavefront to (item, first query). The wavefront loads its item and mesh records with
+//      scalar loads and runs the same packet traversal as the single-mesh kernel, so every
+//      wavefront works on one mesh while different wavefronts of a launch work on different
+//      meshes;
+//   3. batch64: packets that hold near-surface results were listed (one entry per packet with
+//      the lane mask); one wavefront per listed packet redoes those lanes in FP64.
+#include "kernels.hpp"
+#include "traverse_dev.hpp"
+
+#include <hip/hip_runtime.h>
+
+namespace ebgpu {
+
+namespace {
+
+using namespace dev;
+
+__device__ __forceinline__ uint64_t spread3b(uint64_t a)
+{
+  uint64_t b = a & 0x1fffffull;
+  b          = (b | b << 32) & 0x1f00000000ffffull;
+  b          = (b | b << 16) & 0x1f0000ff0000ffull;
+  b          = (b | b << 8) & 0x100f00f00f00f00full;
+  b          = (b | b << 4) & 0x10c30c30c30c30c3ull;
+  b          = (b | b << 2) & 0x1249249249249249ull;
+  return b;
+}
+
+// Sort key of flat query g: (item << 42) | top 42 bits of its 63-bit Morton code in the item's
+// box (bins clamped to 21 bits). Item found by binary search over the item bases.
+__global__ __launch_bounds__(256) void batch_keys_kernel(const BatchItemDev* __restrict__ items, uint32_t num_items,
+                                                         uint64_t total, uint64_t* __restrict__ keys)
+{
+  const uint64_t g = (uint64_t)blockIdx.x * 256u + threadIdx.x;
+  if (g >= total) return;
+  uint32_t lo = 0, hi = num_items; // items[lo].base <= g < items[hi].base
+  while (hi - lo > 1) {
+    const uint32_t mid = (lo + hi) >> 1;
+    if (items[mid].base <= g) lo = mid;
+    else hi = mid;
+  }
+  const BatchItemDev& it = items[lo];
+  const uint64_t      j  = g - it.base;
+  uint64_t            code = 0;
+#pragma unroll
+  for (int c = 0; c < 3; c++) {
+    float b = (it.pts[3 * j + c] - it.box_lo[c]) * it.box_inv[c];
+    b       = fminf(fmaxf(b, 0.0f), 2097151.0f); // NaN -> 0
+    code |= spread3b((uint64_t)b) << c;
+  }
+  keys[g] = ((uint64_t)lo << 42) | (code >> 21);
+}
+
+template <int K, int NB, bool COUNT>
+__global__ __launch_bounds__(256) void batch32_kernel(const BatchArgs a)
+{
+  const uint32_t wpb = blockDim.x >> 6;
+  const uint32_t p   = xcd_remap(blockIdx.x, gridDim.x) * wpb + (threadIdx.x >> 6);
+  if (p >= a.num_packets) return; // whole wavefront (p is wave-uniform)
+  const SWords<4>    pk = sload<4>(a.packets + 4 * (size_t)p);
+  const uint32_t     item = pk.w[0], first = pk.w[1];
+  const BatchItemDev it   = sload_record(a.items + item);
+  const MeshDev      m    = sload_record(a.meshes + it.mesh);
+
+  const uint32_t lane   = __lane_id();
+  const uint32_t j      = first + lane;
+  const bool     active = j < it.n;
+  uint32_t       qi     = 0;
+  float          px = 0.0f, py = 0.0f, pz = 0.0f, bound = -1.0f;
+  if (active) {
+    qi    = a.sorted ? (uint32_t)(a.sorted[it.base + j] - it.base) : j;
+    px    = it.pts[3 * (size_t)qi];
+    py    = it.pts[3 * (size_t)qi + 1];
+    pz    = it.pts[3 * (size_t)qi + 2];
+    bound = INFINITY;
+  }
+  float    best = INFINITY;
+  uint32_t btri = kInvalid, bfeat = 0, n_inner = 0, n_tris = 0, w_inner = 0, w_tris = 0;
+  traverse<K, NB, float, COUNT>(m, px, py, pz, bound, best, btri, bfeat, n_inner, n_tris, w_inner, w_tris);
+  const bool found = active && btri != kInvalid;
+  float      d     = 0.0f;
+  if (found) {
+    d            = resolve32(m, btri, bfeat, best, px, py, pz);
+    it.out[qi]   = d;
+    if (it.tri) it.tri[qi] = m.tris[btri].face;
+  }
+  if (a.recheck) {
+    const uint64_t mask = __ballot(found && fabsf(d) < it.recheck_abs);
+    if (mask && lane == 0) {
+      const uint32_t slot = atomicAdd(a.recheck_count, 1u);
+      a.recheck[slot]     = make_uint4(p, 0u, (uint32_t)mask, (uint32_t)(mask >> 32));
+    }
+  }
+  if (COUNT && a.work) {
+    const uint64_t si = wave_sum_u64(active ? (uint64_t)n_inner : 0ull);
+    const uint64_t st = wave_sum_u64(active ? (uint64_t)n_tris : 0ull);
+    if (lane == 0) {
+      atomicAdd(a.work, (unsigned long long)si);
+      atomicAdd(a.work + 1, (unsigned long long)st);
+      atomicAdd(a.work + 4, (unsigned long long)w_inner);
+      atomicAdd(a.work + 5, (unsigned long long)w_tris);
+    }
+  }
+}
+
+// One wavefront per listed packet; the listed lanes redo their query in FP64, seeded with
+// (|d32| (1 + 1e-4) + slack_abs)^2, which contains the FP64 minimum.
+template <int K, int NB>
+__global__ __launch_bounds__(64) void batch64_kernel(const BatchArgs a)
+{
+  if (blockIdx.x >= *a.recheck_count) return;
+  const SWords<4>    e    = sload<4>(a.recheck + blockIdx.x);
+  const uint32_t     p    = e.w[0];
+  const uint64_t     mask = (uint64_t)e.w[2] | ((uint64_t)e.w[3] << 32);
+  const SWords<4>    pk   = sload<4>(a.packets + 4 * (size_t)p);
+  const uint32_t     item = pk.w[0], first = pk.w[1];
+  const BatchItemDev it   = sload_record(a.items + item);
+  const MeshDev      m    = sload_record(a.meshes + it.mesh);
+
+  const uint32_t lane   = __lane_id();
+  const bool     active = (mask >> lane) & 1ull;
+  const uint32_t j      = first + lane;
+  uint32_t       qi     = 0;
+  double         px = 0.0, py = 0.0, pz = 0.0, bound = -1.0;
+  if (active) {
+    qi               = a.sorted ? (uint32_t)(a.sorted[it.base + j] - it.base) : j;
+    px               = it.pts[3 * (size_t)qi];
+    py               = it.pts[3 * (size_t)qi + 1];
+    pz               = it.pts[3 * (size_t)qi + 2];
+    const double r   = fabs((double)it.out[qi]) * (1.0 + 1e-4) + (double)it.slack_abs;
+    bound            = r * r;
+  }
+  double   best = INFINITY;
+  uint32_t btri = kInvalid, bfeat = 0, n_inner = 0, n_tris = 0, w_inner = 0, w_tris = 0;
+  const double seed2 = bound;
+  traverse<K, NB, double, false>(m, px, py, pz, bound, best, btri, bfeat, n_inner, n_tris, w_inner, w_tris);
+  if (active && btri != kInvalid && best <= seed2) {
+    it.out[qi] = (float)resolve64(m, btri, bfeat, best, px, py, pz);
+    if (it.tri) it.tri[qi] = m.feat[8 * (size_t)btri + 6];
+  }
+}
+
+template <int K, int NB>
+hipError_t launch32(const BatchArgs& a, int wg, bool count, hipStream_t s)
+{
+  const uint32_t wpb    = (uint32_t)wg / 64u;
+  const uint32_t blocks = (a.num_packets + wpb - 1) / wpb;
+  if (blocks == 0) return hipSuccess;
+  if (count) hipLaunchKernelGGL((batch32_kernel<K, NB, true>), dim3(blocks), dim3(wg), 0, s, a);
+  else hipLaunchKernelGGL((batch32_kernel<K, NB, false>), dim3(blocks), dim3(wg), 0, s, a);
+  return hipGetLastError();
+}
+
+template <int K, int NB>
+hipError_t launch64(const BatchArgs& a, hipStream_t s)
+{
+  if (a.num_packets == 0) return hipSuccess;
+  hipLaunchKernelGGL((batch64_kernel<K, NB>), dim3(a.num_packets), dim3(64), 0, s, a);
+  return hipGetLastError();
+}
+
+} // namespace
+
+hipError_t launch_batch_keys(const BatchItemDev* items, uint32_t num_items, uint64_t total, uint64_t* keys,
+                             hipStream_t s)
+{
+  if (total == 0) return hipSuccess;
+  hipLaunchKernelGGL(batch_keys_kernel, dim3((uint32_t)((total + 255) / 256)), dim3(256), 0, s, items, num_items,
+                     total, keys);
+  return hipGetLastError();
+}
+
+hipError_t launch_batch32(const BatchArgs& a, int K, int banks, int wg, bool count, hipStream_t s)
+{
+  if (wg != 64 && wg != 128 && wg != 256) return hipErrorInvalidValue;
+  if (K == 4) {
+    if (banks <= 1) return launch32<4, 1>(a, wg, count, s);
+    if (banks == 2) return launch32<4, 2>(a, wg, count, s);
+    return launch32<4, 4>(a, wg, count, s);
+  }
+  if (banks <= 1) return launch32<8, 1>(a, wg, count, s);
+  if (banks == 2) return launch32<8, 2>(a, wg, count, s);
+  return launch32<8, 4>(a, wg, count, s);
+}
+
+hipError_t launch_batch64(const BatchArgs& a, int K, int banks, hipStream_t s)
+{
+  if (K == 4) {
+    if (banks <= 1) return launch64<4, 1>(a, s);
+    if (banks == 2) return launch64<4, 2>(a, s);
+    return launch64<4, 4>(a, s);
+  }
+  if (banks <= 1) return launch64<8, 1>(a, s);
+  if (banks == 2) return launch64<8, 2>(a, s);
+  return launch64<8, 4>(a, s);
+}
+
+} // namespace ebgpu

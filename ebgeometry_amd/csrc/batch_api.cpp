@@ -1,0 +1,517 @@
+// C-ABI for batches of meshes: build (optionally autotuned), plan (packet schedule, buffers,
+// workgroup tuning, hipGraph capture), launch.
+#include "host_common.hpp"
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstring>
+#include <numeric>
+#include <string>
+#include <thread>
+#include <vector>
+
+using ebgpu::DevBuf;
+using ebgpu::fail;
+using ebgpu::upload;
+
+struct ebg_batch
+{
+  std::vector<ebg_mesh*> meshes; // owned
+  DevBuf                 meshdev;
+  int                    K = 4, leaf = 8, banks = 1;
+  bool                   tuned = false;
+  double                 tune_ms[4]{-1, -1, -1, -1};
+  double                 build_seconds = 0;
+  uint64_t               num_triangles = 0;
+
+  ~ebg_batch()
+  {
+    for (ebg_mesh* m : meshes) ebgpu::mesh_release(m);
+  }
+};
+
+struct ebg_batch_plan
+{
+  ebg_batch*      b = nullptr;
+  uint64_t        total = 0;
+  uint32_t        num_items = 0, num_packets = 0;
+  DevBuf          items, packets, keys, keys_sorted, sorted, scratch, recheck, counter;
+  int             wg = 256;
+  bool            presort = true, fp64 = true;
+  unsigned long long* work = nullptr;
+  hipGraph_t      graph = nullptr;
+  hipGraphExec_t  exec  = nullptr;
+  hipEvent_t      ev0 = nullptr, ev1 = nullptr;
+  bool            timed = false;
+
+  ~ebg_batch_plan()
+  {
+    if (exec) (void)hipGraphExecDestroy(exec);
+    if (graph) (void)hipGraphDestroy(graph);
+    if (ev0) (void)hipEventDestroy(ev0);
+    if (ev1) (void)hipEventDestroy(ev1);
+  }
+};
+
+namespace {
+
+// Deterministic synthetic queries for tuning: half uniform in the 1.2x bounding box, half near
+// the surface (area-weighted triangle, uniform barycentric point, offset N(0, 0.02 diag)).
+struct Rng
+{
+  uint64_t s;
+  uint64_t next()
+  {
+    s += 0x9E3779B97F4A7C15ull;
+    uint64_t z = s;
+    z          = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z          = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    return z ^ (z >> 31);
+  }
+  double uniform() { return double(next() >> 11) * (1.0 / 9007199254740992.0); }
+  double normal()
+  {
+    const double u = std::max(uniform(), 1e-300), v = uniform();
+    return std::sqrt(-2.0 * std::log(u)) * std::cos(6.283185307179586 * v);
+  }
+};
+
+std::vector<float> tuning_queries(const float* xyz, uint64_t nv, const uint32_t* tri, uint64_t nt, uint64_t n,
+                                  uint64_t seed)
+{
+  double lo[3] = {1e300, 1e300, 1e300}, hi[3] = {-1e300, -1e300, -1e300};
+  for (uint64_t v = 0; v < nv; v++)
+    for (int c = 0; c < 3; c++) {
+      lo[c] = std::min(lo[c], double(xyz[3 * v + c]));
+      hi[c] = std::max(hi[c], double(xyz[3 * v + c]));
+    }
+  const double        diag = std::sqrt((hi[0] - lo[0]) * (hi[0] - lo[0]) + (hi[1] - lo[1]) * (hi[1] - lo[1]) +
+                                       (hi[2] - lo[2]) * (hi[2] - lo[2]));
+  std::vector<double> cdf(nt);
+  double              acc = 0;
+  for (uint64_t f = 0; f < nt; f++) {
+    const float* a = xyz + 3 * tri[3 * f];
+    const float* b = xyz + 3 * tri[3 * f + 1];
+    const float* c = xyz + 3 * tri[3 * f + 2];
+    const double ux = b[0] - a[0], uy = b[1] - a[1], uz = b[2] - a[2];
+    const double wx = c[0] - a[0], wy = c[1] - a[1], wz = c[2] - a[2];
+    const double cx = uy * wz - uz * wy, cy = uz * wx - ux * wz, cz = ux * wy - uy * wx;
+    acc += std::sqrt(cx * cx + cy * cy + cz * cz);
+    cdf[f] = acc;
+  }
+  Rng                rng{seed};
+  std::vector<float> q(3 * n);
+  for (uint64_t i = 0; i < n; i++) {
+    if (i % 2 == 0) {
+      for (int c = 0; c < 3; c++) {
+        const double mid = 0.5 * (lo[c] + hi[c]), h = 0.6 * (hi[c] - lo[c]);
+        q[3 * i + c]     = float(mid - h + 2.0 * h * rng.uniform());
+      }
+    }
+    else {
+      const uint64_t f = std::min<uint64_t>(nt - 1, std::lower_bound(cdf.begin(), cdf.end(), rng.uniform() * acc) -
+                                                        cdf.begin());
+      double         r1 = rng.uniform(), r2 = rng.uniform();
+      if (r1 + r2 > 1.0) {
+        r1 = 1.0 - r1;
+        r2 = 1.0 - r2;
+      }
+      const float* a = xyz + 3 * tri[3 * f];
+      const float* b = xyz + 3 * tri[3 * f + 1];
+      const float* c = xyz + 3 * tri[3 * f + 2];
+      double       p[3], u[3], w[3];
+      for (int k = 0; k < 3; k++) {
+        u[k] = b[k] - a[k];
+        w[k] = c[k] - a[k];
+        p[k] = a[k] + r1 * u[k] + r2 * w[k];
+      }
+      double nrm[3] = {u[1] * w[2] - u[2] * w[1], u[2] * w[0] - u[0] * w[2], u[0] * w[1] - u[1] * w[0]};
+      const double l = std::sqrt(nrm[0] * nrm[0] + nrm[1] * nrm[1] + nrm[2] * nrm[2]);
+      const double o = 0.02 * diag * rng.normal() / (l > 0 ? l : 1.0);
+      for (int k = 0; k < 3; k++) q[3 * i + k] = float(p[k] + o * nrm[k]);
+    }
+  }
+  return q;
+}
+
+// Build meshes[i] for the listed indices on `threads` host threads.
+ebg_status build_meshes(const float* const* xyz, const uint64_t* nv, const uint32_t* const* tri, const uint64_t* nt,
+                        const std::vector<uint64_t>& which, int K, int leaf, int threads,
+                        std::vector<ebg_mesh*>& out)
+{
+  out.assign(which.size(), nullptr);
+  std::vector<ebg_status>  st(which.size(), EBG_OK);
+  std::vector<std::string> msg(which.size());
+  std::atomic<uint64_t>    next{0};
+  auto                     work = [&]() {
+    for (;;) {
+      const uint64_t k = next.fetch_add(1);
+      if (k >= which.size()) return;
+      const uint64_t i = which[k];
+      ebg_build_opts o{EBG_BUILD_SAH, K, leaf, 0};
+      st[k] = ebg_mesh_create(xyz[i], nv[i], tri[i], nt[i], &o, &out[k]);
+      if (st[k] != EBG_OK) msg[k] = "mesh " + std::to_string(i) + ": " + ebg_last_error();
+    }
+  };
+  const int                nth = std::max(1, std::min<int>(threads, int(which.size())));
+  std::vector<std::thread> pool;
+  for (int t = 1; t < nth; t++) pool.emplace_back(work);
+  work();
+  for (auto& t : pool) t.join();
+  for (uint64_t k = 0; k < which.size(); k++)
+    if (st[k] != EBG_OK) {
+      for (ebg_mesh* m : out)
+        if (m) ebgpu::mesh_release(m);
+      out.clear();
+      return fail(st[k], msg[k]);
+    }
+  return EBG_OK;
+}
+
+ebg_status enqueue(ebg_batch_plan* p, hipStream_t s, bool events)
+{
+  ebg_batch*       b = p->b;
+  ebgpu::BatchArgs a{};
+  a.items         = p->items.as<ebgpu::BatchItemDev>();
+  a.packets       = p->packets.as<uint32_t>();
+  a.num_packets   = p->num_packets;
+  a.meshes        = b->meshdev.as<ebgpu::MeshDev>();
+  a.sorted        = nullptr;
+  a.recheck       = p->fp64 ? p->recheck.as<uint4>() : nullptr;
+  a.recheck_count = p->counter.as<uint32_t>();
+  a.work          = p->work;
+  if (p->fp64) EBG_HIP(ebgpu::launch_fill_u32(a.recheck_count, 1, 0u, s));
+  if (p->presort && p->total > 1) {
+    EBG_HIP(ebgpu::launch_batch_keys(a.items, p->num_items, p->total, p->keys.as<uint64_t>(), s));
+    int item_bits = 1;
+    while ((1ull << item_bits) < p->num_items) item_bits++;
+    EBG_HIP(ebgpu::radix_sort_pairs(p->keys.as<uint64_t>(), nullptr, p->keys_sorted.as<uint64_t>(),
+                                    p->sorted.as<uint32_t>(), p->total, 42 + item_bits, p->scratch.p, s));
+    a.sorted = p->sorted.as<uint32_t>();
+  }
+  if (events) EBG_HIP(hipEventRecord(p->ev0, s));
+  EBG_HIP(ebgpu::launch_batch32(a, b->K, b->banks, p->wg, p->work != nullptr, s));
+  if (events) EBG_HIP(hipEventRecord(p->ev1, s));
+  if (p->fp64) EBG_HIP(ebgpu::launch_batch64(a, b->K, b->banks, s));
+  return EBG_OK;
+}
+
+double elapsed_ms(hipEvent_t a, hipEvent_t b)
+{
+  float ms = -1.0f;
+  if (hipEventSynchronize(b) != hipSuccess) return -1.0;
+  if (hipEventElapsedTime(&ms, a, b) != hipSuccess) return -1.0;
+  return ms;
+}
+
+} // namespace
+
+extern "C" {
+
+ebg_status ebg_batch_create(const float* const* xyz, const uint64_t* nv, const uint32_t* const* tri,
+                            const uint64_t* nt, uint64_t num_meshes, const ebg_batch_opts* opts, ebg_batch** out)
+{
+  if (!out) return fail(EBG_INVALID_ARG, "out is null");
+  *out = nullptr;
+  if (num_meshes == 0 || !xyz || !nv || !tri || !nt) return fail(EBG_INVALID_ARG, "empty batch");
+  if (num_meshes > 0xFFFFFFFFull) return fail(EBG_INVALID_ARG, "too many meshes");
+  for (uint64_t i = 0; i < num_meshes; i++) {
+    if (!xyz[i] || !tri[i] || nv[i] < 3 || nt[i] < 1)
+      return fail(EBG_INVALID_ARG, "mesh " + std::to_string(i) + ": empty mesh");
+    for (uint64_t k = 0; k < 3 * nt[i]; k++)
+      if (tri[i][k] >= nv[i]) return fail(EBG_INVALID_ARG, "mesh " + std::to_string(i) + ": index out of range");
+  }
+  ebg_batch_opts o{};
+  if (opts) o = *opts;
+  const int threads = o.threads > 0 ? o.threads : std::max(1, int(std::thread::hardware_concurrency()));
+  int       K = o.branching ? o.branching : 4, leaf = o.leaf_size ? o.leaf_size : 8;
+  if (K != 4 && K != 8) return fail(EBG_INVALID_ARG, "branching must be 4 or 8");
+  if (leaf < 1 || leaf > ebgpu::kMaxLeaf) return fail(EBG_INVALID_ARG, "leaf_size must be 1..16");
+
+  const auto t0 = std::chrono::steady_clock::now();
+  auto*      b  = new ebg_batch;
+  if (o.autotune) {
+    // the two largest meshes, 2^17 synthetic queries each, plain traversal time per candidate
+    std::vector<uint64_t> order(num_meshes);
+    std::iota(order.begin(), order.end(), 0);
+    std::sort(order.begin(), order.end(), [&](uint64_t x, uint64_t y) { return nt[x] > nt[y]; });
+    order.resize(std::min<uint64_t>(2, num_meshes));
+    const uint64_t                  nq = 1u << 17;
+    std::vector<std::vector<float>> qh;
+    for (uint64_t i : order) qh.push_back(tuning_queries(xyz[i], nv[i], tri[i], nt[i], nq, 1234 + i));
+    DevBuf qd, dd;
+    hipError_t e;
+    if ((e = qd.reserve(12 * nq * order.size())) != hipSuccess || (e = dd.reserve(4 * nq)) != hipSuccess) {
+      delete b;
+      return fail(EBG_OOM, std::string("tuning buffers: ") + hipGetErrorString(e));
+    }
+    for (std::size_t k = 0; k < order.size(); k++)
+      (void)hipMemcpy(qd.as<float>() + 3 * nq * k, qh[k].data(), 12 * nq, hipMemcpyHostToDevice);
+    const int cand[4][2] = {{4, 4}, {4, 8}, {8, 4}, {8, 8}};
+    double    best       = 1e300;
+    for (int c = 0; c < 4; c++) {
+      std::vector<ebg_mesh*> ms;
+      const ebg_status       st = build_meshes(xyz, nv, tri, nt, order, cand[c][0], cand[c][1], threads, ms);
+      if (st == EBG_STACK_OVERFLOW) continue;
+      if (st != EBG_OK) {
+        delete b;
+        return st;
+      }
+      double total = 0;
+      for (std::size_t k = 0; k < ms.size(); k++) {
+        ebg_query_opts qo;
+        ebg_query_opts_default(&qo);
+        qo.fp64_recheck = 0;
+        double tmin     = 1e300;
+        for (int rep = 0; rep < 3; rep++) {
+          const ebg_status qs = ebg_query_sdf(ms[k], qd.as<float>() + 3 * nq * k, nq, dd.as<float>(), nullptr, &qo);
+          if (qs != EBG_OK) {
+            for (ebg_mesh* m : ms) ebgpu::mesh_release(m);
+            delete b;
+            return qs;
+          }
+          tmin = std::min(tmin, ebg_mesh_last_traversal_ms(ms[k]));
+        }
+        total += tmin;
+      }
+      for (ebg_mesh* m : ms) ebgpu::mesh_release(m);
+      b->tune_ms[c] = total;
+      if (total < best) {
+        best = total;
+        K    = cand[c][0];
+        leaf = cand[c][1];
+      }
+    }
+    b->tuned = true;
+  }
+  std::vector<uint64_t> all(num_meshes);
+  std::iota(all.begin(), all.end(), 0);
+  const ebg_status st = build_meshes(xyz, nv, tri, nt, all, K, leaf, threads, b->meshes);
+  if (st != EBG_OK) {
+    delete b;
+    return st;
+  }
+  b->K    = K;
+  b->leaf = leaf;
+  std::vector<ebgpu::MeshDev> md(num_meshes);
+  for (uint64_t i = 0; i < num_meshes; i++) {
+    md[i]    = b->meshes[i]->dev;
+    b->banks = std::max(b->banks, b->meshes[i]->banks);
+    b->num_triangles += b->meshes[i]->nt;
+  }
+  const hipError_t e = upload(b->meshdev, md);
+  if (e != hipSuccess) {
+    delete b;
+    return fail(EBG_HIP_ERROR, std::string("batch upload: ") + hipGetErrorString(e));
+  }
+  b->build_seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+  *out             = b;
+  return EBG_OK;
+}
+
+ebg_status ebg_batch_destroy(ebg_batch* batch)
+{
+  delete batch;
+  return EBG_OK;
+}
+
+ebg_status ebg_batch_get_info(const ebg_batch* b, ebg_batch_info* info)
+{
+  if (!b || !info) return fail(EBG_INVALID_ARG, "null argument");
+  std::memset(info, 0, sizeof(*info));
+  info->num_meshes     = b->meshes.size();
+  info->num_triangles  = b->num_triangles;
+  info->branching      = b->K;
+  info->leaf_size      = b->leaf;
+  info->stack_capacity = 64 * b->banks;
+  info->tuned          = b->tuned ? 1 : 0;
+  for (int c = 0; c < 4; c++) info->tune_ms[c] = b->tune_ms[c];
+  info->build_seconds = b->build_seconds;
+  return EBG_OK;
+}
+
+ebg_status ebg_batch_mesh(const ebg_batch* b, uint64_t i, ebg_mesh** mesh)
+{
+  if (!b || !mesh) return fail(EBG_INVALID_ARG, "null argument");
+  if (i >= b->meshes.size()) return fail(EBG_INVALID_ARG, "mesh index out of range");
+  *mesh = b->meshes[i];
+  return EBG_OK;
+}
+
+ebg_status ebg_batch_plan_create(ebg_batch* b, const ebg_batch_item* items, uint64_t num_items,
+                                 const ebg_query_opts* opts, int32_t workgroup, int32_t capture,
+                                 ebg_batch_plan** out)
+{
+  if (!out) return fail(EBG_INVALID_ARG, "out is null");
+  *out = nullptr;
+  if (!b) return fail(EBG_INVALID_ARG, "batch is null");
+  if (num_items && !items) return fail(EBG_INVALID_ARG, "items is null");
+  if (num_items > 0xFFFFFFull) return fail(EBG_INVALID_ARG, "too many items");
+  if (workgroup != 0 && workgroup != 64 && workgroup != 128 && workgroup != 256)
+    return fail(EBG_INVALID_ARG, "workgroup must be 0, 64, 128 or 256");
+  ebg_query_opts o;
+  ebg_query_opts_default(&o);
+  if (opts) o = *opts;
+  if (o.learned_bound || o.seed_radius) return fail(EBG_INVALID_ARG, "batches take no learned or seeded bound");
+
+  // items -> device records and the packet schedule (most expensive meshes first)
+  std::vector<ebgpu::BatchItemDev> rec(num_items);
+  uint64_t                         total = 0;
+  for (uint64_t i = 0; i < num_items; i++) {
+    const ebg_batch_item& it = items[i];
+    if (it.mesh >= b->meshes.size()) return fail(EBG_INVALID_ARG, "item " + std::to_string(i) + ": bad mesh index");
+    if (it.n > 0xFFFFFFFFull) return fail(EBG_INVALID_ARG, "item " + std::to_string(i) + ": too many queries");
+    if (it.n && (!it.d_pts || !it.d_dist))
+      return fail(EBG_INVALID_ARG, "item " + std::to_string(i) + ": null point or output array");
+    const ebg_mesh*      m = b->meshes[it.mesh];
+    ebgpu::BatchItemDev& r = rec[i];
+    std::memset(&r, 0, sizeof(r));
+    r.pts  = it.d_pts;
+    r.out  = it.d_dist;
+    r.tri  = it.d_tri;
+    r.base = total;
+    r.n    = uint32_t(it.n);
+    r.mesh = it.mesh;
+    for (int c = 0; c < 3; c++) {
+      const double mid = 0.5 * (m->lo[c] + m->hi[c]), h = 0.625 * (m->hi[c] - m->lo[c]);
+      r.box_lo[c]      = float(mid - h);
+      r.box_inv[c]     = h > 0 ? float(2097151.0 / (2.0 * h)) : 0.0f;
+    }
+    r.recheck_abs = o.fp64_all ? INFINITY : float(o.recheck_eps * m->diag);
+    r.slack_abs   = float(1e-5 * m->diag);
+    total += it.n;
+  }
+  if (total > 0xFFFFFFFFull) return fail(EBG_INVALID_ARG, "at most 2^32-1 queries per plan");
+  std::vector<uint64_t> ord(num_items);
+  std::iota(ord.begin(), ord.end(), 0);
+  std::stable_sort(ord.begin(), ord.end(), [&](uint64_t x, uint64_t y) {
+    return b->meshes[items[x].mesh]->nt > b->meshes[items[y].mesh]->nt;
+  });
+  std::vector<uint32_t> pk;
+  for (uint64_t i : ord)
+    for (uint64_t f = 0; f < items[i].n; f += 64) {
+      pk.push_back(uint32_t(i));
+      pk.push_back(uint32_t(f));
+      pk.push_back(0u);
+      pk.push_back(0u);
+    }
+
+  auto* p        = new ebg_batch_plan;
+  p->b           = b;
+  p->total       = total;
+  p->num_items   = uint32_t(num_items);
+  p->num_packets = uint32_t(pk.size() / 4);
+  p->presort     = o.presort != 0;
+  p->fp64        = (o.fp64_recheck || o.fp64_all) && p->num_packets > 0;
+  p->work        = reinterpret_cast<unsigned long long*>(o.work);
+  p->wg          = workgroup ? workgroup : 256;
+  auto bail      = [&](hipError_t e, const char* what) {
+    delete p;
+    return fail(e == hipErrorOutOfMemory ? EBG_OOM : EBG_HIP_ERROR, std::string(what) + ": " + hipGetErrorString(e));
+  };
+  hipError_t e;
+  if ((e = upload(p->items, rec)) != hipSuccess) return bail(e, "items");
+  if ((e = upload(p->packets, pk)) != hipSuccess) return bail(e, "packets");
+  const uint64_t cap = std::max<uint64_t>(total, 1);
+  if ((e = p->keys.reserve(8 * cap)) != hipSuccess) return bail(e, "keys");
+  if ((e = p->keys_sorted.reserve(8 * cap)) != hipSuccess) return bail(e, "keys");
+  if ((e = p->sorted.reserve(4 * cap)) != hipSuccess) return bail(e, "sort");
+  if ((e = p->scratch.reserve(ebgpu::radix_sort_scratch_bytes(cap))) != hipSuccess) return bail(e, "sort scratch");
+  if ((e = p->recheck.reserve(16 * std::max<uint64_t>(p->num_packets, 1))) != hipSuccess) return bail(e, "recheck");
+  if ((e = p->counter.reserve(256)) != hipSuccess) return bail(e, "counter");
+  if ((e = hipEventCreate(&p->ev0)) != hipSuccess || (e = hipEventCreate(&p->ev1)) != hipSuccess)
+    return bail(e, "events");
+
+  hipStream_t s = static_cast<hipStream_t>(o.stream);
+  if (!workgroup && p->num_packets > 0) {
+    // pick the workgroup shape by timing the traversal on the items as they are
+    double best = 1e300;
+    int    pick = 256;
+    for (int wg : {256, 128, 64}) {
+      p->wg = wg;
+      double tmin = 1e300;
+      for (int rep = 0; rep < 3; rep++) {
+        const ebg_status st = enqueue(p, s, true);
+        if (st != EBG_OK) {
+          delete p;
+          return st;
+        }
+        tmin = std::min(tmin, elapsed_ms(p->ev0, p->ev1));
+      }
+      if (tmin < best) {
+        best = tmin;
+        pick = wg;
+      }
+    }
+    p->wg = pick;
+  }
+  if (capture) {
+    hipStream_t cs = nullptr;
+    if ((e = hipStreamCreateWithFlags(&cs, hipStreamNonBlocking)) != hipSuccess) return bail(e, "capture stream");
+    e = hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal);
+    ebg_status st = e == hipSuccess ? enqueue(p, cs, true) : EBG_HIP_ERROR;
+    hipGraph_t g  = nullptr;
+    const hipError_t e2 = hipStreamEndCapture(cs, &g);
+    (void)hipStreamDestroy(cs);
+    if (e != hipSuccess) return bail(e, "begin capture");
+    if (st != EBG_OK) {
+      if (g) (void)hipGraphDestroy(g);
+      delete p;
+      return st;
+    }
+    if (e2 != hipSuccess) return bail(e2, "end capture");
+    p->graph = g;
+    if ((e = hipGraphInstantiate(&p->exec, g, nullptr, nullptr, 0)) != hipSuccess) return bail(e, "instantiate");
+  }
+  *out = p;
+  return EBG_OK;
+}
+
+ebg_status ebg_batch_plan_launch(ebg_batch_plan* p, void* stream)
+{
+  if (!p) return fail(EBG_INVALID_ARG, "plan is null");
+  if (p->num_packets == 0) return EBG_OK;
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  if (p->exec) EBG_HIP(hipGraphLaunch(p->exec, s));
+  else {
+    const ebg_status st = enqueue(p, s, true);
+    if (st != EBG_OK) return st;
+  }
+  p->timed = true;
+  return EBG_OK;
+}
+
+ebg_status ebg_batch_plan_destroy(ebg_batch_plan* p)
+{
+  delete p;
+  return EBG_OK;
+}
+
+int32_t ebg_batch_plan_workgroup(const ebg_batch_plan* p) { return p ? p->wg : 0; }
+
+double ebg_batch_plan_last_traversal_ms(const ebg_batch_plan* p)
+{
+  if (!p || !p->timed) return -1.0;
+  return elapsed_ms(p->ev0, p->ev1);
+}
+
+ebg_status ebg_batch_query(ebg_batch* b, const ebg_batch_item* items, uint64_t num_items, const ebg_query_opts* opts)
+{
+  ebg_batch_plan*  p  = nullptr;
+  const ebg_status st = ebg_batch_plan_create(b, items, num_items, opts, 256, 0, &p);
+  if (st != EBG_OK) return st;
+  ebg_query_opts o;
+  ebg_query_opts_default(&o);
+  if (opts) o = *opts;
+  ebg_status r = ebg_batch_plan_launch(p, o.stream);
+  // the plan's buffers must outlive the enqueued work
+  if (r == EBG_OK && hipStreamSynchronize(static_cast<hipStream_t>(o.stream)) != hipSuccess)
+    r = fail(EBG_HIP_ERROR, "batch query failed");
+  ebg_batch_plan_destroy(p);
+  return r;
+}
+
+} // extern "C"

@@ -40,6 +40,41 @@ hipError_t launch_scene32(const SceneDev& sc, int KM, int banks, const QueryArgs
 // FP64 pass over a.idx / a.count_ptr, seeded with |v32| (1 + slack_rel) + slack_abs.
 hipError_t launch_scene64(const SceneDev& sc, int KM, int banks, const QueryArgs& a, hipStream_t s);
 
+// Many meshes in one launch (batch.hip).
+struct alignas(16) BatchItemDev
+{
+  const float* pts;
+  float*       out;
+  uint32_t*    tri;
+  uint64_t     base;       // flat index of the item's first query
+  uint32_t     n, mesh;
+  float        box_lo[3];  // Morton box of the item's mesh: bin = (p - lo) * inv, 21 bits
+  float        box_inv[3];
+  float        recheck_abs; // |d| below this is redone in FP64 (recheck_eps * mesh diagonal)
+  float        slack_abs;   // FP64 seed slack (1e-5 * mesh diagonal)
+  uint32_t     pad[2];
+};
+static_assert(sizeof(BatchItemDev) == 80, "BatchItemDev layout");
+
+struct BatchArgs
+{
+  const BatchItemDev* items;
+  const uint32_t*     packets;     // 4 per packet: item, first query (in the item's sorted order), 0, 0
+  uint32_t            num_packets;
+  const MeshDev*      meshes;
+  const uint32_t*     sorted;      // flat query indices in sort order (null: no presort)
+  uint4*              recheck;     // per listed packet: packet, 0, lane mask lo, hi
+  uint32_t*           recheck_count;
+  unsigned long long* work;        // [0] inner, [1] triangles, [4]/[5] per wavefront (null = off)
+};
+
+hipError_t launch_batch_keys(const BatchItemDev* items, uint32_t num_items, uint64_t total, uint64_t* keys,
+                             hipStream_t s);
+// wg = 64, 128 or 256 lanes per workgroup (1, 2 or 4 packets).
+hipError_t launch_batch32(const BatchArgs& a, int K, int banks, int wg, bool count, hipStream_t s);
+// a.recheck == null skips; grid = num_packets one-wavefront blocks, those past the count exit.
+hipError_t launch_batch64(const BatchArgs& a, int K, int banks, hipStream_t s);
+
 // util.hip
 hipError_t launch_fill_u32(uint32_t* p, uint64_t n, uint32_t v, hipStream_t s);
 hipError_t launch_add_counts(unsigned long long* work, const uint32_t* recheck, const uint32_t* fallback, hipStream_t s);

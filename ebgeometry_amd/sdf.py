@@ -80,7 +80,7 @@ class TriMeshSDF(SignedDistanceFunction):
 
     def __del__(self):
         h = getattr(self, "_h", None)
-        if h:
+        if h and getattr(self, "_owned", True):
             try:
                 lib().ebg_mesh_destroy(h)
             except Exception:  # pragma: no cover - interpreter teardown

@@ -198,6 +198,76 @@ ebg_status ebg_scene_query_host(const ebg_scene* scene, const float* pts, uint64
 double     ebg_scene_last_traversal_ms(const ebg_scene* scene);
 
 /* ---------------------------------------------------------------------------------------- */
+/* Batched queries over many meshes -- many TriMeshSDF objects (MeshDistanceFunctions.hpp:    */
+/* 376-387) queried by one traversal launch. The reference has no counterpart: its callers    */
+/* loop over meshes and points. A batch owns its meshes (built from raw arrays, optionally    */
+/* with the branching factor and leaf size picked by timing), a plan fixes one set of query   */
+/* arrays and replays the whole pipeline (presort, traversal, FP64 pass) as one hipGraph.     */
+/* ---------------------------------------------------------------------------------------- */
+
+typedef struct ebg_batch_plan ebg_batch_plan;
+
+typedef struct ebg_batch_opts
+{
+  int32_t autotune;   /* 1: time (branching, leaf_size) in {4,8}x{4,8} on the largest meshes
+                         with synthetic queries and build every mesh with the fastest         */
+  int32_t branching;  /* used when autotune = 0 (0 = 4)                                       */
+  int32_t leaf_size;  /* used when autotune = 0 (0 = 8)                                       */
+  int32_t threads;    /* host threads for building meshes (0 = hardware concurrency)          */
+} ebg_batch_opts;
+
+typedef struct ebg_batch_info
+{
+  uint64_t num_meshes, num_triangles;
+  int32_t  branching, leaf_size, stack_capacity;
+  int32_t  tuned;                /* 1 if the autotuner chose branching / leaf_size           */
+  double   tune_ms[4];           /* traversal ms of the candidates (4,4) (4,8) (8,4) (8,8)    */
+  double   build_seconds;
+} ebg_batch_info;
+
+/* One query array against one mesh of the batch (device pointers). */
+typedef struct ebg_batch_item
+{
+  uint32_t     mesh;     /* index into the batch's meshes                                    */
+  uint32_t     reserved;
+  uint64_t     n;        /* < 2^32                                                            */
+  const float* d_pts;    /* n float32 xyz                                                     */
+  float*       d_dist;   /* n float32 signed distances                                        */
+  uint32_t*    d_tri;    /* nullable: closest face per query                                  */
+} ebg_batch_item;
+
+/* meshes: num_meshes (xyz, nv, tri, nt) arrays as ebg_mesh_create takes them (host, copied). */
+ebg_status ebg_batch_create(const float* const* xyz, const uint64_t* nv, const uint32_t* const* tri,
+                            const uint64_t* nt, uint64_t num_meshes, const ebg_batch_opts* opts,
+                            ebg_batch** out);
+ebg_status ebg_batch_destroy(ebg_batch* batch);
+ebg_status ebg_batch_get_info(const ebg_batch* batch, ebg_batch_info* info);
+/* The batch's mesh i as a mesh handle (owned by the batch; valid while the batch lives). */
+ebg_status ebg_batch_mesh(const ebg_batch* batch, uint64_t i, ebg_mesh** mesh);
+
+/* Plan one set of items: checks them, builds the packet schedule (64-query packets of one item
+ * each, largest meshes first), allocates every buffer, and -- if opts->fp64_all is 0 and
+ * `workgroup` is 0 -- picks the workgroup size (64, 128 or 256 lanes) by timing the traversal on
+ * the items as they are (their points must be valid then). With capture = 1 the pipeline is
+ * recorded once into a hipGraph; ebg_batch_plan_launch then replays it. The item arrays must
+ * stay allocated while the plan is used; their contents may change between launches.
+ * opts: presort, fp64_recheck, recheck_eps, fp64_all, work are honoured; stream is used for
+ * tuning; learned_bound and seed_radius must be off. */
+ebg_status ebg_batch_plan_create(ebg_batch* batch, const ebg_batch_item* items, uint64_t num_items,
+                                 const ebg_query_opts* opts, int32_t workgroup, int32_t capture,
+                                 ebg_batch_plan** out);
+ebg_status ebg_batch_plan_launch(ebg_batch_plan* plan, void* stream);
+ebg_status ebg_batch_plan_destroy(ebg_batch_plan* plan);
+/* Workgroup size the plan uses, and its traversal kernel time of the last launch (HIP events
+ * around that launch on the launch stream; waits for it). */
+int32_t    ebg_batch_plan_workgroup(const ebg_batch_plan* plan);
+double     ebg_batch_plan_last_traversal_ms(const ebg_batch_plan* plan);
+
+/* One-shot convenience: plan (no tuning, no graph), launch on opts->stream, release. */
+ebg_status ebg_batch_query(ebg_batch* batch, const ebg_batch_item* items, uint64_t num_items,
+                           const ebg_query_opts* opts);
+
+/* ---------------------------------------------------------------------------------------- */
 /* Space-filling curve -- SFC::computeBins<T>, Morton::encode, SFC::order                      */
 /* (Source/EBGeometry_SFCImplem.hpp:31-58, 228-300). Device pointers, async on stream.        */
 /* Bit-exact with the reference; equal codes are ordered by increasing index (stable).        */

@@ -35,7 +35,8 @@ def test_struct_layout_matches_header(tmp_path):
     import subprocess
     from ebgeometry_amd import _lib
     structs = {"ebg_build_opts": _lib.BuildOpts, "ebg_query_opts": _lib.QueryOpts,
-               "ebg_mesh_info": _lib.MeshInfo, "ebg_mlp_opts": _lib.MlpOpts, "ebg_prim": _lib.Prim}
+               "ebg_mesh_info": _lib.MeshInfo, "ebg_mlp_opts": _lib.MlpOpts, "ebg_prim": _lib.Prim,
+               "ebg_batch_opts": _lib.BatchOpts, "ebg_batch_info": _lib.BatchInfo, "ebg_batch_item": _lib.BatchItem}
     lines = ['#include <stdio.h>', '#include <stddef.h>', '#include "ebgpu.h"', "int main(void){"]
     for cname, py in structs.items():
         lines.append(f'printf("{cname} %zu\\n", sizeof({cname}));')

@@ -1,0 +1,148 @@
+"""GPU parity of the many-mesh path: every item of a batch must equal the single-mesh query of
+the same mesh (same kernels, so the distances agree bit for bit up to exact ties), and the oracle
+within 1e-5 * diag with equal signs; captured graphs must give the eager result on every replay."""
+import numpy as np
+import pytest
+
+from ebgeometry_amd import MeshBatch, TriMeshSDF
+from ebgeometry_amd import workloads as wl
+from oracle import port
+
+pytestmark = pytest.mark.gpu
+
+D_TOL = 1e-5
+
+
+def diag_of(v):
+    v = np.asarray(v, np.float64)
+    return float(np.linalg.norm(v.max(0) - v.min(0)))
+
+
+@pytest.fixture(scope="module")
+def small_batch():
+    w = wl.batched_meshes(10, 5, level_range=(1, 4), query_range=(1, 5000))
+    return w, MeshBatch(list(zip(w.verts, w.faces)), branching=4, leaf_size=8)
+
+
+def items_for(w, cuda, extra=()):
+    import torch
+    items = [(i, torch.from_numpy(q).to(cuda)) for i, q in enumerate(w.queries)]
+    for mesh, q in extra:
+        items.append((mesh, torch.from_numpy(q).to(cuda)))
+    return items
+
+
+def test_batch_equals_single_mesh_queries(small_batch, cuda):
+    import torch
+    w, b = small_batch
+    # a repeated mesh, an empty item and a ragged one on top of one item per mesh
+    extra = [(3, w.queries[7][:37]), (0, np.zeros((0, 3), np.float32)), (9, w.queries[1][:65])]
+    items = items_for(w, cuda, extra)
+    outs, tris = b.query(items, with_triangle=True)
+    torch.cuda.synchronize()
+    for k, (mesh, q) in enumerate(items):
+        if q.shape[0] == 0:
+            assert outs[k].shape[0] == 0
+            continue
+        single = TriMeshSDF(w.verts[mesh], w.faces[mesh], branching=4, leaf_size=8)
+        d1, t1 = single.closest_triangle(q)
+        torch.cuda.synchronize()
+        d0 = outs[k].cpu().numpy()
+        d1 = d1.cpu().numpy()
+        diag = diag_of(w.verts[mesh])
+        assert np.abs(d0 - d1).max() <= 1e-6 * diag, (k, mesh)
+        assert np.array_equal(np.sign(d0), np.sign(d1)), (k, mesh)
+
+
+@pytest.mark.parametrize("mesh", [0, 4, 9])
+def test_batch_matches_oracle(small_batch, cuda, mesh):
+    import torch
+    w, b = small_batch
+    items = items_for(w, cuda)
+    outs = b.query(items)
+    torch.cuda.synchronize()
+    d = outs[mesh].cpu().numpy().astype(np.float64)
+    ref, _ = port.Mesh(w.verts[mesh].astype(np.float64), w.faces[mesh]).sdf(w.queries[mesh].astype(np.float64))
+    diag = diag_of(w.verts[mesh])
+    assert np.abs(d - ref).max() <= D_TOL * diag
+    nz = ref != 0
+    assert np.array_equal(np.sign(d[nz]), np.sign(ref[nz]))
+
+
+def test_plan_graph_replays_match_eager(small_batch, cuda):
+    import torch
+    w, b = small_batch
+    items = items_for(w, cuda)
+    eager = [o.clone() for o in b.query(items)]
+    plan = b.plan(items, capture=True)
+    assert plan.workgroup in (64, 128, 256)
+    for rep in range(3):
+        for o in plan.out:
+            o.fill_(12345.0)
+        plan.launch()
+        torch.cuda.synchronize()
+        for k in range(len(items)):
+            assert torch.equal(plan.out[k], eager[k]), (rep, k)
+        # unrelated eager work between replays
+        _ = b.query(items[:2])
+        torch.cuda.synchronize()
+    assert plan.last_traversal_ms() > 0
+
+
+def test_plan_sees_new_point_values(small_batch, cuda):
+    """The plan fixes the arrays, not their contents."""
+    import torch
+    w, b = small_batch
+    items = items_for(w, cuda)
+    plan = b.plan(items, workgroup=128, capture=True)
+    plan.launch()
+    for _, q in items:
+        q.mul_(1.01)
+    plan.launch()
+    torch.cuda.synchronize()
+    fresh = b.query(items)
+    torch.cuda.synchronize()
+    for k in range(len(items)):
+        assert torch.equal(plan.out[k], fresh[k])
+
+
+def test_autotuned_batch(cuda):
+    import torch
+    w = wl.batched_meshes(4, 11, level_range=(2, 4), query_range=(1000, 3000))
+    b = MeshBatch(list(zip(w.verts, w.faces)), autotune=True)
+    info = b.info()
+    assert info["tuned"] == 1
+    assert (info["branching"], info["leaf_size"]) in [(4, 4), (4, 8), (8, 4), (8, 8)]
+    assert min(t for t in info["tune_ms"] if t >= 0) > 0
+    items = items_for(w, cuda)
+    outs = b.query(items)
+    torch.cuda.synchronize()
+    for k in range(len(items)):
+        ref, _ = port.Mesh(w.verts[k].astype(np.float64), w.faces[k]).sdf(w.queries[k].astype(np.float64))
+        d = outs[k].cpu().numpy()
+        assert np.abs(d - ref).max() <= D_TOL * diag_of(w.verts[k])
+        assert np.array_equal(np.sign(d[ref != 0]), np.sign(ref[ref != 0]))
+
+
+def test_work_counters_and_fp64_all(small_batch, cuda):
+    import torch
+    w, b = small_batch
+    items = items_for(w, cuda)
+    wk = torch.zeros(8, dtype=torch.int64, device=cuda)
+    outs = b.query(items, work=wk.data_ptr())
+    exact = b.query(items, fp64_all=True)
+    torch.cuda.synchronize()
+    c = wk.cpu().numpy()
+    assert c[0] > 0 and c[1] > 0 and c[4] > 0 and c[5] > 0
+    for k in range(len(items)):
+        diag = diag_of(w.verts[k])
+        assert (outs[k] - exact[k]).abs().max().item() <= D_TOL * diag
+
+
+def test_batch_rejects_bad_items(small_batch, cuda):
+    import torch
+    from ebgeometry_amd import _lib
+    w, b = small_batch
+    q = torch.zeros((10, 3), dtype=torch.float32, device=cuda)
+    with pytest.raises(_lib.EbgError):
+        b.query([(99, q)])
