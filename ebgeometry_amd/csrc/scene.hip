@@ -119,8 +119,10 @@ __device__ __forceinline__ void scene_traverse(const SceneDev& sc, R px, R py, R
         uint32_t btri = kInvalid, bfeat = 0;
         traverse<KM, NB, R, COUNT>(md, qx, qy, qz, pn ? R(INFINITY) : R(-1), best, btri, bfeat, c.n_inner, c.n_tris,
                                    c.w_inner, c.w_tris);
+        // Lanes that did not ask for this instance (bound -1) still saw the triangles the
+        // wavefront evaluated, so their btri names some triangle but not their nearest one.
         val = R(INFINITY);
-        if (btri != kInvalid) {
+        if (pn && btri != kInvalid) {
           if constexpr (sizeof(R) == 4) val = s * resolve32(md, btri, bfeat, best, qx, qy, qz);
           else val = s * resolve64(md, btri, bfeat, best, qx, qy, qz);
         }

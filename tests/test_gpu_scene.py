@@ -180,3 +180,20 @@ def test_scene_keeps_destroyed_mesh_alive(cuda):
     _lib.lib().ebg_mesh_destroy(m.handle)  # the caller's handle goes away; the scene retains the mesh
     m._h = None
     np.testing.assert_array_equal(u.value(q), before)
+
+
+def test_mesh_instances_in_separate_leaves(cuda):
+    """One instance per scene leaf: after the first instance lowers some lanes' bound, the next
+    instance is walked by a wavefront in which only some lanes need it (regression: the other
+    lanes must not take a value from the triangles the wavefront evaluated for them)."""
+    sc = wl.csg_scene(41, n_spheres=0, n_boxes=0, n_meshes=4, mesh_level=3, extent=4.0)
+    meshes = [TriMeshSDF(v, f) for v, f in zip(sc.mesh_verts, sc.mesh_faces)]
+    prims = [Translate(Rotate(Rotate(Rotate(Scale(m, float(sc.mesh_scale[i])), float(sc.mesh_rot_deg[i][0]), 0),
+                                    float(sc.mesh_rot_deg[i][1]), 1), float(sc.mesh_rot_deg[i][2]), 2),
+                       sc.mesh_translate[i].astype(np.float64)) for i, m in enumerate(meshes)]
+    q = wl.uniform_box(8192, [-0.5] * 3, [4.5] * 3, 43)
+    ref = oracle_scene(sc).value(q.astype(np.float64))
+    for leaf in (1, 2, 4):
+        u = BVHUnionIF(prims, leaf_size=leaf)
+        _, _, diag = u.bounds()
+        check(u.value(q), ref, diag, f"meshes only, leaf {leaf}")
