@@ -65,6 +65,7 @@ SIGNATURES = [
     ("ebg_mesh_create", C.c_int, [_vp, _u64, _vp, _u64, C.POINTER(BuildOpts), C.POINTER(_vp)]),
     ("ebg_mesh_destroy", C.c_int, [_vp]),
     ("ebg_mesh_get_info", C.c_int, [_vp, C.POINTER(MeshInfo)]),
+    ("ebg_mesh_leaf_order", C.c_int, [_vp, _vp]),
     ("ebg_query_opts_default", None, [C.POINTER(QueryOpts)]),
     ("ebg_query_sdf", C.c_int, [_vp, _vp, _u64, _vp, _vp, C.POINTER(QueryOpts)]),
     ("ebg_query_sdf_host", C.c_int, [_vp, _vp, _u64, _vp, _vp, C.POINTER(QueryOpts)]),

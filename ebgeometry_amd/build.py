@@ -39,6 +39,7 @@ SOURCES = [
     ("scene.hip", DEVICE),
     ("batch.hip", DEVICE),
     ("morton.hip", DEVICE),
+    ("morton_build.hip", DEVICE),
     ("util.hip", DEVICE),
     ("mlp.hip", DEVICE),
 ]

@@ -139,14 +139,21 @@ ebg_status ebg_mesh_create(const float* xyz, uint64_t nv, const uint32_t* tri, u
   if (!ebgpu::compute_pseudo_normals(xyz, nv, tri, nt, pn, err)) return fail(EBG_INVALID_ARG, err);
   ebgpu::PackedMesh pm;
   if (o.builder == EBG_BUILD_MORTON) {
-    return fail(EBG_UNSUPPORTED, "Morton builder not available in this build");
+    if (!ebgpu::build_morton_mesh_gpu(xyz, nv, tri, nt, K, o.leaf_size ? o.leaf_size : 1, pn, pm, err))
+      return fail(EBG_HIP_ERROR, err);
   }
-  if (!ebgpu::build_sah_mesh(xyz, nv, tri, nt, K, leaf, pn, pm, err)) return fail(EBG_INVALID_ARG, err);
+  else if (o.builder == EBG_BUILD_SAH) {
+    if (!ebgpu::build_sah_mesh(xyz, nv, tri, nt, K, leaf, pn, pm, err)) return fail(EBG_INVALID_ARG, err);
+  }
+  else {
+    return fail(EBG_INVALID_ARG, "unknown builder");
+  }
 
   auto* m = new ebg_mesh;
   m->nv   = nv;
   m->nt   = nt;
   m->host_xyz.assign(xyz, xyz + 3 * nv);
+  m->order = std::move(pm.order);
   const ebg_status st = upload_mesh(m, pm);
   if (st != EBG_OK) {
     delete m;
@@ -180,6 +187,13 @@ ebg_status ebg_pseudo_normals_host(const float* xyz, uint64_t nv, const uint32_t
         en[9 * f + 3 * v + c] = pn.edge[9 * f + 3 * v + c];
       }
   }
+  return EBG_OK;
+}
+
+ebg_status ebg_mesh_leaf_order(const ebg_mesh* m, uint32_t* faces)
+{
+  if (!m || !faces) return fail(EBG_INVALID_ARG, "null argument");
+  std::copy(m->order.begin(), m->order.end(), faces);
   return EBG_OK;
 }
 

@@ -123,6 +123,7 @@ struct ebg_mesh
   ebgpu::DevBuf          nodes, tris, tripos, tsph, feat, vn32, en32, vn64, en64, fn64;
   ebgpu::DevBuf          area_cdf; // normalised cumulative triangle area, leaf order (training samples)
   std::vector<float>     host_xyz; // vertex positions (scene instance bounds)
+  std::vector<uint32_t>  order;    // original face index of each leaf-order triangle
   ebgpu::MeshDev         dev{};
   ebgpu::MlpTrainBuffers train;
   std::size_t            device_bytes = 0;

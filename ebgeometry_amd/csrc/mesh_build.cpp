@@ -117,6 +117,7 @@ void pack_triangles(const float* xyz, const uint32_t* tri, const std::vector<uin
                     const PseudoNormals& pn, std::size_t nv, PackedMesh& out)
 {
   const std::size_t nt = order.size();
+  out.order = order;
   out.tris.resize(nt);
   out.tripos.assign(12 * nt, 0.0f);
   out.feat.assign(8 * nt, 0u);

@@ -37,6 +37,7 @@ struct PackedMesh
   std::vector<uint32_t> feat;   // 8 per triangle, leaf order
   std::vector<float>    tsph;   // 4 per triangle (bounding sphere), leaf order, + kSphPad zero entries
   std::vector<float>    vn32, en32;
+  std::vector<uint32_t> order;  // original face index of each leaf-order triangle
   std::vector<double>   vn64, en64, fn64;
 };
 
@@ -50,6 +51,13 @@ bool build_sah_mesh(const float* xyz, std::size_t nv, const uint32_t* tri, std::
 // (order[i] = original face index) and the normal arrays. Shared by both builders.
 void pack_triangles(const float* xyz, const uint32_t* tri, const std::vector<uint32_t>& order,
                     const PseudoNormals& pn, std::size_t nv, PackedMesh& out);
+
+// GPU Morton build (morton_build.hip): the reference's bottom-up SFC partition into a balanced
+// K-ary tree (Source/EBGeometry_BVHImplem.hpp:207-312) over triangle box centroids. Codes, sort
+// and node boxes are computed on the device; the triangle records are packed here.
+int  morton_depth(uint64_t nt, int K, int leaf_size);
+bool build_morton_mesh_gpu(const float* xyz, std::size_t nv, const uint32_t* tri, std::size_t nt, int K, int leaf_size,
+                           const PseudoNormals& pn, PackedMesh& out, std::string& err);
 
 // Maximum traversal stack depth any query can need in a packed K-wide tree.
 int compute_stack_need(const PackedMesh& m);

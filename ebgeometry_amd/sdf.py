@@ -96,6 +96,12 @@ class TriMeshSDF(SignedDistanceFunction):
         check(lib().ebg_mesh_get_info(self._h, C.byref(i)))
         return {f: (list(getattr(i, f)) if f.startswith("bbox") else getattr(i, f)) for f, _ in MeshInfo._fields_}
 
+    def leaf_order(self) -> np.ndarray:
+        """Original face index of each triangle in BVH leaf order."""
+        out = np.empty(self.num_triangles, dtype=np.uint32)
+        check(lib().ebg_mesh_leaf_order(self._h, out.ctypes.data))
+        return out
+
     @staticmethod
     def query_opts(presort=True, fp64_recheck=True, recheck_eps=1e-4, learned_bound=False, fp64_all=False,
                    bound_scale=1.25, bound_offset=1e-3, stream=None, work=None, seed_radius=None) -> QueryOpts:

@@ -62,7 +62,9 @@ enum ebg_builder
 {
   EBG_BUILD_SAH    = 0, /* binned SAH on the host (default; best traversal quality)            */
   EBG_BUILD_MORTON = 1  /* Morton codes + radix sort + balanced K-ary tree on the GPU
-                           (the reference's Build::Morton, Source/EBGeometry_BVHImplem.hpp:207-312) */
+                           (the reference's Build::Morton, Source/EBGeometry_BVHImplem.hpp:207-312);
+                           leaf_size 0 or 1 gives the reference's depth floor(log n / log K),
+                           larger values fewer, fuller leaves (at most 16 triangles each)     */
 };
 
 typedef struct ebg_build_opts
@@ -89,6 +91,10 @@ ebg_status ebg_mesh_create(const float* xyz, uint64_t num_vertices, const uint32
                            uint64_t num_triangles, const ebg_build_opts* opts, ebg_mesh** out);
 ebg_status ebg_mesh_destroy(ebg_mesh* mesh);
 ebg_status ebg_mesh_get_info(const ebg_mesh* mesh, ebg_mesh_info* info);
+/* Original face index of each triangle in BVH leaf order (num_triangles uint32, host). For the
+ * Morton builder this is the stable (code, index) sort of the triangle box centroids' Morton
+ * codes, the order SFC::order gives up to equal codes (Source/EBGeometry_SFCImplem.hpp:279-300). */
+ebg_status ebg_mesh_leaf_order(const ebg_mesh* mesh, uint32_t* faces);
 
 /* Host-only (no GPU needed): the pseudo-normals ebg_mesh_create computes, per face: face normal
  * (3 doubles), vertex normals of v0,v1,v2 (9), normals of the edges v0->v1, v1->v2, v2->v0 (9).
