@@ -114,7 +114,7 @@ def test_transformed_mesh_equals_mesh_of_transformed_vertices(cuda):
     w = (t + s * (v.astype(np.float64) @ instance_matrix(rot))).astype(np.float32)
     mw = TriMeshSDF(w, f)
     q = np.concatenate([wl.uniform_box(20000, w.min(0) - 0.3, w.max(0) + 0.3, 4), wl.near_surface(5000, w, f, 0.02, 5)])
-    ref = port.Mesh(w.astype(np.float64), f).sdf(q.astype(np.float64))
+    ref, _ = port.Mesh(w.astype(np.float64), f).sdf(q.astype(np.float64))
     _, _, diag = u.bounds()
     a = u.value(q)
     check(a, ref, diag, "instance vs baked vertices")
