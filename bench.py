@@ -45,6 +45,8 @@ def parse():
     ap.add_argument("--branching", type=int, default=4)
     ap.add_argument("--leaf", type=int, default=8)
     ap.add_argument("--no-scene", action="store_true", help="skip the CSG scene measurement")
+    ap.add_argument("--plain-only", action="store_true",
+                    help="profiling aid: time only the plain pipeline (no training, no learned bound)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a 2 GiB device copy (to scale rocprofv3 FETCH_SIZE / WRITE_SIZE)")
     return ap.parse_args()
@@ -216,14 +218,29 @@ def main():
     out = torch.empty(n, dtype=torch.float32, device=dev)
     out_l = torch.empty(n, dtype=torch.float32, device=dev)
 
+    plain = lambda: mesh.query(qt, out=out)  # noqa: E731
+    learned = lambda: mesh.query(qt, out=out_l, learned_bound=True)  # noqa: E731
+    K = info["branching"]
+    kname = f"sdf32_kernel<{K},{info['stack_capacity'] // 64},false,{{mode}}>"
+
+    if a.plain_only:
+        # profiling aid: the plain pipeline alone (rocprofv3 --pmc cannot follow the training)
+        t_plain, trav_plain = timed(plain, a.steps, a.warmup, mesh.last_traversal_ms)
+        w = torch.zeros(8, dtype=torch.int64, device=dev)
+        mesh.query(qt, out=out_l, work=w.data_ptr())
+        torch.cuda.synchronize()
+        w = [int(x) for x in w.cpu().numpy()]
+        byts = 16 * n + w[0] * 28 * K + w[1] * 36
+        print(json.dumps({"profile_run": "plain-only", "ms_per_step": round(t_plain * 1e3, 3),
+                          "kernel": kname.format(mode=0), "kernel_ms": round(trav_plain, 3),
+                          "algorithmic_bytes_per_launch": int(byts)}), flush=True)
+        return
+
     t0 = time.perf_counter()
     loss = mesh.train_bound(num_samples=a.train_samples, epochs=a.epochs, seed=3)
     torch.cuda.synchronize()
     train_s = time.perf_counter() - t0
     log(f"bound trained in {train_s:.1f}s, final loss {loss:.4f}")
-
-    plain = lambda: mesh.query(qt, out=out)  # noqa: E731
-    learned = lambda: mesh.query(qt, out=out_l, learned_bound=True)  # noqa: E731
 
     t_plain, trav_plain = timed(plain, a.steps, a.warmup, mesh.last_traversal_ms)
     t_learn, trav_learn = timed(learned, a.steps, a.warmup, mesh.last_traversal_ms)
@@ -243,7 +260,6 @@ def main():
         return [int(x) for x in w.cpu().numpy()]
 
     wp, wl_ = work(), work(learned_bound=True)
-    K = info["branching"]
 
     def roof(w, ms, kernel, traffic_key):
         # main traversal launch only: per query 16 B (point in, distance out) + per interior node
@@ -258,7 +274,6 @@ def main():
                               "fallback_interior_nodes": round(w[6] / n, 3),
                               "fallback_triangles": round(w[7] / n, 3)}}
 
-    kname = f"sdf32_kernel<{K},{info['stack_capacity'] // 64},false,{{mode}}>"
     rl = roof(wl_, trav_learn, kname.format(mode=1), "seeded")
     rp = roof(wp, trav_plain, kname.format(mode=0), "plain")
 

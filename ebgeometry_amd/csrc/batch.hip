@@ -115,10 +115,9 @@ __global__ __launch_bounds__(256) void batch32_kernel(const BatchArgs a)
 // One wavefront per listed packet; the listed lanes redo their query in FP64, seeded with
 // (|d32| (1 + 1e-4) + slack_abs)^2, which contains the FP64 minimum.
 template <int K, int NB>
-__global__ __launch_bounds__(64) void batch64_kernel(const BatchArgs a)
+__device__ __forceinline__ void batch64_entry(const BatchArgs& a, uint32_t entry)
 {
-  if (blockIdx.x >= *a.recheck_count) return;
-  const SWords<4>    e    = sload<4>(a.recheck + blockIdx.x);
+  const SWords<4>    e    = sload<4>(a.recheck + entry);
   const uint32_t     p    = e.w[0];
   const uint64_t     mask = (uint64_t)e.w[2] | ((uint64_t)e.w[3] << 32);
   const SWords<4>    pk   = sload<4>(a.packets + 4 * (size_t)p);
@@ -149,6 +148,15 @@ __global__ __launch_bounds__(64) void batch64_kernel(const BatchArgs a)
   }
 }
 
+// One wavefront per workgroup; a fixed grid strides over the listed packets (their number is
+// only known on the device).
+template <int K, int NB>
+__global__ __launch_bounds__(64) void batch64_kernel(const BatchArgs a)
+{
+  const uint32_t count = *a.recheck_count;
+  for (uint32_t e = blockIdx.x; e < count; e += gridDim.x) batch64_entry<K, NB>(a, e);
+}
+
 template <int K, int NB>
 hipError_t launch32(const BatchArgs& a, int wg, bool count, hipStream_t s)
 {
@@ -164,7 +172,7 @@ template <int K, int NB>
 hipError_t launch64(const BatchArgs& a, hipStream_t s)
 {
   if (a.num_packets == 0) return hipSuccess;
-  hipLaunchKernelGGL((batch64_kernel<K, NB>), dim3(a.num_packets), dim3(64), 0, s, a);
+  hipLaunchKernelGGL((batch64_kernel<K, NB>), dim3(std::min<uint32_t>(a.num_packets, 8192u)), dim3(64), 0, s, a);
   return hipGetLastError();
 }
 

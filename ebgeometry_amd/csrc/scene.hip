@@ -177,11 +177,8 @@ __global__ __launch_bounds__(256) void scene32_kernel(const SceneDev sc, const Q
 // above, so every primitive that can attain the minimum is visited; the result replaces the
 // FP32 value only if some primitive came in at or below the seed.
 template <int KM, int NB>
-__global__ __launch_bounds__(256) void scene64_kernel(const SceneDev sc, const QueryArgs a)
+__device__ __forceinline__ void scene64_block(const SceneDev& sc, const QueryArgs& a, uint64_t n, uint64_t tid)
 {
-  const uint64_t n = a.count_ptr ? (uint64_t)*a.count_ptr : a.n;
-  if ((uint64_t)blockIdx.x * 256u >= n) return;
-  const uint64_t tid    = (uint64_t)blockIdx.x * 256u + threadIdx.x;
   const bool     active = tid < n;
   uint32_t       qi     = 0;
   double         px = 0.0, py = 0.0, pz = 0.0, seed = -1.0;
@@ -199,6 +196,14 @@ __global__ __launch_bounds__(256) void scene64_kernel(const SceneDev sc, const Q
   if (active && v < seed) a.out_d[qi] = (float)v;
 }
 
+// The list's length is only known on the device: a fixed grid strides over it.
+template <int KM, int NB>
+__global__ __launch_bounds__(256) void scene64_kernel(const SceneDev sc, const QueryArgs a)
+{
+  const uint64_t n = a.count_ptr ? (uint64_t)*a.count_ptr : a.n;
+  for (uint64_t b = blockIdx.x; b * 256u < n; b += gridDim.x) scene64_block<KM, NB>(sc, a, n, b * 256u + threadIdx.x);
+}
+
 template <int KM, int NB>
 hipError_t launch32(const SceneDev& sc, const QueryArgs& a, bool count, hipStream_t s)
 {
@@ -212,7 +217,7 @@ hipError_t launch32(const SceneDev& sc, const QueryArgs& a, bool count, hipStrea
 template <int KM, int NB>
 hipError_t launch64(const SceneDev& sc, const QueryArgs& a, hipStream_t s)
 {
-  const uint32_t blocks = (uint32_t)((a.n + 255) / 256);
+  const uint32_t blocks = std::min<uint32_t>((uint32_t)((a.n + 255) / 256), 2048u);
   if (blocks == 0) return hipSuccess;
   hipLaunchKernelGGL((scene64_kernel<KM, NB>), dim3(blocks), dim3(256), 0, s, sc, a);
   return hipGetLastError();

@@ -34,6 +34,9 @@ namespace {
 
 using namespace dev;
 
+// Workgroups of a list-driven launch (8 per CU); each strides over the list.
+constexpr uint32_t kListGrid = 2048;
+
 // Launch modes of the FP32 pass (distinct kernels, so that profiles separate them):
 //   kPlain     every query, unseeded
 //   kSeeded    every query, cull radius seed_r[q] (learned or caller-supplied bound)
@@ -101,18 +104,23 @@ __device__ __forceinline__ void sdf32_packet(const MeshDev& m, const QueryArgs& 
 template <int K, int NB, bool COUNT, int MODE>
 __global__ __launch_bounds__(256) void sdf32_kernel(const MeshDev m, const QueryArgs a)
 {
+  if (MODE == kFallback) {
+    // list-driven: the count is only known on the device, so a fixed grid strides over the
+    // list (a grid sized for the worst case would spend milliseconds retiring empty blocks)
+    const uint64_t n = *a.count_ptr;
+    for (uint64_t b = blockIdx.x; b * 256u < n; b += gridDim.x)
+      sdf32_packet<K, NB, COUNT, MODE>(m, a, n, b * 4u + (threadIdx.x >> 6));
+    return;
+  }
   const uint32_t blk = xcd_remap(blockIdx.x, gridDim.x);
-  const uint64_t n   = MODE == kFallback ? (uint64_t)*a.count_ptr : a.n;
+  const uint64_t n   = a.n;
   if ((uint64_t)blk * 256u >= n) return; // whole workgroup past the end (uniform per block)
   sdf32_packet<K, NB, COUNT, MODE>(m, a, n, (uint64_t)blk * 4u + (threadIdx.x >> 6));
 }
 
 template <int K, int NB>
-__global__ __launch_bounds__(256) void sdf64_kernel(const MeshDev m, const QueryArgs a)
+__device__ __forceinline__ void sdf64_block(const MeshDev& m, const QueryArgs& a, uint64_t n, uint64_t tid)
 {
-  const uint64_t tid = (uint64_t)blockIdx.x * 256u + threadIdx.x;
-  const uint64_t n   = a.count_ptr ? (uint64_t)*a.count_ptr : a.n;
-  if ((uint64_t)blockIdx.x * 256u >= n) return;
   const bool active = tid < n;
   uint32_t   qi     = 0;
   double     px = 0.0, py = 0.0, pz = 0.0, bound = -1.0;
@@ -141,6 +149,15 @@ __global__ __launch_bounds__(256) void sdf64_kernel(const MeshDev m, const Query
   if (a.fallback_list) wave_append(active && !found, qi, a.fallback_list, a.fallback_count);
 }
 
+// FP64 pass over a device-side list (a.idx, count *a.count_ptr) or over all a.n queries; a fixed
+// grid strides over the entries.
+template <int K, int NB>
+__global__ __launch_bounds__(256) void sdf64_kernel(const MeshDev m, const QueryArgs a)
+{
+  const uint64_t n = a.count_ptr ? (uint64_t)*a.count_ptr : a.n;
+  for (uint64_t b = blockIdx.x; b * 256u < n; b += gridDim.x) sdf64_block<K, NB>(m, a, n, b * 256u + threadIdx.x);
+}
+
 template <int K, int NB, int MODE>
 void launch32m(const MeshDev& m, const QueryArgs& a, bool count, uint32_t blocks, hipStream_t s)
 {
@@ -155,7 +172,7 @@ hipError_t launch32(const MeshDev& m, const QueryArgs& a, bool count, hipStream_
   if (blocks == 0) return hipSuccess;
   if (a.count_ptr) {
     if (!a.idx || a.seed_r) return hipErrorInvalidValue;
-    launch32m<K, NB, kFallback>(m, a, count, blocks, s);
+    launch32m<K, NB, kFallback>(m, a, count, std::min<uint32_t>(blocks, kListGrid), s);
   }
   else if (a.seed_r) {
     if (!a.fallback_list || !a.fallback_count) return hipErrorInvalidValue;
@@ -170,7 +187,7 @@ hipError_t launch32(const MeshDev& m, const QueryArgs& a, bool count, hipStream_
 template <int K, int NB>
 hipError_t launch64(const MeshDev& m, const QueryArgs& a, hipStream_t s)
 {
-  const uint32_t blocks = (uint32_t)((a.n + 255) / 256);
+  const uint32_t blocks = std::min<uint32_t>((uint32_t)((a.n + 255) / 256), kListGrid);
   if (blocks == 0) return hipSuccess;
   hipLaunchKernelGGL((sdf64_kernel<K, NB>), dim3(blocks), dim3(256), 0, s, m, a);
   return hipGetLastError();
