@@ -4,9 +4,9 @@
 //   1. Morton presort: query bbox -> 63-bit codes -> stable radix sort -> permutation
 //   2. [learned bound] MLP forward -> per-query seeded cull radius
 //   3. FP32 packet traversal over the permutation (writes d, face id by query index)
-//   4. [learned bound] FP32 traversal again, unseeded, for queries whose seeded radius held
-//      no triangle (exactness fallback)
-//   5. FP64 traversal for queries with |d| < recheck_eps * diag (sign is ill-conditioned in
+//      with the exactness fallback inside the same wavefront: queries whose seeded radius held
+//      no triangle walk the tree again unseeded
+//   4. FP64 traversal for queries with |d| < recheck_eps * diag (sign is ill-conditioned in
 //      FP32 there), seeded with |d32| + slack
 #include "host_common.hpp"
 #include "mesh_build.hpp"
@@ -267,7 +267,7 @@ ebg_status ebg_query_sdf(const ebg_mesh* cm, const float* d_pts, uint64_t n, flo
   a.recheck_abs    = o.fp64_all ? INFINITY : (recheck ? float(o.recheck_eps * m->diag) : 0.0f);
   a.recheck_list   = recheck ? ws.recheck_list() : nullptr;
   a.recheck_count  = cnt;
-  a.fallback_list  = seeds ? ws.fallback_list() : nullptr;
+  a.fallback_list  = nullptr;
   a.fallback_count = cnt + 1;
   a.work           = reinterpret_cast<unsigned long long*>(o.work);
   const bool count = o.work != nullptr;
@@ -277,14 +277,6 @@ ebg_status ebg_query_sdf(const ebg_mesh* cm, const float* d_pts, uint64_t n, flo
   EBG_HIP(ebgpu::launch_sdf32(m->dev, m->K, m->banks, a, count, s));
   EBG_HIP(hipEventRecord(m->ev1, s));
   m->timed = true;
-  if (seeds) {
-    ebgpu::QueryArgs f = a;
-    f.idx              = ws.fallback_list();
-    f.count_ptr        = cnt + 1;
-    f.seed_r           = nullptr;
-    f.fallback_list    = nullptr;
-    EBG_HIP(ebgpu::launch_sdf32(m->dev, m->K, m->banks, f, count, s));
-  }
 
   if (recheck) {
     ebgpu::QueryArgs r = a;

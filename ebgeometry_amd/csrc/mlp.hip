@@ -12,6 +12,7 @@
 
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 #include <cstring>
 #include <vector>
@@ -57,16 +58,21 @@ __device__ __forceinline__ float feature(int f, float x0, float x1, float x2)
 }
 
 // ------------------------------------------------------------------------------------------
-// fused inference: one wavefront = 64 queries (4 tiles of 16); 4 waves per workgroup
+// fused inference: one wavefront = 64 queries (4 tiles of 16); 4 waves per workgroup.
+// The BF16 weight matrices (112 KB, the inference K order) are copied once per workgroup into
+// LDS and the workgroup then strides over query tiles (one workgroup per CU), so every A
+// fragment comes from LDS instead of an L2 round trip per wavefront.
 // ------------------------------------------------------------------------------------------
-__global__ __launch_bounds__(256) void infer_kernel(const float* __restrict__ pts, const uint32_t* __restrict__ idx,
-                                                     uint64_t n, MlpDev p, float scale, float offset, int seed_mode,
-                                                     float* __restrict__ out)
+
+constexpr int kInferWaves = 4;
+
+__device__ __forceinline__ void infer_tile(const uint16_t* __restrict__ sw, const float* __restrict__ pts,
+                                           const uint32_t* __restrict__ idx, uint64_t n, uint64_t base,
+                                           const MlpDev& p, float scale, float offset, int seed_mode,
+                                           float* __restrict__ out)
 {
-  const int      lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  const int      nn = lane & 15, g = lane >> 4;
-  const uint64_t base = ((uint64_t)blockIdx.x * 4 + wave) * 64;
-  if (base >= n) return;
+  const int lane = threadIdx.x & 63;
+  const int nn = lane & 15, g = lane >> 4;
 
   u16x8    bq[4][4];
   uint32_t qid[4];
@@ -95,7 +101,7 @@ __global__ __launch_bounds__(256) void infer_kernel(const float* __restrict__ pt
   for (int t = 0; t < 8; t++)
 #pragma unroll
     for (int b = 0; b < 2; b++) {
-      const u16x8 a = *reinterpret_cast<const u16x8*>(p.winfer + kBfW1 + (16 * t + nn) * 64 + 32 * b + 8 * g);
+      const u16x8 a = *reinterpret_cast<const u16x8*>(sw + kBfW1 + (16 * t + nn) * 64 + 32 * b + 8 * g);
 #pragma unroll
       for (int q = 0; q < 4; q++) acc[q][t] = mfma(a, bq[q][b], acc[q][t]);
     }
@@ -129,7 +135,7 @@ __global__ __launch_bounds__(256) void infer_kernel(const float* __restrict__ pt
     for (int t = 0; t < 8; t++)
 #pragma unroll
       for (int b = 0; b < 4; b++) {
-        const u16x8 a = *reinterpret_cast<const u16x8*>(p.winfer + wofs[L] + (16 * t + nn) * 128 + 32 * b + 8 * g);
+        const u16x8 a = *reinterpret_cast<const u16x8*>(sw + wofs[L] + (16 * t + nn) * 128 + 32 * b + 8 * g);
 #pragma unroll
         for (int q = 0; q < 4; q++) acc[q][t] = mfma(a, bq[q][b], acc[q][t]);
       }
@@ -154,6 +160,23 @@ __global__ __launch_bounds__(256) void infer_kernel(const float* __restrict__ pt
       const float y = (expf(s + b5) - kMlpLogC) * p.diag;
       out[qid[q]]   = seed_mode ? scale * fmaxf(y, 0.0f) + offset : y;
     }
+  }
+}
+
+__global__ __launch_bounds__(64 * kInferWaves) void infer_kernel(const float* __restrict__ pts,
+                                                                 const uint32_t* __restrict__ idx, uint64_t n, MlpDev p,
+                                                                 float scale, float offset, int seed_mode,
+                                                                 float* __restrict__ out)
+{
+  __shared__ __attribute__((aligned(16))) uint16_t sw[kBfTotal];
+  for (int i = 8 * threadIdx.x; i < kBfTotal; i += 8 * 64 * kInferWaves)
+    *reinterpret_cast<u16x8*>(sw + i) = *reinterpret_cast<const u16x8*>(p.winfer + i);
+  __syncthreads();
+  // every wavefront runs the same number of iterations of this loop shape (no barrier inside)
+  const uint64_t per = 64u * kInferWaves;
+  for (uint64_t tile = blockIdx.x; tile * per < n; tile += gridDim.x) {
+    const uint64_t base = tile * per + 64u * (threadIdx.x >> 6);
+    if (base < n) infer_tile(sw, pts, idx, n, base, p, scale, offset, seed_mode, out);
   }
 }
 
@@ -544,8 +567,16 @@ hipError_t mlp_forward(const Mlp& m, const float* pts, const uint32_t* idx, uint
                        float offset, float* out, hipStream_t s)
 {
   if (n == 0) return hipSuccess;
-  const uint64_t blocks = (n + 255) / 256;
-  hipLaunchKernelGGL(infer_kernel, dim3((uint32_t)blocks), dim3(256), 0, s, pts, idx, n, dev_of(m), scale, offset,
+  // one workgroup per CU (the weights take 112 KB of the CU's LDS)
+  static int cus = 0;
+  if (!cus) {
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess || hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || cus <= 0)
+      cus = 256;
+  }
+  const uint64_t tiles  = (n + 64 * kInferWaves - 1) / (64 * kInferWaves);
+  const uint32_t blocks = (uint32_t)std::min<uint64_t>(tiles, (uint64_t)cus);
+  hipLaunchKernelGGL(infer_kernel, dim3(blocks), dim3(64 * kInferWaves), 0, s, pts, idx, n, dev_of(m), scale, offset,
                      seed_mode ? 1 : 0, out);
   return hipGetLastError();
 }

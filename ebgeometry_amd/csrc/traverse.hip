@@ -39,13 +39,12 @@ constexpr uint32_t kListGrid = 2048;
 
 // Launch modes of the FP32 pass (distinct kernels, so that profiles separate them):
 //   kPlain     every query, unseeded
-//   kSeeded    every query, cull radius seed_r[q] (learned or caller-supplied bound)
-//   kFallback  the queries of a device-side list (count read from the device), unseeded
+//   kSeeded    every query, cull radius seed_r[q] (learned or caller-supplied bound), with the
+//              unseeded redo of the queries whose radius held no triangle
 enum Mode : int
 {
-  kPlain    = 0,
-  kSeeded   = 1,
-  kFallback = 2
+  kPlain  = 0,
+  kSeeded = 1
 };
 
 // One packet: the 64 queries pk*64 .. pk*64+63 of the (presorted) query order, one per lane.
@@ -76,23 +75,48 @@ __device__ __forceinline__ void sdf32_packet(const MeshDev& m, const QueryArgs& 
 
   // Exact only if some triangle lies within the seeded radius: otherwise nodes between the
   // seeded radius and the true distance were pruned, and the query must be redone unseeded.
-  const bool found = active && btri != kInvalid && best <= seed2;
-  float      d     = 0.0f;
+  // The redo runs here, in the same wavefront (the other lanes sit it out with bound -1), so a
+  // rare expensive query is hidden among the other wavefronts of the launch instead of being
+  // serialised in a launch of its own.
+  bool found = active && btri != kInvalid && best <= seed2;
+  if (MODE == kSeeded) {
+    const bool     redo  = active && !found;
+    const uint64_t rmask = __ballot(redo);
+    if (rmask) {
+      float    best2 = INFINITY;
+      uint32_t btri2 = kInvalid, bfeat2 = 0, f_inner = 0, f_tris = 0;
+      traverse<K, NB, float, COUNT>(m, px, py, pz, redo ? INFINITY : -1.0f, best2, btri2, bfeat2, f_inner, f_tris,
+                                    w_inner, w_tris);
+      if (redo) {
+        best  = best2;
+        btri  = btri2;
+        bfeat = bfeat2;
+      }
+      found = active && btri != kInvalid;
+      if (__lane_id() == 0) atomicAdd(a.fallback_count, (uint32_t)__popcll(rmask));
+      if (COUNT && a.work) {
+        const uint64_t fi = wave_sum_u64(redo ? (uint64_t)f_inner : 0ull);
+        const uint64_t ft = wave_sum_u64(redo ? (uint64_t)f_tris : 0ull);
+        if (__lane_id() == 0) {
+          atomicAdd(a.work + 6, (unsigned long long)fi);
+          atomicAdd(a.work + 7, (unsigned long long)ft);
+        }
+      }
+    }
+  }
+  float d = 0.0f;
   if (found) {
     d = resolve32(m, btri, bfeat, best, px, py, pz);
     a.out_d[qi] = d;
     if (a.out_tri) a.out_tri[qi] = m.tris[btri].face;
   }
-  if (MODE == kSeeded) wave_append(active && !found, qi, a.fallback_list, a.fallback_count);
   if (a.recheck_list) wave_append(found && fabsf(d) < a.recheck_abs, qi, a.recheck_list, a.recheck_count);
   if (COUNT && a.work) {
-    // per-query counts of the fallback pass go to their own slots ([6], [7])
-    const int      slot = MODE == kFallback ? 6 : 0;
-    const uint64_t si   = wave_sum_u64(active ? (uint64_t)n_inner : 0ull);
-    const uint64_t st   = wave_sum_u64(active ? (uint64_t)n_tris : 0ull);
+    const uint64_t si = wave_sum_u64(active ? (uint64_t)n_inner : 0ull);
+    const uint64_t st = wave_sum_u64(active ? (uint64_t)n_tris : 0ull);
     if (__lane_id() == 0) {
-      atomicAdd(a.work + slot, (unsigned long long)si);
-      atomicAdd(a.work + slot + 1, (unsigned long long)st);
+      atomicAdd(a.work, (unsigned long long)si);
+      atomicAdd(a.work + 1, (unsigned long long)st);
       atomicAdd(a.work + 4, (unsigned long long)w_inner);
       atomicAdd(a.work + 5, (unsigned long long)w_tris);
     }
@@ -104,14 +128,6 @@ __device__ __forceinline__ void sdf32_packet(const MeshDev& m, const QueryArgs& 
 template <int K, int NB, bool COUNT, int MODE>
 __global__ __launch_bounds__(256) void sdf32_kernel(const MeshDev m, const QueryArgs a)
 {
-  if (MODE == kFallback) {
-    // list-driven: the count is only known on the device, so a fixed grid strides over the
-    // list (a grid sized for the worst case would spend milliseconds retiring empty blocks)
-    const uint64_t n = *a.count_ptr;
-    for (uint64_t b = blockIdx.x; b * 256u < n; b += gridDim.x)
-      sdf32_packet<K, NB, COUNT, MODE>(m, a, n, b * 4u + (threadIdx.x >> 6));
-    return;
-  }
   const uint32_t blk = xcd_remap(blockIdx.x, gridDim.x);
   const uint64_t n   = a.n;
   if ((uint64_t)blk * 256u >= n) return; // whole workgroup past the end (uniform per block)
@@ -170,12 +186,9 @@ hipError_t launch32(const MeshDev& m, const QueryArgs& a, bool count, hipStream_
 {
   const uint32_t blocks = (uint32_t)((a.n + 255) / 256);
   if (blocks == 0) return hipSuccess;
-  if (a.count_ptr) {
-    if (!a.idx || a.seed_r) return hipErrorInvalidValue;
-    launch32m<K, NB, kFallback>(m, a, count, std::min<uint32_t>(blocks, kListGrid), s);
-  }
-  else if (a.seed_r) {
-    if (!a.fallback_list || !a.fallback_count) return hipErrorInvalidValue;
+  if (a.count_ptr) return hipErrorInvalidValue;
+  if (a.seed_r) {
+    if (!a.fallback_count) return hipErrorInvalidValue;
     launch32m<K, NB, kSeeded>(m, a, count, blocks, s);
   }
   else {
