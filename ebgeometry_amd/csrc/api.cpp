@@ -2,7 +2,8 @@
 //
 // Query pipeline for one batch (all on the caller's stream, no host synchronisation):
 //   1. Morton presort: query bbox -> 63-bit codes -> stable radix sort -> permutation
-//   2. [learned bound] MLP forward -> per-query seeded cull radius
+//   2. [learned bound] MLP forward -> per-query seeded cull radius (on a second stream,
+//      concurrently with step 1)
 //   3. FP32 packet traversal over the permutation (writes d, face id by query index)
 //      with the exactness fallback inside the same wavefront: queries whose seeded radius held
 //      no triangle walk the tree again unseeded
@@ -243,18 +244,29 @@ ebg_status ebg_query_sdf(const ebg_mesh* cm, const float* d_pts, uint64_t n, flo
   uint32_t* cnt = ws.counters(); // [0] recheck, [1] fallback
   EBG_HIP(ebgpu::launch_fill_u32(cnt, 2, 0u, s));
 
+  // The bound network needs no presort (it writes the radius by query index), so it runs on a
+  // second stream of the handle, in query order, while the presort runs on the caller's stream.
+  const float* seeds = o.seed_radius;
+  if (o.learned_bound) {
+    if (!m->aux) {
+      EBG_HIP(hipStreamCreateWithFlags(&m->aux, hipStreamNonBlocking));
+      EBG_HIP(hipEventCreateWithFlags(&m->ev_fork, hipEventDisableTiming));
+      EBG_HIP(hipEventCreateWithFlags(&m->ev_join, hipEventDisableTiming));
+    }
+    EBG_HIP(hipEventRecord(m->ev_fork, s));
+    EBG_HIP(hipStreamWaitEvent(m->aux, m->ev_fork, 0));
+    EBG_HIP(ebgpu::mlp_forward(m->mlp, d_pts, nullptr, n, true, o.bound_scale, float(o.bound_offset * m->diag),
+                               ws.seeds.as<float>(), m->aux));
+    EBG_HIP(hipEventRecord(m->ev_join, m->aux));
+    seeds = ws.seeds.as<float>();
+  }
+
   const uint32_t* idx = nullptr;
   if (o.presort && n > 1) {
     EBG_HIP(ws.presort(d_pts, n, s));
     idx = ws.perm.as<uint32_t>();
   }
-
-  const float* seeds = o.seed_radius;
-  if (o.learned_bound) {
-    EBG_HIP(ebgpu::mlp_forward(m->mlp, d_pts, idx, n, true, o.bound_scale, float(o.bound_offset * m->diag),
-                               ws.seeds.as<float>(), s));
-    seeds = ws.seeds.as<float>();
-  }
+  if (o.learned_bound) EBG_HIP(hipStreamWaitEvent(s, m->ev_join, 0));
 
   const bool recheck = o.fp64_recheck || o.fp64_all;
   ebgpu::QueryArgs a{};
@@ -395,7 +407,7 @@ void ebg_mlp_opts_default(ebg_mlp_opts* o)
   o->seed        = 12345;
   o->near_sigma  = 0.02f;
   o->box_scale   = 1.2f;
-  o->under_weight = 8.0f;
+  o->under_weight = 16.0f;
 }
 
 uint64_t ebg_mlp_param_count(void) { return uint64_t(ebgpu::kMlpParams); }

@@ -130,6 +130,8 @@ struct ebg_mesh
   std::recursive_mutex   mu; // training runs queries while holding it
   ebgpu::Workspace       ws;
   hipEvent_t             ev0 = nullptr, ev1 = nullptr;
+  hipStream_t            aux = nullptr; // bound network, concurrent with the presort
+  hipEvent_t             ev_fork = nullptr, ev_join = nullptr;
   bool                   timed = false;
   ebgpu::Mlp             mlp;
   std::atomic<int>       refs{1};
@@ -138,6 +140,9 @@ struct ebg_mesh
   {
     if (ev0) (void)hipEventDestroy(ev0);
     if (ev1) (void)hipEventDestroy(ev1);
+    if (ev_fork) (void)hipEventDestroy(ev_fork);
+    if (ev_join) (void)hipEventDestroy(ev_join);
+    if (aux) (void)hipStreamDestroy(aux);
   }
 };
 

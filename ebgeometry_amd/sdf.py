@@ -170,7 +170,7 @@ class TriMeshSDF(SignedDistanceFunction):
     # -- learned bound --------------------------------------------------------------------
 
     def train_bound(self, num_samples: int = 4_000_000, epochs: int = 8, batch: int = 4096, lr: float = 2e-3,
-                    seed: int = 12345, near_sigma: float = 0.02, box_scale: float = 1.2, under_weight: float = 8.0,
+                    seed: int = 12345, near_sigma: float = 0.02, box_scale: float = 1.2, under_weight: float = 16.0,
                     stream=None) -> float:
         """Train the mesh's learned cull bound on the GPU; returns the final mean loss."""
         o = _lib.MlpOpts()

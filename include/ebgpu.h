@@ -300,7 +300,7 @@ typedef struct ebg_mlp_opts
   float    near_sigma;   /* surface offset sigma for half the samples (default 0.02)         */
   float    box_scale;    /* uniform half sampled in box_scale x bbox (default 1.2)            */
   void*    stream;
-  float    under_weight; /* loss weight of under-predictions (default 8): higher = fewer
+  float    under_weight; /* loss weight of under-predictions (default 16): higher = fewer
                             fallbacks, looser radii                                          */
   int32_t  reserved;
 } ebg_mlp_opts;
