@@ -25,12 +25,8 @@ typedef __bf16   bf16x8 __attribute__((ext_vector_type(8)));
 typedef float    f32x4 __attribute__((ext_vector_type(4)));
 typedef uint16_t u16x8 __attribute__((ext_vector_type(8)));
 
-__device__ __forceinline__ uint16_t f2bf(float x)
-{
-  uint32_t u = __float_as_uint(x);
-  u += 0x7FFFu + ((u >> 16) & 1u); // round to nearest even (inputs are finite)
-  return (uint16_t)(u >> 16);
-}
+// FP32 -> BF16, round to nearest even (v_cvt_pk_bf16_f32 on gfx950)
+__device__ __forceinline__ uint16_t f2bf(float x) { return __builtin_bit_cast(uint16_t, (__bf16)x); }
 __device__ __forceinline__ float bf2f(uint16_t h) { return __uint_as_float((uint32_t)h << 16); }
 __device__ __forceinline__ f32x4 mfma(u16x8 a, u16x8 b, f32x4 c)
 {
@@ -38,7 +34,8 @@ __device__ __forceinline__ f32x4 mfma(u16x8 a, u16x8 b, f32x4 c)
                                                  0, 0);
 }
 
-// Fourier frequencies w_k = pi * 2^(0.4 k), k = 0..15 (float32-rounded, same table on the host)
+// Fourier frequencies w_k = pi * 2^(0.4 k), k = 0..15, in revolutions per unit (w_k / 2 pi,
+// float32-rounded): v_sin_f32 takes its argument in revolutions.
 __constant__ float c_freq[kMlpFreqs];
 
 struct MlpDev
@@ -54,7 +51,7 @@ __device__ __forceinline__ float feature(int f, float x0, float x1, float x2)
   const int   c = f < 3 ? f : (f - 3) % 3;
   const float x = c == 0 ? x0 : (c == 1 ? x1 : x2);
   if (f < 3) return x;
-  return sinf(c_freq[(f - 3) / 3] * x);
+  return __builtin_amdgcn_sinf(c_freq[(f - 3) / 3] * x);
 }
 
 // ------------------------------------------------------------------------------------------
@@ -64,7 +61,8 @@ __device__ __forceinline__ float feature(int f, float x0, float x1, float x2)
 // fragment comes from LDS instead of an L2 round trip per wavefront.
 // ------------------------------------------------------------------------------------------
 
-constexpr int kInferWaves = 4;
+constexpr int kInferWaves = 8; // 2 per SIMD
+constexpr int kInferQT    = 2; // 16-query tiles per wavefront (32 queries)
 
 __device__ __forceinline__ void infer_tile(const uint16_t* __restrict__ sw, const float* __restrict__ pts,
                                            const uint32_t* __restrict__ idx, uint64_t n, uint64_t base,
@@ -74,11 +72,11 @@ __device__ __forceinline__ void infer_tile(const uint16_t* __restrict__ sw, cons
   const int lane = threadIdx.x & 63;
   const int nn = lane & 15, g = lane >> 4;
 
-  u16x8    bq[4][4];
-  uint32_t qid[4];
-  bool     valid[4];
+  u16x8    bq[kInferQT][4];
+  uint32_t qid[kInferQT];
+  bool     valid[kInferQT];
 #pragma unroll
-  for (int q = 0; q < 4; q++) {
+  for (int q = 0; q < kInferQT; q++) {
     const uint64_t i = base + q * 16 + nn;
     valid[q]         = i < n;
     qid[q]           = valid[q] ? (idx ? idx[i] : (uint32_t)i) : 0u;
@@ -91,9 +89,9 @@ __device__ __forceinline__ void infer_tile(const uint16_t* __restrict__ sw, cons
       for (int e = 0; e < 8; e++) bq[q][b][e] = f2bf(feature(32 * b + 8 * g + e, x[0], x[1], x[2]));
   }
 
-  f32x4 acc[4][8];
+  f32x4 acc[kInferQT][8];
 #pragma unroll
-  for (int q = 0; q < 4; q++)
+  for (int q = 0; q < kInferQT; q++)
 #pragma unroll
     for (int t = 0; t < 8; t++) acc[q][t] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
   // layer 1 (K = 64)
@@ -103,7 +101,7 @@ __device__ __forceinline__ void infer_tile(const uint16_t* __restrict__ sw, cons
     for (int b = 0; b < 2; b++) {
       const u16x8 a = *reinterpret_cast<const u16x8*>(sw + kBfW1 + (16 * t + nn) * 64 + 32 * b + 8 * g);
 #pragma unroll
-      for (int q = 0; q < 4; q++) acc[q][t] = mfma(a, bq[q][b], acc[q][t]);
+      for (int q = 0; q < kInferQT; q++) acc[q][t] = mfma(a, bq[q][b], acc[q][t]);
     }
   // layers 2..4 (K = 128)
   const int wofs[3] = {kBfW2, kBfW3, kBfW4};
@@ -120,7 +118,7 @@ __device__ __forceinline__ void infer_tile(const uint16_t* __restrict__ sw, cons
         bl[e + 4] = bias[32 * b + 16 + 4 * g + e];
       }
 #pragma unroll
-      for (int q = 0; q < 4; q++)
+      for (int q = 0; q < kInferQT; q++)
 #pragma unroll
         for (int e = 0; e < 8; e++) {
           const float z = (e < 4 ? acc[q][2 * b][e] : acc[q][2 * b + 1][e - 4]) + bl[e];
@@ -128,7 +126,7 @@ __device__ __forceinline__ void infer_tile(const uint16_t* __restrict__ sw, cons
         }
     }
 #pragma unroll
-    for (int q = 0; q < 4; q++)
+    for (int q = 0; q < kInferQT; q++)
 #pragma unroll
       for (int t = 0; t < 8; t++) acc[q][t] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
 #pragma unroll
@@ -137,7 +135,7 @@ __device__ __forceinline__ void infer_tile(const uint16_t* __restrict__ sw, cons
       for (int b = 0; b < 4; b++) {
         const u16x8 a = *reinterpret_cast<const u16x8*>(sw + wofs[L] + (16 * t + nn) * 128 + 32 * b + 8 * g);
 #pragma unroll
-        for (int q = 0; q < 4; q++) acc[q][t] = mfma(a, bq[q][b], acc[q][t]);
+        for (int q = 0; q < kInferQT; q++) acc[q][t] = mfma(a, bq[q][b], acc[q][t]);
       }
   }
   // output layer: y = relu(h4 + b4) . w5 + b5 (h4 rounded to bf16 as in training)
@@ -145,7 +143,7 @@ __device__ __forceinline__ void infer_tile(const uint16_t* __restrict__ sw, cons
   const float* w5 = p.master + kOffW5;
   const float  b5 = p.master[kOffB5];
 #pragma unroll
-  for (int q = 0; q < 4; q++) {
+  for (int q = 0; q < kInferQT; q++) {
     float s = 0.0f;
 #pragma unroll
     for (int t = 0; t < 8; t++)
@@ -173,9 +171,9 @@ __global__ __launch_bounds__(64 * kInferWaves) void infer_kernel(const float* __
     *reinterpret_cast<u16x8*>(sw + i) = *reinterpret_cast<const u16x8*>(p.winfer + i);
   __syncthreads();
   // every wavefront runs the same number of iterations of this loop shape (no barrier inside)
-  const uint64_t per = 64u * kInferWaves;
+  const uint64_t per = 16u * kInferQT * kInferWaves;
   for (uint64_t tile = blockIdx.x; tile * per < n; tile += gridDim.x) {
-    const uint64_t base = tile * per + 64u * (threadIdx.x >> 6);
+    const uint64_t base = tile * per + 16u * kInferQT * (threadIdx.x >> 6);
     if (base < n) infer_tile(sw, pts, idx, n, base, p, scale, offset, seed_mode, out);
   }
 }
@@ -436,7 +434,7 @@ hipError_t upload_freqs()
   static bool done = false;
   if (done) return hipSuccess;
   float f[kMlpFreqs];
-  for (int k = 0; k < kMlpFreqs; k++) f[k] = (float)(M_PI * std::pow(2.0, 0.4 * k));
+  for (int k = 0; k < kMlpFreqs; k++) f[k] = (float)(0.5 * std::pow(2.0, 0.4 * k)); // pi 2^(0.4k) / (2 pi)
   hipError_t e = hipMemcpyToSymbol(HIP_SYMBOL(c_freq), f, sizeof(f));
   if (e == hipSuccess) done = true;
   return e;
@@ -574,7 +572,7 @@ hipError_t mlp_forward(const Mlp& m, const float* pts, const uint32_t* idx, uint
     if (hipGetDevice(&dev) != hipSuccess || hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || cus <= 0)
       cus = 256;
   }
-  const uint64_t tiles  = (n + 64 * kInferWaves - 1) / (64 * kInferWaves);
+  const uint64_t tiles  = (n + 16 * kInferQT * kInferWaves - 1) / (16 * kInferQT * kInferWaves);
   const uint32_t blocks = (uint32_t)std::min<uint64_t>(tiles, (uint64_t)cus);
   hipLaunchKernelGGL(infer_kernel, dim3(blocks), dim3(64 * kInferWaves), 0, s, pts, idx, n, dev_of(m), scale, offset,
                      seed_mode ? 1 : 0, out);
