@@ -175,10 +175,19 @@ struct LaneStack
 template <class R>
 __device__ __forceinline__ R box_d2(R px, R py, R pz, float lx, float ly, float lz, float hx, float hy, float hz)
 {
-  const R dx = max(R(0), max(R(lx) - px, px - R(hx)));
-  const R dy = max(R(0), max(R(ly) - py, py - R(hy)));
-  const R dz = max(R(0), max(R(lz) - pz, pz - R(hz)));
-  return dx * dx + dy * dy + dz * dz;
+  if constexpr (sizeof(R) == 4) {
+    // p minus its clamp to the box (v_med3_f32): zero inside, the gap outside
+    const float dx = px - __builtin_amdgcn_fmed3f(px, lx, hx);
+    const float dy = py - __builtin_amdgcn_fmed3f(py, ly, hy);
+    const float dz = pz - __builtin_amdgcn_fmed3f(pz, lz, hz);
+    return dx * dx + dy * dy + dz * dz;
+  }
+  else {
+    const R dx = max(R(0), max(R(lx) - px, px - R(hx)));
+    const R dy = max(R(0), max(R(ly) - py, py - R(hy)));
+    const R dz = max(R(0), max(R(lz) - pz, pz - R(hz)));
+    return dx * dx + dy * dy + dz * dz;
+  }
 }
 
 // Squared distance from p to one triangle and the closest feature (FP32, precomputed record).
@@ -188,37 +197,31 @@ __device__ __forceinline__ void tri_eval32(const TriRec& r, float px, float py, 
   const float p2x = p1x - r.e0[0], p2y = p1y - r.e0[1], p2z = p1z - r.e0[2];
   const float p3x = p1x + r.e2[0], p3y = p1y + r.e2[1], p3z = p1z + r.e2[2];
 
-  const bool s0 = (r.c0[0] * p1x + r.c0[1] * p1y + r.c0[2] * p1z) > 0.0f;
-  const bool s1 = (r.c1[0] * p2x + r.c1[1] * p2y + r.c1[2] * p2z) > 0.0f;
-  const bool s2 = (r.c2[0] * p3x + r.c2[1] * p3y + r.c2[2] * p3z) > 0.0f;
-  const bool inface = (s0 == s1) && (s1 == s2);
-  const float fd    = r.n[0] * p1x + r.n[1] * p1y + r.n[2] * p1z;
+  const bool  s0 = (r.c0[0] * p1x + r.c0[1] * p1y + r.c0[2] * p1z) > 0.0f;
+  const bool  s1 = (r.c1[0] * p2x + r.c1[1] * p2y + r.c1[2] * p2z) > 0.0f;
+  const bool  s2 = (r.c2[0] * p3x + r.c2[1] * p3y + r.c2[2] * p3z) > 0.0f;
+  const bool  inface = (s0 == s1) && (s1 == s2);
+  const float fd     = r.n[0] * p1x + r.n[1] * p1y + r.n[2] * p1z;
 
-  float    r2 = p1x * p1x + p1y * p1y + p1z * p1z;
-  uint32_t f  = kVert0;
-  float    q  = p2x * p2x + p2y * p2y + p2z * p2z;
-  if (q < r2) { r2 = q; f = kVert1; }
-  q = p3x * p3x + p3y * p3y + p3z * p3z;
-  if (q < r2) { r2 = q; f = kVert2; }
-
-  float t = (p1x * r.e0[0] + p1y * r.e0[1] + p1z * r.e0[2]) * r.inv[0];
-  if (t > 0.0f && t < 1.0f) {
-    const float yx = p1x - t * r.e0[0], yy = p1y - t * r.e0[1], yz = p1z - t * r.e0[2];
-    q              = yx * yx + yy * yy + yz * yz;
-    if (q < r2) { r2 = q; f = kEdge0; }
-  }
-  t = (p2x * r.e1[0] + p2y * r.e1[1] + p2z * r.e1[2]) * r.inv[1];
-  if (t > 0.0f && t < 1.0f) {
-    const float yx = p2x - t * r.e1[0], yy = p2y - t * r.e1[1], yz = p2z - t * r.e1[2];
-    q              = yx * yx + yy * yy + yz * yz;
-    if (q < r2) { r2 = q; f = kEdge1; }
-  }
-  t = (p3x * r.e2[0] + p3y * r.e2[1] + p3z * r.e2[2]) * r.inv[2];
-  if (t > 0.0f && t < 1.0f) {
-    const float yx = p3x - t * r.e2[0], yy = p3y - t * r.e2[1], yz = p3z - t * r.e2[2];
-    q              = yx * yx + yy * yy + yz * yz;
-    if (q < r2) { r2 = q; f = kEdge2; }
-  }
+  // Outside the face region: the closest point of each edge at its clamped parameter (t <= 0
+  // gives the edge's first vertex, t >= 1 its second). The minimum over the three edges is the
+  // reference's minimum over the three vertices and the edge interiors with t in (0,1); at the
+  // clamps the same vertex vectors p1, p2, p3 are formed, so the distances agree bit for bit
+  // (up to the rounding of p3 - e2 and p2 - e1 against p1 and p3).
+  const float t0  = __builtin_amdgcn_fmed3f((p1x * r.e0[0] + p1y * r.e0[1] + p1z * r.e0[2]) * r.inv[0], 0.0f, 1.0f);
+  const float t1  = __builtin_amdgcn_fmed3f((p2x * r.e1[0] + p2y * r.e1[1] + p2z * r.e1[2]) * r.inv[1], 0.0f, 1.0f);
+  const float t2  = __builtin_amdgcn_fmed3f((p3x * r.e2[0] + p3y * r.e2[1] + p3z * r.e2[2]) * r.inv[2], 0.0f, 1.0f);
+  const float y0x = p1x - t0 * r.e0[0], y0y = p1y - t0 * r.e0[1], y0z = p1z - t0 * r.e0[2];
+  const float y1x = p2x - t1 * r.e1[0], y1y = p2y - t1 * r.e1[1], y1z = p2z - t1 * r.e1[2];
+  const float y2x = p3x - t2 * r.e2[0], y2y = p3y - t2 * r.e2[1], y2z = p3z - t2 * r.e2[2];
+  float       r2  = y0x * y0x + y0y * y0y + y0z * y0z;
+  const float q1  = y1x * y1x + y1y * y1y + y1z * y1z;
+  const float q2  = y2x * y2x + y2y * y2y + y2z * y2z;
+  uint32_t    e   = 0;
+  float       te  = t0;
+  if (q1 < r2) { r2 = q1; e = 1; te = t1; }
+  if (q2 < r2) { r2 = q2; e = 2; te = t2; }
+  const uint32_t f = te <= 0.0f ? kVert0 + e : (te >= 1.0f ? kVert0 + (e == 2 ? 0u : e + 1u) : kEdge0 + e);
   d2   = inface ? fd * fd : r2;
   feat = inface ? (uint32_t)kFace : f;
 }
@@ -450,13 +453,18 @@ __device__ __forceinline__ void traverse(const MeshDev& m, R px, R py, R pz, R b
         while (need) {
           const uint32_t t = first + c0 + (uint32_t)__builtin_ctz(need);
           need &= need - 1u;
-          if (COUNT) w_tris++;
           R        d2;
           uint32_t f;
           if constexpr (sizeof(R) == 4) {
-            tri_eval32(sload_record(m.tris + t), px, py, pz, d2, f);
+            const TriRec rec = sload_record(m.tris + t);
+            // the distance to the triangle's plane is a lower bound on the distance to it
+            const float  fd  = rec.n[0] * (px - rec.v0[0]) + rec.n[1] * (py - rec.v0[1]) + rec.n[2] * (pz - rec.v0[2]);
+            if (__ballot(fd * fd <= min(bound, best)) == 0) continue;
+            if (COUNT) w_tris++;
+            tri_eval32(rec, px, py, pz, d2, f);
           }
           else {
+            if (COUNT) w_tris++;
             const SWords<12> pos = sload<12>(m.tripos + 12 * (size_t)t);
             const SWords<8>  fn  = sload<8>(m.fn64 + 4 * (size_t)t);
             double           fnd[4];
