@@ -43,7 +43,7 @@ def parse():
     ap.add_argument("--train-samples", type=int, default=4_000_000)
     ap.add_argument("--epochs", type=int, default=6)
     ap.add_argument("--branching", type=int, default=4)
-    ap.add_argument("--leaf", type=int, default=8)
+    ap.add_argument("--leaf", type=int, default=16)
     ap.add_argument("--no-scene", action="store_true", help="skip the CSG scene measurement")
     ap.add_argument("--plain-only", action="store_true",
                     help="profiling aid: time only the plain pipeline (no training, no learned bound)")

@@ -250,7 +250,7 @@ ebg_status ebg_batch_create(const float* const* xyz, const uint64_t* nv, const u
     }
     for (std::size_t k = 0; k < order.size(); k++)
       (void)hipMemcpy(qd.as<float>() + 3 * nq * k, qh[k].data(), 12 * nq, hipMemcpyHostToDevice);
-    const int cand[4][2] = {{4, 4}, {4, 8}, {8, 4}, {8, 8}};
+    const int cand[4][2] = {{4, 8}, {4, 16}, {8, 8}, {8, 16}};
     double    best       = 1e300;
     for (int c = 0; c < 4; c++) {
       std::vector<ebg_mesh*> ms;

@@ -122,7 +122,7 @@ void pack_triangles(const float* xyz, const uint32_t* tri, const std::vector<uin
   out.tripos.assign(12 * nt, 0.0f);
   out.feat.assign(8 * nt, 0u);
   out.fn64.assign(4 * nt, 0.0);
-  out.tsph.assign(4 * (nt + kSphPad), 0.0f);
+  out.tsph.assign(8 * (nt + kSphPad), 0.0f);
   for (std::size_t i = 0; i < nt; i++) {
     const uint32_t f  = order[i];
     const D3       x0 = ld3f(xyz + 3 * tri[3 * f]), x1 = ld3f(xyz + 3 * tri[3 * f + 1]),
@@ -154,8 +154,10 @@ void pack_triangles(const float* xyz, const uint32_t* tri, const std::vector<uin
     const double rad = std::max(length(x0 - cen), std::max(length(x1 - cen), length(x2 - cen)));
     put(r.sc, cen);
     r.sr = float(rad * (1.0 + 1e-5) + 1e-7 * (length(cen) + rad));
-    for (int c = 0; c < 3; c++) out.tsph[4 * i + c] = r.sc[c];
-    out.tsph[4 * i + 3] = r.sr;
+    for (int c = 0; c < 3; c++) out.tsph[8 * i + c] = r.sc[c];
+    out.tsph[8 * i + 3] = r.sr;
+    for (int c = 0; c < 3; c++) out.tsph[8 * i + 4 + c] = r.n[c];
+    out.tsph[8 * i + 7] = float(double(r.n[0]) * r.v0[0] + double(r.n[1]) * r.v0[1] + double(r.n[2]) * r.v0[2]);
     for (int v = 0; v < 3; v++)
       for (int c = 0; c < 3; c++) out.tripos[12 * i + 3 * v + c] = xyz[3 * tri[3 * f + v] + c];
     for (int v = 0; v < 3; v++) {

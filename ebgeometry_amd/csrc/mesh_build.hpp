@@ -35,7 +35,7 @@ struct PackedMesh
   std::vector<TriRec>   tris;   // leaf order
   std::vector<float>    tripos; // 12 per triangle, leaf order
   std::vector<uint32_t> feat;   // 8 per triangle, leaf order
-  std::vector<float>    tsph;   // 4 per triangle (bounding sphere), leaf order, + kSphPad zero entries
+  std::vector<float>    tsph;   // 8 per triangle (sphere, plane), leaf order, + kSphPad zero entries
   std::vector<float>    vn32, en32;
   std::vector<uint32_t> order;  // original face index of each leaf-order triangle
   std::vector<double>   vn64, en64, fn64;

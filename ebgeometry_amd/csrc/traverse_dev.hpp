@@ -430,22 +430,31 @@ __device__ __forceinline__ void traverse(const MeshDev& m, R px, R py, R pz, R b
     }
     else {
       // leaf: a triangle is evaluated by the whole wavefront (keeping the minimum is exact for
-      // every lane) unless its bounding sphere lies beyond every lane's bound. Inactive lanes
-      // have bound < 0, so rb is NaN and they never ask for a triangle. The spheres of up to 8
-      // triangles arrive in one scalar round trip (the array is padded past the last leaf),
-      // then each needed triangle record in one more.
+      // every lane) unless, for every lane, its bounding sphere or its plane lies beyond the
+      // lane's bound (both are lower bounds on the distance to the triangle). Inactive lanes
+      // have bound < 0, so rb is NaN and they never ask for a triangle. The culling data of 4
+      // triangles arrives in one scalar round trip (the array is padded past the last leaf),
+      // then each needed triangle record in one more, where the plane is tested again against
+      // the bound as tightened by the leaf's earlier triangles.
       const uint32_t first = uni(e.ref), cnt = uni(e.cnt);
       const R        rb    = sqrt(bound);
-      for (uint32_t c0 = 0; c0 < cnt; c0 += 8) {
-        const SWords<32> sp   = sload<32>(m.tsph + 4 * (size_t)(first + c0));
+      for (uint32_t c0 = 0; c0 < cnt; c0 += 4) {
+        const SWords<32> sp   = sload<32>(m.tsph + 8 * (size_t)(first + c0));
         uint32_t         need = 0;
 #pragma unroll
-        for (int i = 0; i < 8; i++) {
+        for (int i = 0; i < 4; i++) {
           if (c0 + i < cnt) {
-            const R    sx  = px - R(__uint_as_float(sp.w[4 * i])), sy = py - R(__uint_as_float(sp.w[4 * i + 1])),
-                       sz  = pz - R(__uint_as_float(sp.w[4 * i + 2]));
-            const R    lim = rb + R(__uint_as_float(sp.w[4 * i + 3]));
-            const bool tn  = sx * sx + sy * sy + sz * sz <= lim * lim;
+            const R    sx  = px - R(__uint_as_float(sp.w[8 * i])), sy = py - R(__uint_as_float(sp.w[8 * i + 1])),
+                       sz  = pz - R(__uint_as_float(sp.w[8 * i + 2]));
+            const R    lim = rb + R(__uint_as_float(sp.w[8 * i + 3]));
+            bool       tn  = sx * sx + sy * sy + sz * sz <= lim * lim;
+            if constexpr (sizeof(R) == 4) {
+              // the FP32 plane (rounded normal and offset) may misjudge by a few ulps of the
+              // coordinates; the FP64 pass, whose seed only just contains the minimum, skips it
+              const float pd = __uint_as_float(sp.w[8 * i + 4]) * px + __uint_as_float(sp.w[8 * i + 5]) * py +
+                               __uint_as_float(sp.w[8 * i + 6]) * pz - __uint_as_float(sp.w[8 * i + 7]);
+              tn = tn && pd * pd <= bound;
+            }
             if (COUNT) n_tris += tn ? 1u : 0u;
             need |= __ballot(tn) ? (1u << i) : 0u;
           }

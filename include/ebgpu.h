@@ -215,7 +215,7 @@ typedef struct ebg_batch_plan ebg_batch_plan;
 
 typedef struct ebg_batch_opts
 {
-  int32_t autotune;   /* 1: time (branching, leaf_size) in {4,8}x{4,8} on the largest meshes
+  int32_t autotune;   /* 1: time (branching, leaf_size) in {4,8}x{8,16} on the largest meshes
                          with synthetic queries and build every mesh with the fastest         */
   int32_t branching;  /* used when autotune = 0 (0 = 4)                                       */
   int32_t leaf_size;  /* used when autotune = 0 (0 = 8)                                       */
@@ -227,7 +227,7 @@ typedef struct ebg_batch_info
   uint64_t num_meshes, num_triangles;
   int32_t  branching, leaf_size, stack_capacity;
   int32_t  tuned;                /* 1 if the autotuner chose branching / leaf_size           */
-  double   tune_ms[4];           /* traversal ms of the candidates (4,4) (4,8) (8,4) (8,8)    */
+  double   tune_ms[4];           /* traversal ms of the candidates (4,8) (4,16) (8,8) (8,16)  */
   double   build_seconds;
 } ebg_batch_info;
 
