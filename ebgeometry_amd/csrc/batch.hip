@@ -94,7 +94,7 @@ __global__ __launch_bounds__(256) void batch32_kernel(const BatchArgs a)
     if (it.tri) it.tri[qi] = m.tris[btri].face;
   }
   if (a.recheck) {
-    const uint64_t mask = __ballot(found && fabsf(d) < it.recheck_abs);
+    const uint64_t mask = ballot(found && fabsf(d) < it.recheck_abs);
     if (mask && lane == 0) {
       const uint32_t slot = atomicAdd(a.recheck_count, 1u);
       a.recheck[slot]     = make_uint4(p, 0u, (uint32_t)mask, (uint32_t)(mask >> 32));

@@ -9,7 +9,7 @@
 //                                edge x normal, 1/|edge|^2). Also read with scalar loads.
 //   tripos  float[12*num_tris]   raw vertex positions (v0,v1,v2,pad) for the FP64 path.
 //   tsph    float8[num_tris+16]  per-triangle culling data, contiguous per leaf: bounding sphere
-//                                (centre, radius) and plane (unit normal, n.v0), so that the
+//                                (centre, radius) and plane (unit normal, -n.v0), so that the
 //                                culling data of 4 leaf triangles arrives in one scalar round trip.
 //   feat    uint32[8*num_tris]   vertex ids, half-edge ids, original face id: lets the sign
 //                                step fetch the one pseudo-normal a query needs.
@@ -71,7 +71,7 @@ struct MeshDev
   const void*     nodes;
   const TriRec*   tris;
   const float*    tripos; // 12 per triangle
-  const float*    tsph;   // 8 floats per triangle: sphere centre, radius, normal, n.v0 (leaf order)
+  const float*    tsph;   // 8 floats per triangle: sphere centre, radius, normal, -n.v0 (leaf order)
   const uint32_t* feat;   // 8 per triangle: vid0..2, he0..2, face, 0
   const float*    vn32;   // float4 per vertex
   const float*    en32;   // float4 per half-edge (3*face + i)

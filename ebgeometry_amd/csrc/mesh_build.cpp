@@ -157,7 +157,7 @@ void pack_triangles(const float* xyz, const uint32_t* tri, const std::vector<uin
     for (int c = 0; c < 3; c++) out.tsph[8 * i + c] = r.sc[c];
     out.tsph[8 * i + 3] = r.sr;
     for (int c = 0; c < 3; c++) out.tsph[8 * i + 4 + c] = r.n[c];
-    out.tsph[8 * i + 7] = float(double(r.n[0]) * r.v0[0] + double(r.n[1]) * r.v0[1] + double(r.n[2]) * r.v0[2]);
+    out.tsph[8 * i + 7] = float(-(double(r.n[0]) * r.v0[0] + double(r.n[1]) * r.v0[1] + double(r.n[2]) * r.v0[2]));
     for (int v = 0; v < 3; v++)
       for (int c = 0; c < 3; c++) out.tripos[12 * i + 3 * v + c] = xyz[3 * tri[3 * f + v] + c];
     for (int v = 0; v < 3; v++) {

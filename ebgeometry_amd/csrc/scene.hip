@@ -66,7 +66,7 @@ __device__ __forceinline__ void scene_traverse(const SceneDev& sc, R px, R py, R
     const Entry e    = st.get(--top);
     const R     d2e  = box_d2<R>(px, py, pz, e.lx, e.ly, e.lz, e.hx, e.hy, e.hz);
     const bool  need = d2e <= bound;
-    if (__ballot(need) == 0) continue;
+    if (ballot(need) == 0) continue;
 
     if (e.cnt == 0u) {
       if (COUNT) { c.n_inner += need ? 1u : 0u; c.w_inner++; }
@@ -77,7 +77,7 @@ __device__ __forceinline__ void scene_traverse(const SceneDev& sc, R px, R py, R
     for (uint32_t i = first; i < first + cnt; i++) {
       const PrimRec  pr = sload_record(sc.prims + i);
       const bool     pn = box_d2<R>(px, py, pz, pr.bv[0], pr.bv[1], pr.bv[2], pr.bv[3], pr.bv[4], pr.bv[5]) <= bound;
-      if (__ballot(pn) == 0) continue;
+      if (ballot(pn) == 0) continue;
       if (COUNT) { c.n_prims += pn ? 1u : 0u; c.w_prims++; }
       const uint32_t type = uni(pr.type);
       R              val;

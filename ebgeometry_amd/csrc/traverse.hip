@@ -81,7 +81,7 @@ __device__ __forceinline__ void sdf32_packet(const MeshDev& m, const QueryArgs& 
   bool found = active && btri != kInvalid && best <= seed2;
   if (MODE == kSeeded) {
     const bool     redo  = active && !found;
-    const uint64_t rmask = __ballot(redo);
+    const uint64_t rmask = ballot(redo);
     if (rmask) {
       float    best2 = INFINITY;
       uint32_t btri2 = kInvalid, bfeat2 = 0, f_inner = 0, f_tris = 0;

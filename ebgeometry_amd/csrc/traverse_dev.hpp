@@ -23,6 +23,9 @@ __device__ __forceinline__ uint32_t wrlane(uint32_t v, bool me, uint32_t old) { 
 __device__ __forceinline__ float    wrlanef(float v, bool me, float old) { return me ? v : old; }
 __device__ __forceinline__ uint32_t uni(uint32_t v) { return (uint32_t)__builtin_amdgcn_readfirstlane((int)v); }
 
+// Lane mask of a predicate, taken straight from the compare (no round trip through an int).
+__device__ __forceinline__ uint64_t ballot(bool p) { return __builtin_amdgcn_ballot_w64(p); }
+
 // Minimum over the 64 lanes of an unsigned value (all lanes active): DPP within rows of 16,
 // then the four row results combined in scalar registers.
 __device__ __forceinline__ uint32_t wave_min_u32(uint32_t v)
@@ -190,7 +193,11 @@ __device__ __forceinline__ R box_d2(R px, R py, R pz, float lx, float ly, float 
   }
 }
 
-// Squared distance from p to one triangle and the closest feature (FP32, precomputed record).
+// Squared distance from p to one triangle (FP32, precomputed record) and, with FEAT, its
+// closest feature. The traversal evaluates candidates with FEAT = false (the distance is all
+// it compares) and classifies the winner once at the end with FEAT = true; both compute d2 with
+// the same operations, so the classification belongs to the distance that won.
+template <bool FEAT>
 __device__ __forceinline__ void tri_eval32(const TriRec& r, float px, float py, float pz, float& d2, uint32_t& feat)
 {
   const float p1x = px - r.v0[0], p1y = py - r.v0[1], p1z = pz - r.v0[2];
@@ -214,16 +221,19 @@ __device__ __forceinline__ void tri_eval32(const TriRec& r, float px, float py, 
   const float y0x = p1x - t0 * r.e0[0], y0y = p1y - t0 * r.e0[1], y0z = p1z - t0 * r.e0[2];
   const float y1x = p2x - t1 * r.e1[0], y1y = p2y - t1 * r.e1[1], y1z = p2z - t1 * r.e1[2];
   const float y2x = p3x - t2 * r.e2[0], y2y = p3y - t2 * r.e2[1], y2z = p3z - t2 * r.e2[2];
-  float       r2  = y0x * y0x + y0y * y0y + y0z * y0z;
+  const float q0  = y0x * y0x + y0y * y0y + y0z * y0z;
   const float q1  = y1x * y1x + y1y * y1y + y1z * y1z;
   const float q2  = y2x * y2x + y2y * y2y + y2z * y2z;
-  uint32_t    e   = 0;
-  float       te  = t0;
-  if (q1 < r2) { r2 = q1; e = 1; te = t1; }
-  if (q2 < r2) { r2 = q2; e = 2; te = t2; }
-  const uint32_t f = te <= 0.0f ? kVert0 + e : (te >= 1.0f ? kVert0 + (e == 2 ? 0u : e + 1u) : kEdge0 + e);
-  d2   = inface ? fd * fd : r2;
-  feat = inface ? (uint32_t)kFace : f;
+  // minimum of the three (a NaN from a zero-length edge loses to a number)
+  const float r2  = fminf(fminf(q0, q1), q2);
+  d2              = inface ? fd * fd : r2;
+  if constexpr (FEAT) {
+    // the first edge attaining the minimum, and its clamped parameter
+    const uint32_t e  = q0 == r2 ? 0u : (q1 == r2 ? 1u : 2u);
+    const float    te = e == 0 ? t0 : (e == 1 ? t1 : t2);
+    const uint32_t f  = te <= 0.0f ? kVert0 + e : (te >= 1.0f ? kVert0 + (e == 2 ? 0u : e + 1u) : kEdge0 + e);
+    feat              = inface ? (uint32_t)kFace : f;
+  }
 }
 
 // Signed distance of the chosen feature (FP32): face -> n.(p-v0); vertex/edge ->
@@ -354,7 +364,7 @@ __device__ __forceinline__ double resolve64(const MeshDev& m, uint32_t t, uint32
 // Wave-aggregated append of `idx` for the lanes where pred holds.
 __device__ __forceinline__ void wave_append(bool pred, uint32_t idx, uint32_t* list, uint32_t* count)
 {
-  const uint64_t mask = __ballot(pred);
+  const uint64_t mask = ballot(pred);
   if (mask == 0) return;
   const int      lane   = __lane_id();
   const uint32_t n      = (uint32_t)__popcll(mask);
@@ -384,7 +394,7 @@ __device__ __forceinline__ void push_children(const Node<K>& nd, R px, R py, R p
     if (nd.cnt[k] != kInvalid) {
       const R    d2 = box_d2<R>(px, py, pz, nd.lox[k], nd.loy[k], nd.loz[k], nd.hix[k], nd.hiy[k], nd.hiz[k]);
       const bool nk = d2 <= bound;
-      msk[k]        = __ballot(nk);
+      msk[k]        = ballot(nk);
       if (msk[k]) key[k] = wave_min_u32(nk ? __float_as_uint(float(d2)) : 0xFFFFFFFFu);
     }
   }
@@ -422,7 +432,7 @@ __device__ __forceinline__ void traverse(const MeshDev& m, R px, R py, R pz, R b
     const Entry e = st.get(--top);
     const R     d2e  = box_d2<R>(px, py, pz, e.lx, e.ly, e.lz, e.hx, e.hy, e.hz);
     const bool  need = d2e <= bound;
-    if (__ballot(need) == 0) continue;
+    if (ballot(need) == 0) continue;
 
     if (e.cnt == 0u) {
       if (COUNT) { n_inner += need ? 1u : 0u; w_inner++; }
@@ -437,7 +447,9 @@ __device__ __forceinline__ void traverse(const MeshDev& m, R px, R py, R pz, R b
       // then each needed triangle record in one more, where the plane is tested again against
       // the bound as tightened by the leaf's earlier triangles.
       const uint32_t first = uni(e.ref), cnt = uni(e.cnt);
-      const R        rb    = sqrt(bound);
+      R              rb;
+      if constexpr (sizeof(R) == 4) rb = __builtin_amdgcn_sqrtf(bound) * 1.000001f; // 1-ulp hardware root, rounded up
+      else rb = sqrt(bound);
       for (uint32_t c0 = 0; c0 < cnt; c0 += 4) {
         const SWords<32> sp   = sload<32>(m.tsph + 8 * (size_t)(first + c0));
         uint32_t         need = 0;
@@ -451,26 +463,27 @@ __device__ __forceinline__ void traverse(const MeshDev& m, R px, R py, R pz, R b
             if constexpr (sizeof(R) == 4) {
               // the FP32 plane (rounded normal and offset) may misjudge by a few ulps of the
               // coordinates; the FP64 pass, whose seed only just contains the minimum, skips it
-              const float pd = __uint_as_float(sp.w[8 * i + 4]) * px + __uint_as_float(sp.w[8 * i + 5]) * py +
-                               __uint_as_float(sp.w[8 * i + 6]) * pz - __uint_as_float(sp.w[8 * i + 7]);
-              tn = tn && pd * pd <= bound;
+              const float pd = fmaf(__uint_as_float(sp.w[8 * i + 4]), px,
+                                    fmaf(__uint_as_float(sp.w[8 * i + 5]), py,
+                                         fmaf(__uint_as_float(sp.w[8 * i + 6]), pz, __uint_as_float(sp.w[8 * i + 7]))));
+              tn = tn && fabsf(pd) <= rb;
             }
             if (COUNT) n_tris += tn ? 1u : 0u;
-            need |= __ballot(tn) ? (1u << i) : 0u;
+            need |= ballot(tn) ? (1u << i) : 0u;
           }
         }
         while (need) {
           const uint32_t t = first + c0 + (uint32_t)__builtin_ctz(need);
           need &= need - 1u;
           R        d2;
-          uint32_t f;
+          uint32_t f = 0;
           if constexpr (sizeof(R) == 4) {
             const TriRec rec = sload_record(m.tris + t);
             // the distance to the triangle's plane is a lower bound on the distance to it
             const float  fd  = rec.n[0] * (px - rec.v0[0]) + rec.n[1] * (py - rec.v0[1]) + rec.n[2] * (pz - rec.v0[2]);
-            if (__ballot(fd * fd <= min(bound, best)) == 0) continue;
+            if (ballot(fd * fd <= bound) == 0) continue;
             if (COUNT) w_tris++;
-            tri_eval32(rec, px, py, pz, d2, f);
+            tri_eval32<false>(rec, px, py, pz, d2, f);
           }
           else {
             if (COUNT) w_tris++;
@@ -486,10 +499,17 @@ __device__ __forceinline__ void traverse(const MeshDev& m, R px, R py, R pz, R b
             best  = d2;
             btri  = t;
             bfeat = f;
+            bound = d2 < bound ? d2 : bound; // bound stays min(bound, best)
           }
         }
       }
-      bound = min(bound, best);
+    }
+  }
+  // FP32: classify the winning triangle's closest feature (per-lane record load)
+  if constexpr (sizeof(R) == 4) {
+    if (btri != kInvalid) {
+      float d2w;
+      tri_eval32<true>(m.tris[btri], px, py, pz, d2w, bfeat);
     }
   }
 }
