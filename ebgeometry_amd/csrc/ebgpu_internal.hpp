@@ -48,6 +48,15 @@ struct alignas(64) Node
 static_assert(sizeof(Node<4>) == 128, "Node<4> layout");
 static_assert(sizeof(Node<8>) == 256, "Node<8> layout");
 
+// A subtree root (node reference and box, as on the traversal stack): the roots of a frontier
+// that covers the whole tree let several wavefronts share one expensive packet.
+struct SplitRoot
+{
+  uint32_t ref, cnt; // cnt == 0: interior node ref; 1..16: leaf of cnt triangles from ref
+  float    lo[3], hi[3];
+};
+static_assert(sizeof(SplitRoot) == 32, "SplitRoot layout");
+
 struct alignas(64) TriRec
 {
   float    v0[3];  // first vertex

@@ -76,7 +76,7 @@ hipError_t upload(DevBuf& b, const std::vector<T>& v)
 // Query workspace, grown on demand.
 struct Workspace
 {
-  DevBuf   codes, codes_sorted, perm, sort_scratch, box, lists, seeds;
+  DevBuf   codes, codes_sorted, perm, sort_scratch, box, lists, seeds, sched;
   uint64_t cap = 0;
 
   hipError_t ensure(uint64_t n)
@@ -92,12 +92,24 @@ struct Workspace
     // recheck list, fallback list, 2 counters (+pad)
     if ((e = lists.reserve(8 * m + 256)) != hipSuccess) return e;
     if ((e = seeds.reserve(4 * m)) != hipSuccess) return e;
+    // split packets: one flag byte per packet, the list of split packets, 64 result keys each
+    if ((e = sched.reserve(sched_keys_offset(m) + 512 * (size_t)heavy_cap(m))) != hipSuccess) return e;
     cap = m;
     return hipSuccess;
   }
+  static uint64_t packets(uint64_t n) { return (n + 63) / 64; }
+  static uint32_t heavy_cap(uint64_t n) { return (uint32_t)(packets(n) / 64 + 64); }
+  static size_t   sched_list_offset(uint64_t n) { return (packets(n) + 255) & ~size_t(255); }
+  static size_t   sched_keys_offset(uint64_t n) { return sched_list_offset(n) + ((4 * (size_t)heavy_cap(n) + 255) & ~size_t(255)); }
+  uint8_t*        heavy_flag() const { return sched.as<uint8_t>(); }
+  uint32_t*       heavy_list() const { return reinterpret_cast<uint32_t*>(sched.as<uint8_t>() + sched_list_offset(cap)); }
+  unsigned long long* heavy_keys() const
+  {
+    return reinterpret_cast<unsigned long long*>(sched.as<uint8_t>() + sched_keys_offset(cap));
+  }
   uint32_t* recheck_list() const { return lists.as<uint32_t>(); }
   uint32_t* fallback_list() const { return lists.as<uint32_t>() + cap; }
-  uint32_t* counters() const { return lists.as<uint32_t>() + 2 * cap; }
+  uint32_t* counters() const { return lists.as<uint32_t>() + 2 * cap; } // [0] recheck [1] fallback [2] heavy
 
   // Morton presort of n device points: returns the permutation (valid until the next call).
   hipError_t presort(const float* d_pts, uint64_t n, hipStream_t s)
@@ -122,6 +134,16 @@ struct ebg_mesh
   double                 build_seconds = 0;
   ebgpu::DevBuf          nodes, tris, tripos, tsph, feat, vn32, en32, vn64, en64, fn64;
   ebgpu::DevBuf          area_cdf; // normalised cumulative triangle area, leaf order (training samples)
+  // cost grid of the split schedule (built at the first large presorted query): per cell, the
+  // work of a packet of 64 queries spread over the cell
+  ebgpu::DevBuf          cost_grid;
+  ebgpu::DevBuf          split_roots; // SplitRoot frontier covering the tree
+  uint32_t               num_split_roots = 0;
+  bool                   grid_ready = false;
+  float                  grid_lo[3]{}, grid_scale[3]{};
+  double                 grid_mean = 0; // mean packet cost over the grid cells
+  bool                   last_split     = false; // the last query listed split packets ...
+  int64_t                last_split_cap = 0;     // ... up to this many
   std::vector<float>     host_xyz; // vertex positions (scene instance bounds)
   std::vector<uint32_t>  order;    // original face index of each leaf-order triangle
   ebgpu::MeshDev         dev{};

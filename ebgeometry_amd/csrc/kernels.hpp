@@ -25,7 +25,41 @@ struct QueryArgs
   unsigned long long* work;      // [inner nodes, triangles] accumulated (null = not counted)
   float               slack_abs; // FP64 recheck: bound = (|d| + slack_abs + slack_rel*|d|)^2
   float               slack_rel;
+  uint32_t*           packet_cost; // counting launches: per packet, the nodes + triangles its wavefront fetched
+  // Split packets (split_blocks = 0: none). The first split_blocks workgroups of the FP32 launch
+  // stride over the items (j, r), j < min(*split_count, split_cap), r < split_nroots: walk subtree
+  // split_roots[r] for the 64 queries of packet split_list[j] (unseeded) and fold each lane's
+  // (d^2, triangle) into split_keys[64 j + lane] with an atomic minimum. The other workgroups
+  // skip the packets flagged in heavy_flag; a resolve launch finishes the split packets.
+  const uint8_t*      heavy_flag;
+  const SplitRoot*    split_roots;
+  const uint32_t*     split_list;
+  const uint32_t*     split_count;
+  unsigned long long* split_keys;
+  uint32_t            split_nroots, split_cap, split_blocks;
 };
+
+// Expensive packets: a packet whose most expensive query (cost-grid value of its cell, grid of
+// g^3 cells over [lo, lo + g/scale)) reaches threshold is listed (up to cap), flagged, and its
+// 64 result keys are reset. count must be zero on entry.
+struct HeavyMarkArgs
+{
+  const float*        pts;
+  const uint32_t*     idx;
+  uint64_t            n;
+  const uint32_t*     grid;
+  float               lo[3], scale[3];
+  uint32_t            g, threshold, cap;
+  uint32_t*           list;
+  uint32_t*           count;
+  uint8_t*            flag;
+  unsigned long long* keys; // cap * 64
+};
+hipError_t launch_heavy_mark(const HeavyMarkArgs& h, hipStream_t s);
+
+// After the FP32 launch: the split packets' winning triangles classified, signed and written like
+// the plain pass (outputs, recheck list).
+hipError_t launch_sdf32_resolve(const MeshDev& m, const QueryArgs& a, hipStream_t s);
 
 // FP32 packet traversal. K = 4 or 8; banks = stack capacity / 64 (1..4).
 hipError_t launch_sdf32(const MeshDev& m, int K, int banks, const QueryArgs& a, bool count, hipStream_t s);

@@ -111,6 +111,7 @@ __device__ __forceinline__ void sdf32_packet(const MeshDev& m, const QueryArgs& 
     if (a.out_tri) a.out_tri[qi] = m.tris[btri].face;
   }
   if (a.recheck_list) wave_append(found && fabsf(d) < a.recheck_abs, qi, a.recheck_list, a.recheck_count);
+  if (COUNT && a.packet_cost && __lane_id() == 0) a.packet_cost[pk] = w_inner + w_tris;
   if (COUNT && a.work) {
     const uint64_t si = wave_sum_u64(active ? (uint64_t)n_inner : 0ull);
     const uint64_t st = wave_sum_u64(active ? (uint64_t)n_tris : 0ull);
@@ -123,15 +124,99 @@ __device__ __forceinline__ void sdf32_packet(const MeshDev& m, const QueryArgs& 
   }
 }
 
+// One split item: subtree root r for the 64 queries of listed packet j, unseeded; each lane's
+// (d^2, triangle) is folded into the packet's keys. d^2 >= 0, so its bits order like the value;
+// on equal d^2 the lower triangle index wins.
+template <int K, int NB, bool COUNT>
+__device__ __forceinline__ void split_item(const MeshDev& m, const QueryArgs& a, uint32_t j, uint32_t r)
+{
+  const uint64_t pk     = a.split_list[j];
+  const uint32_t lane   = __lane_id();
+  const uint64_t tid    = pk * 64u + lane;
+  const bool     active = tid < a.n;
+  float          px = 0.0f, py = 0.0f, pz = 0.0f;
+  if (active) {
+    const uint32_t qi = a.idx ? a.idx[tid] : (uint32_t)tid;
+    px                = a.pts[3 * (size_t)qi];
+    py                = a.pts[3 * (size_t)qi + 1];
+    pz                = a.pts[3 * (size_t)qi + 2];
+  }
+  const SplitRoot rt = sload_record(a.split_roots + r);
+  const Entry     start{rt.ref, rt.cnt, rt.lo[0], rt.lo[1], rt.lo[2], rt.hi[0], rt.hi[1], rt.hi[2]};
+  float           best = INFINITY;
+  uint32_t        btri = kInvalid, bfeat = 0, n_inner = 0, n_tris = 0, w_inner = 0, w_tris = 0;
+  traverse_from<K, NB, float, COUNT>(m, start, px, py, pz, active ? INFINITY : -1.0f, best, btri, bfeat, n_inner,
+                                     n_tris, w_inner, w_tris);
+  if (active && btri != kInvalid)
+    atomicMin(a.split_keys + 64 * (size_t)j + lane, ((unsigned long long)__float_as_uint(best) << 32) | btri);
+  if (COUNT && a.work) {
+    const uint64_t si = wave_sum_u64(active ? (uint64_t)n_inner : 0ull);
+    const uint64_t st = wave_sum_u64(active ? (uint64_t)n_tris : 0ull);
+    if (lane == 0) {
+      atomicAdd(a.work, (unsigned long long)si);
+      atomicAdd(a.work + 1, (unsigned long long)st);
+      atomicAdd(a.work + 4, (unsigned long long)w_inner);
+      atomicAdd(a.work + 5, (unsigned long long)w_tris);
+    }
+  }
+}
+
 // Grid launch: one wavefront per 64-query packet, 4 packets per workgroup; workgroups are
 // mapped XCD-aware so that each XCD works through one contiguous (Morton) range of queries.
+// The leading split_blocks workgroups (dispatched first) work through the split items instead,
+// beside the packets of the bulk; packets flagged in heavy_flag are skipped.
 template <int K, int NB, bool COUNT, int MODE>
 __global__ __launch_bounds__(256) void sdf32_kernel(const MeshDev m, const QueryArgs a)
 {
-  const uint32_t blk = xcd_remap(blockIdx.x, gridDim.x);
+  const uint32_t wave = threadIdx.x >> 6;
+  uint32_t       b = blockIdx.x, nb = gridDim.x;
+  if (a.split_blocks) {
+    if (b < a.split_blocks) {
+      const uint32_t items = min(*a.split_count, a.split_cap) * a.split_nroots;
+      for (uint32_t w = b * 4u + wave; w < items; w += a.split_blocks * 4u) {
+        const uint32_t j = w / a.split_nroots;
+        split_item<K, NB, COUNT>(m, a, j, w - j * a.split_nroots);
+      }
+      return;
+    }
+    b -= a.split_blocks;
+    nb -= a.split_blocks;
+  }
+  const uint32_t blk = xcd_remap(b, nb);
   const uint64_t n   = a.n;
-  if ((uint64_t)blk * 256u >= n) return; // whole workgroup past the end (uniform per block)
-  sdf32_packet<K, NB, COUNT, MODE>(m, a, n, (uint64_t)blk * 4u + (threadIdx.x >> 6));
+  const uint64_t pk  = (uint64_t)blk * 4u + wave;
+  if (pk * 64u >= n) return;                    // whole wavefront past the end
+  if (a.heavy_flag && a.heavy_flag[pk]) return; // whole wavefront: a split packet
+  sdf32_packet<K, NB, COUNT, MODE>(m, a, n, pk);
+}
+
+// One wavefront per listed packet: the winning triangle of each query, as the plain pass ends.
+__global__ __launch_bounds__(256) void sdf32_resolve_kernel(const MeshDev m, const QueryArgs a)
+{
+  const uint32_t j = blockIdx.x * 4u + (threadIdx.x >> 6);
+  if (j >= min(*a.split_count, a.split_cap)) return; // whole wavefront
+  const uint64_t pk     = a.split_list[j];
+  const uint64_t tid    = pk * 64u + __lane_id();
+  const bool     active = tid < a.n;
+  bool           found  = false;
+  uint32_t       qi     = 0;
+  float          d      = 0.0f;
+  if (active) {
+    qi                           = a.idx ? a.idx[tid] : (uint32_t)tid;
+    const unsigned long long key = a.split_keys[64 * (size_t)j + __lane_id()];
+    const uint32_t           t   = (uint32_t)key;
+    if (t != kInvalid) {
+      const float px = a.pts[3 * (size_t)qi], py = a.pts[3 * (size_t)qi + 1], pz = a.pts[3 * (size_t)qi + 2];
+      float       d2;
+      uint32_t    feat;
+      tri_eval32<true>(m.tris[t], px, py, pz, d2, feat);
+      d           = resolve32(m, t, feat, __uint_as_float((uint32_t)(key >> 32)), px, py, pz);
+      a.out_d[qi] = d;
+      if (a.out_tri) a.out_tri[qi] = m.tris[t].face;
+      found = true;
+    }
+  }
+  if (a.recheck_list) wave_append(found && fabsf(d) < a.recheck_abs, qi, a.recheck_list, a.recheck_count);
 }
 
 template <int K, int NB>
@@ -184,9 +269,10 @@ void launch32m(const MeshDev& m, const QueryArgs& a, bool count, uint32_t blocks
 template <int K, int NB>
 hipError_t launch32(const MeshDev& m, const QueryArgs& a, bool count, hipStream_t s)
 {
-  const uint32_t blocks = (uint32_t)((a.n + 255) / 256);
+  const uint32_t blocks = (uint32_t)((a.n + 255) / 256) + a.split_blocks;
   if (blocks == 0) return hipSuccess;
   if (a.count_ptr) return hipErrorInvalidValue;
+  if (a.split_blocks && (!a.split_nroots || !a.split_count || !a.heavy_flag)) return hipErrorInvalidValue;
   if (a.seed_r) {
     if (!a.fallback_count) return hipErrorInvalidValue;
     launch32m<K, NB, kSeeded>(m, a, count, blocks, s);
@@ -218,6 +304,56 @@ hipError_t launch_sdf32(const MeshDev& m, int K, int banks, const QueryArgs& a, 
   if (banks <= 1) return launch32<8, 1>(m, a, count, s);
   if (banks == 2) return launch32<8, 2>(m, a, count, s);
   return launch32<8, 4>(m, a, count, s);
+}
+
+namespace {
+
+// One wavefront per packet (the 64 presorted queries pk*64 ..): the packet's cost estimate is the
+// largest cost-grid value over its queries' cells.
+__global__ __launch_bounds__(256) void heavy_mark_kernel(const HeavyMarkArgs h)
+{
+  const uint64_t pk     = (uint64_t)blockIdx.x * 4u + (threadIdx.x >> 6);
+  const uint64_t tid    = pk * 64u + (uint64_t)__lane_id();
+  if (pk * 64u >= h.n) return; // whole wavefront
+  uint32_t c = 0;
+  if (tid < h.n) {
+    const uint32_t qi = h.idx ? h.idx[tid] : (uint32_t)tid;
+    uint32_t       cell[3];
+#pragma unroll
+    for (int k = 0; k < 3; k++) {
+      const float x = (h.pts[3 * (size_t)qi + k] - h.lo[k]) * h.scale[k];
+      cell[k]       = (uint32_t)fminf(fmaxf(x, 0.0f), float(h.g - 1)); // NaN -> 0
+    }
+    c = h.grid[((size_t)cell[2] * h.g + cell[1]) * h.g + cell[0]];
+  }
+  const uint32_t est  = wave_max_u32(c);
+  uint32_t       slot = kInvalid;
+  if (est >= h.threshold && __lane_id() == 0) slot = atomicAdd(h.count, 1u);
+  slot = rdlane(slot, 0);
+  const bool heavy = slot < h.cap;
+  if (heavy) h.keys[64 * (size_t)slot + __lane_id()] = ~0ull;
+  if (__lane_id() == 0) {
+    if (heavy) h.list[slot] = (uint32_t)pk;
+    h.flag[pk] = heavy ? 1 : 0;
+  }
+}
+
+} // namespace
+
+hipError_t launch_heavy_mark(const HeavyMarkArgs& h, hipStream_t s)
+{
+  const uint64_t packets = (h.n + 63) / 64;
+  if (packets == 0) return hipSuccess;
+  hipLaunchKernelGGL(heavy_mark_kernel, dim3((uint32_t)((packets + 3) / 4)), dim3(256), 0, s, h);
+  return hipGetLastError();
+}
+
+hipError_t launch_sdf32_resolve(const MeshDev& m, const QueryArgs& a, hipStream_t s)
+{
+  const uint32_t blocks = (a.split_cap + 3) / 4;
+  if (blocks == 0) return hipSuccess;
+  hipLaunchKernelGGL(sdf32_resolve_kernel, dim3(blocks), dim3(256), 0, s, m, a);
+  return hipGetLastError();
 }
 
 hipError_t launch_sdf64(const MeshDev& m, int K, int banks, const QueryArgs& a, hipStream_t s)

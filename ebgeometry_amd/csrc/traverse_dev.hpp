@@ -38,6 +38,16 @@ __device__ __forceinline__ uint32_t wave_min_u32(uint32_t v)
   return min(min(a, b), min(c, d));
 }
 
+__device__ __forceinline__ uint32_t wave_max_u32(uint32_t v)
+{
+  v = max(v, (uint32_t)__builtin_amdgcn_mov_dpp((int)v, 0xB1, 0xF, 0xF, false));
+  v = max(v, (uint32_t)__builtin_amdgcn_mov_dpp((int)v, 0x4E, 0xF, 0xF, false));
+  v = max(v, (uint32_t)__builtin_amdgcn_mov_dpp((int)v, 0x141, 0xF, 0xF, false));
+  v = max(v, (uint32_t)__builtin_amdgcn_mov_dpp((int)v, 0x140, 0xF, 0xF, false));
+  const uint32_t a = rdlane(v, 0), b = rdlane(v, 16), c = rdlane(v, 32), d = rdlane(v, 48);
+  return max(max(a, b), max(c, d));
+}
+
 __device__ __forceinline__ uint64_t wave_sum_u64(uint64_t v)
 {
   // few calls per wave: plain butterfly through ds_swizzle-free shuffles
@@ -416,17 +426,18 @@ __device__ __forceinline__ void push_children(const Node<K>& nd, R px, R py, R p
 }
 
 // R = float: FP32 pass on TriRec records. R = double: FP64 recheck on raw positions.
+// The walk starts at `start` (the root, or a subtree root of a split packet).
 template <int K, int NB, class R, bool COUNT>
-__device__ __forceinline__ void traverse(const MeshDev& m, R px, R py, R pz, R bound, R& best, uint32_t& btri,
-                                         uint32_t& bfeat, uint32_t& n_inner, uint32_t& n_tris, uint32_t& w_inner, uint32_t& w_tris)
+__device__ __forceinline__ void traverse_from(const MeshDev& m, const Entry& start, R px, R py, R pz, R bound, R& best,
+                                              uint32_t& btri, uint32_t& bfeat, uint32_t& n_inner, uint32_t& n_tris,
+                                              uint32_t& w_inner, uint32_t& w_tris)
 {
   const Node<K>* __restrict__ nodes = static_cast<const Node<K>*>(m.nodes);
   LaneStack<NB>               st;
   st.init();
   const int lane = __lane_id();
   int       top  = 0;
-  Entry root{0u, 0u, m.root_lo[0], m.root_lo[1], m.root_lo[2], m.root_hi[0], m.root_hi[1], m.root_hi[2]};
-  st.put(top++, root, lane);
+  st.put(top++, start, lane);
 
   while (top > 0) {
     const Entry e = st.get(--top);
@@ -512,6 +523,14 @@ __device__ __forceinline__ void traverse(const MeshDev& m, R px, R py, R pz, R b
       tri_eval32<true>(m.tris[btri], px, py, pz, d2w, bfeat);
     }
   }
+}
+
+template <int K, int NB, class R, bool COUNT>
+__device__ __forceinline__ void traverse(const MeshDev& m, R px, R py, R pz, R bound, R& best, uint32_t& btri,
+                                         uint32_t& bfeat, uint32_t& n_inner, uint32_t& n_tris, uint32_t& w_inner, uint32_t& w_tris)
+{
+  const Entry root{0u, 0u, m.root_lo[0], m.root_lo[1], m.root_lo[2], m.root_hi[0], m.root_hi[1], m.root_hi[2]};
+  traverse_from<K, NB, R, COUNT>(m, root, px, py, pz, bound, best, btri, bfeat, n_inner, n_tris, w_inner, w_tris);
 }
 
 } // namespace dev

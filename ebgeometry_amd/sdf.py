@@ -167,6 +167,10 @@ class TriMeshSDF(SignedDistanceFunction):
     def last_traversal_ms(self) -> float:
         return float(lib().ebg_mesh_last_traversal_ms(self._h))
 
+    def last_split_packets(self) -> int:
+        """Packets of the last query that were shared among subtree wavefronts (waits)."""
+        return int(lib().ebg_mesh_last_split_packets(self._h))
+
     # -- learned bound --------------------------------------------------------------------
 
     def train_bound(self, num_samples: int = 4_000_000, epochs: int = 8, batch: int = 4096, lr: float = 2e-3,

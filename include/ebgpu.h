@@ -138,7 +138,11 @@ typedef struct ebg_query_opts
 void ebg_query_opts_default(ebg_query_opts* opts);
 
 /* d_pts: n float32 xyz triples; d_dist: n float32; d_tri (nullable): n uint32 face indices
- * (closest face, up to exact ties). Device pointers, asynchronous on opts->stream. */
+ * (closest face, up to exact ties). Device pointers, asynchronous on opts->stream.
+ * Split packets: with presort on and n >= 2^14, packets of 64 queries whose estimated cost (a
+ * 32^3 grid of measured packet costs over the mesh box, built at the first such call) is a few
+ * times the grid mean are each shared among 16 wavefronts walking disjoint subtrees, unseeded,
+ * inside the same traversal launch; results are those of the plain traversal. */
 ebg_status ebg_query_sdf(const ebg_mesh* mesh, const float* d_pts, uint64_t n, float* d_dist,
                          uint32_t* d_tri, const ebg_query_opts* opts);
 /* Host-pointer convenience form; synchronous. */
@@ -146,9 +150,13 @@ ebg_status ebg_query_sdf_host(const ebg_mesh* mesh, const float* pts, uint64_t n
                               uint32_t* tri, const ebg_query_opts* opts);
 
 /* Time the dominant traversal kernel of the last query on `mesh` (HIP events recorded on the
- * launch stream around the main FP32 traversal launch -- plain or seeded; the fallback re-run
- * and the FP64 pass are not included). Milliseconds, or <0 if none. Waits for that launch. */
+ * launch stream around the FP32 traversal -- the launch, plain or seeded, with its split
+ * packets, and their resolve; the FP64 pass is not included). Milliseconds, or <0 if none.
+ * Waits for that launch. */
 double ebg_mesh_last_traversal_ms(const ebg_mesh* mesh);
+/* Number of split packets of the last query on `mesh` (0 if none were split, <0 on error).
+ * Diagnostic with no counterpart in the reference; waits for the device. */
+int64_t ebg_mesh_last_split_packets(const ebg_mesh* mesh);
 
 /* ---------------------------------------------------------------------------------------- */
 /* CSG union scenes -- BVHUnionIF(primitives, bounding volumes) and its value()               */

@@ -120,16 +120,19 @@ def test_training_learns_and_bound_is_exact(cuda):
     assert fallbacks < 0.10 * q.shape[0], fallbacks
 
 
-def test_poisoned_bound_falls_back_everywhere(cuda):
+@pytest.mark.parametrize("n", [16_000, 60_000])
+def test_poisoned_bound_falls_back_everywhere(cuda, n):
     """A network predicting 0 gives radius 1e-3*diag: every query beyond it must fall back and
-    still return the exact answer."""
+    still return the exact answer. From 2^14 queries on, expensive packets are split over
+    subtrees and run unseeded (exact without a fallback), so the fallback count is checked
+    below that size only."""
     import torch
     v, f = wl.displaced_icosphere(4, 31)
     m = TriMeshSDF(v, f)
     theta = np.zeros(int(lib().ebg_mlp_param_count()), np.float32)
     theta[-1] = -20.0  # y = -20: prediction exp(-20) - 0.005 < 0, clamped to a zero radius
     m.set_bound_params(theta)
-    q = wl.uniform_box(20_000, [-1.5] * 3, [1.5] * 3, 32)
+    q = wl.uniform_box(n, [-1.5] * 3, [1.5] * 3, 32)
     qt = torch.from_numpy(q).to(cuda)
     d0 = m.query(qt)
     w = torch.zeros(8, dtype=torch.int64, device=cuda)
@@ -137,7 +140,8 @@ def test_poisoned_bound_falls_back_everywhere(cuda):
     torch.cuda.synchronize()
     diag = float(np.linalg.norm(v.max(0).astype(np.float64) - v.min(0)))
     far = (torch.abs(d0) > 2e-3 * diag).sum().item()
-    assert w[3].item() >= far
+    if n < 1 << 14:
+        assert w[3].item() >= far
     assert torch.equal(torch.sign(d0), torch.sign(d1))
     assert (d0 - d1).abs().max().item() <= 1e-6 * diag
 

@@ -1,0 +1,85 @@
+"""GPU parity of split packets: expensive packets (queries deep inside a closed surface, where
+many triangles lie at almost the same distance) are shared among wavefronts that walk disjoint
+subtrees; their results must be those of the plain traversal.
+
+Bar: against the same queries in chunks below the split size (2^14, so nothing is split), equal
+sign everywhere and |d - d_chunked| <= 1e-6 * diag (the winning triangle may differ only on exact
+ties); against the oracle's FP64 SDF on a strided subset, |d - d_ref| <= 1e-5 * diag and equal
+sign; closest faces equal up to ties (where they differ, the face's own FP64 distance is within
+2e-6 * diag of the minimum).
+"""
+import numpy as np
+import pytest
+
+from ebgeometry_amd import TriMeshSDF
+from ebgeometry_amd import workloads as wl
+from oracle import port
+
+pytestmark = pytest.mark.gpu
+
+
+def deep_queries(n, seed):
+    """Half inside a ball of radius 0.3 about the centre of the unit-ish displaced sphere (the
+    expensive packets), half the usual uniform / near-surface mix."""
+    rng = np.random.default_rng(seed)
+    d = rng.normal(size=(n // 2, 3))
+    d /= np.linalg.norm(d, axis=1, keepdims=True)
+    r = 0.3 * rng.random(n // 2) ** (1.0 / 3.0)
+    return (d * r[:, None]).astype(np.float32)
+
+
+@pytest.mark.parametrize("branching,leaf", [(4, 16), (8, 8), (4, 4)])
+def test_split_packets_match_plain(cuda, branching, leaf):
+    import torch
+    v, f = wl.displaced_icosphere(5, 17)
+    m = TriMeshSDF(v, f, branching=branching, leaf_size=leaf)
+    n = 120_000
+    q = np.concatenate([deep_queries(n, 3), wl.mixed_queries(n - n // 2, v, f, 4)])
+    qt = torch.from_numpy(q).to(cuda)
+    d, tri = m.query(qt, with_triangle=True)
+    split = m.last_split_packets()
+    assert split > 0, "the deep queries should have produced split packets"
+
+    chunks = [m.query(qt[i:i + 16_000], with_triangle=True) for i in range(0, n, 16_000)]
+    assert m.last_split_packets() == 0
+    d_ref = torch.cat([c[0] for c in chunks])
+    tri_ref = torch.cat([c[1] for c in chunks])
+    diag = float(np.linalg.norm(v.max(0).astype(np.float64) - v.min(0)))
+    assert torch.equal(torch.sign(d), torch.sign(d_ref))
+    assert (d - d_ref).abs().max().item() <= 1e-6 * diag
+    om = port.Mesh(v.astype(np.float64), f)
+    # a different face only on a tie (vertex and edge features shared by several faces, whose
+    # FP32 distances can round equal): its FP64 distance must be the minimum as well
+    diff = (tri != tri_ref).nonzero().flatten().cpu().numpy()
+    if diff.size:
+        fn, vn, en = om.normals()
+        qq = q[diff].astype(np.float64)
+        dmin, _ = om.sdf(qq)
+        t = tri.cpu().numpy()[diff]
+        for i in range(diff.size):
+            x = v.astype(np.float64)[f.astype(np.int64)[t[i]]].ravel()
+            a = abs(port.triangle_sdf(x, fn[t[i]], vn[t[i]], en[t[i]], qq[i]))
+            assert a - abs(dmin[i]) <= 2e-6 * diag, (diff[i], a, dmin[i])
+
+    sub = np.arange(0, n, 60)
+    ref, _ = om.sdf(q[sub].astype(np.float64))
+    dd = d.cpu().numpy()[sub]
+    assert np.abs(dd - ref).max() <= 1e-5 * diag
+    assert np.array_equal(np.sign(dd), np.sign(ref))
+
+
+def test_split_packets_with_learned_bound(cuda):
+    """A seeded launch leaves split packets unseeded: results equal the plain launch's."""
+    import torch
+    v, f = wl.displaced_icosphere(4, 19)
+    m = TriMeshSDF(v, f)
+    m.train_bound(num_samples=200_000, epochs=2, seed=5)
+    n = 80_000
+    q = np.concatenate([deep_queries(n, 7), wl.mixed_queries(n - n // 2, v, f, 8)])
+    qt = torch.from_numpy(q).to(cuda)
+    d0 = m.query(qt)
+    d1 = m.query(qt, learned_bound=True)
+    assert m.last_split_packets() > 0
+    diag = float(np.linalg.norm(v.max(0).astype(np.float64) - v.min(0)))
+    assert torch.equal(torch.sign(d0), torch.sign(d1))
+    assert (d0 - d1).abs().max().item() <= 1e-6 * diag
