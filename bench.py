@@ -322,7 +322,12 @@ def main():
                 "per_query": {"interior_nodes": round(w[0] / n, 2), "triangles": round(w[1] / n, 2),
                               "fp64_rechecked": w[2], "fallbacks": w[3],
                               "fallback_interior_nodes": round(w[6] / n, 3),
-                              "fallback_triangles": round(w[7] / n, 3)}}
+                              "fallback_triangles": round(w[7] / n, 3)},
+                # what the wavefronts actually fetched (one scalar load serves 64 queries): the
+                # per-query algorithmic bytes above exceed it, so frac can pass 1
+                "per_wavefront": {"interior_nodes": round(w[4] / (n / 64), 1),
+                                  "triangles": round(w[5] / (n / 64), 1),
+                                  "fetched_bytes": int(16 * n + w[4] * 32 * K + w[5] * 128)}}
 
     rl = roof(wl_, trav_learn, kname.format(mode=1), "seeded")
     rp = roof(wp, trav_plain, kname.format(mode=0), "plain")

@@ -338,7 +338,8 @@ ebg_status ebg_query_sdf(const ebg_mesh* cm, const float* d_pts, uint64_t n, flo
     EBG_HIP(hipEventCreate(&m->ev0));
     EBG_HIP(hipEventCreate(&m->ev1));
   }
-  const bool heavy_first = o.presort && n >= kHeavyMinQ;
+  // counting launches run unsplit: the counters describe the plain traversal's own work
+  const bool heavy_first = o.presort && n >= kHeavyMinQ && !o.work;
   m->last_split           = heavy_first;
   m->last_split_cap       = heavy_first ? ebgpu::Workspace::heavy_cap(n) : 0;
   if (heavy_first) {

@@ -35,8 +35,9 @@ __device__ __forceinline__ uint64_t spread3b(uint64_t a)
   return b;
 }
 
-// Sort key of flat query g: (item << 42) | top 42 bits of its 63-bit Morton code in the item's
-// box (bins clamped to 21 bits). Item found by binary search over the item bases.
+// Sort key of flat query g: (item << 24) | top 24 bits of its 63-bit Morton code in the item's
+// box (bins clamped to 21 bits; 8 bits per axis order 64-query packets well at any item size).
+// Item found by binary search over the item bases.
 __global__ __launch_bounds__(256) void batch_keys_kernel(const BatchItemDev* __restrict__ items, uint32_t num_items,
                                                          uint64_t total, uint64_t* __restrict__ keys)
 {
@@ -57,7 +58,7 @@ __global__ __launch_bounds__(256) void batch_keys_kernel(const BatchItemDev* __r
     b       = fminf(fmaxf(b, 0.0f), 2097151.0f); // NaN -> 0
     code |= spread3b((uint64_t)b) << c;
   }
-  keys[g] = ((uint64_t)lo << 42) | (code >> 21);
+  keys[g] = ((uint64_t)lo << 24) | (code >> 39);
 }
 
 template <int K, int NB, bool COUNT>

@@ -189,7 +189,7 @@ ebg_status enqueue(ebg_batch_plan* p, hipStream_t s, bool events)
     int item_bits = 1;
     while ((1ull << item_bits) < p->num_items) item_bits++;
     EBG_HIP(ebgpu::radix_sort_pairs(p->keys.as<uint64_t>(), nullptr, p->keys_sorted.as<uint64_t>(),
-                                    p->sorted.as<uint32_t>(), p->total, 42 + item_bits, p->scratch.p, s));
+                                    p->sorted.as<uint32_t>(), p->total, 24 + item_bits, p->scratch.p, s));
     a.sorted = p->sorted.as<uint32_t>();
   }
   if (events) EBG_HIP(hipEventRecord(p->ev0, s));

@@ -116,9 +116,10 @@ struct Workspace
   {
     hipError_t e;
     if ((e = launch_bbox_f32(d_pts, n, box.as<float>(), box.as<float>() + 8, s)) != hipSuccess) return e;
-    if ((e = launch_codes_f32(d_pts, n, box.as<float>(), codes.as<uint64_t>(), s)) != hipSuccess) return e;
+    // top 32 of the 63 code bits (about 10.7 per axis): 4 radix passes instead of 8
+    if ((e = launch_codes_f32(d_pts, n, box.as<float>(), codes.as<uint64_t>(), s, 31)) != hipSuccess) return e;
     return radix_sort_pairs(codes.as<uint64_t>(), nullptr, codes_sorted.as<uint64_t>(), perm.as<uint32_t>(), n,
-                            63, sort_scratch.p, s);
+                            32, sort_scratch.p, s);
   }
 };
 

@@ -118,7 +118,10 @@ hipError_t launch_add_counts(unsigned long long* work, const uint32_t* recheck, 
 constexpr int kBboxParts = 1024;
 hipError_t launch_bbox_f32(const float* pts, uint64_t n, float* d_box6, void* part, hipStream_t s);
 hipError_t launch_bbox_f64(const double* pts, uint64_t n, double* d_box6, void* part, hipStream_t s);
-hipError_t launch_codes_f32(const float* pts, uint64_t n, const float* d_box6, uint64_t* codes, hipStream_t s);
+// codes[i] = 63-bit Morton code >> shift (0: the reference's codes; the query presort keeps the
+// top 32 bits, which order 64-query packets as well and sort in half the passes)
+hipError_t launch_codes_f32(const float* pts, uint64_t n, const float* d_box6, uint64_t* codes, hipStream_t s,
+                            int shift = 0);
 hipError_t launch_codes_f64(const double* pts, uint64_t n, const double* d_box6, uint64_t* codes, hipStream_t s);
 // Bytes of scratch radix_sort needs for n keys.
 std::size_t radix_sort_scratch_bytes(uint64_t n);

@@ -55,14 +55,15 @@ __device__ __forceinline__ float feature(int f, float x0, float x1, float x2)
 }
 
 // ------------------------------------------------------------------------------------------
-// fused inference: one wavefront = 64 queries (4 tiles of 16); 4 waves per workgroup.
+// fused inference: one wavefront = kInferQT tiles of 16 queries, each weight fragment read from
+// LDS once per kInferQT MFMAs; kInferWaves waves per workgroup.
 // The BF16 weight matrices (112 KB, the inference K order) are copied once per workgroup into
 // LDS and the workgroup then strides over query tiles (one workgroup per CU), so every A
 // fragment comes from LDS instead of an L2 round trip per wavefront.
 // ------------------------------------------------------------------------------------------
 
 constexpr int kInferWaves = 8; // 2 per SIMD
-constexpr int kInferQT    = 2; // 16-query tiles per wavefront (32 queries)
+constexpr int kInferQT    = 3; // 16-query tiles per wavefront (48 queries; 4 spill at 2 waves/SIMD)
 
 __device__ __forceinline__ void infer_tile(const uint16_t* __restrict__ sw, const float* __restrict__ pts,
                                            const uint32_t* __restrict__ idx, uint64_t n, uint64_t base,

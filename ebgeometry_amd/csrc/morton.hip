@@ -85,7 +85,7 @@ hipError_t bbox_impl(const T* pts, uint64_t n, T* box, T* part, hipStream_t s)
 // ------------------------------- codes ---------------------------------------------------
 
 __global__ __launch_bounds__(256) void codes_f32(const float* __restrict__ p, uint64_t n, const float* __restrict__ box,
-                                                 uint64_t* __restrict__ codes)
+                                                 uint64_t* __restrict__ codes, int shift)
 {
   const uint64_t i = (uint64_t)blockIdx.x * 256 + threadIdx.x;
   if (i >= n) return;
@@ -100,7 +100,7 @@ __global__ __launch_bounds__(256) void codes_f32(const float* __restrict__ p, ui
     const uint32_t b = (uint32_t)(f < 0.0f ? 0.0f : (f > (float)kValidSpan ? (float)kValidSpan : f));
     code |= spread3(b) << c;
   }
-  codes[i] = code;
+  codes[i] = code >> shift;
 }
 
 __global__ __launch_bounds__(256) void codes_f64(const double* __restrict__ p, uint64_t n, const double* __restrict__ box,
@@ -275,10 +275,11 @@ hipError_t launch_bbox_f64(const double* pts, uint64_t n, double* d_box6, void* 
   return bbox_impl<double>(pts, n, d_box6, static_cast<double*>(part), s);
 }
 
-hipError_t launch_codes_f32(const float* pts, uint64_t n, const float* box, uint64_t* codes, hipStream_t s)
+hipError_t launch_codes_f32(const float* pts, uint64_t n, const float* box, uint64_t* codes, hipStream_t s, int shift)
 {
   if (n == 0) return hipSuccess;
-  hipLaunchKernelGGL(codes_f32, dim3((uint32_t)((n + 255) / 256)), dim3(256), 0, s, pts, n, box, codes);
+  if (shift < 0 || shift > 63) return hipErrorInvalidValue;
+  hipLaunchKernelGGL(codes_f32, dim3((uint32_t)((n + 255) / 256)), dim3(256), 0, s, pts, n, box, codes, shift);
   return hipGetLastError();
 }
 hipError_t launch_codes_f64(const double* pts, uint64_t n, const double* box, uint64_t* codes, hipStream_t s)

@@ -128,7 +128,8 @@ typedef struct ebg_query_opts
                               [5] triangle records fetched per wavefront (all launches),
                               [6], [7] as [0], [1] for the unseeded re-run of fallbacks
                               ([0], [1] then cover the seeded launch only).
-                              Counting uses a slower kernel variant; time without it.       */
+                              Counting uses a slower kernel variant and splits no packets
+                              (the counts are the plain traversal's); time without it.      */
   const float* seed_radius; /* optional device array (indexed by query): a caller-supplied
                               cull radius per query, used like the learned bound (queries
                               with no triangle inside it are redone unseeded). NULL = none. */
