@@ -100,7 +100,7 @@ __device__ __forceinline__ void infer_tile(const uint16_t* __restrict__ sw, cons
   for (int t = 0; t < 8; t++)
 #pragma unroll
     for (int b = 0; b < 2; b++) {
-      const u16x8 a = *reinterpret_cast<const u16x8*>(sw + kBfW1 + (16 * t + nn) * 64 + 32 * b + 8 * g);
+      const u16x8 a = *reinterpret_cast<const u16x8*>(sw + kBfW1 + ((2 * t + b) * 64 + lane) * 8);
 #pragma unroll
       for (int q = 0; q < kInferQT; q++) acc[q][t] = mfma(a, bq[q][b], acc[q][t]);
     }
@@ -134,7 +134,7 @@ __device__ __forceinline__ void infer_tile(const uint16_t* __restrict__ sw, cons
     for (int t = 0; t < 8; t++)
 #pragma unroll
       for (int b = 0; b < 4; b++) {
-        const u16x8 a = *reinterpret_cast<const u16x8*>(sw + wofs[L] + (16 * t + nn) * 128 + 32 * b + 8 * g);
+        const u16x8 a = *reinterpret_cast<const u16x8*>(sw + wofs[L] + ((4 * t + b) * 64 + lane) * 8);
 #pragma unroll
         for (int q = 0; q < kInferQT; q++) acc[q][t] = mfma(a, bq[q][b], acc[q][t]);
       }
@@ -339,19 +339,20 @@ __global__ __launch_bounds__(256) void sync_bf16_kernel(const float* __restrict_
   else if (i < kBfW4) src = kOffW3 + (i - kBfW3);
   else src = kOffW4 + (i - kBfW4);
   wtrain[i] = f2bf(master[src]);
-  // inference copy
-  if (i < kBfW2) {
-    const int r = i / kMlpInPad, c = i % kMlpInPad; // W1^T[out r][in c]
-    winfer[i] = f2bf(master[kOffW1 + c * kMlpHidden + r]);
-  }
-  else {
-    const int base = i < kBfW3 ? kBfW2 : (i < kBfW4 ? kBfW3 : kBfW4);
-    const int moff = i < kBfW3 ? kOffW2 : (i < kBfW4 ? kOffW3 : kOffW4);
-    const int j = i - base, r = j / kMlpHidden, c = j % kMlpHidden;
-    const int b = c / 32, g = (c % 32) / 8, e = c % 8;
-    const int k = 32 * b + (e < 4 ? 4 * g + e : 16 + 4 * g + (e - 4));
-    winfer[i] = f2bf(master[moff + k * kMlpHidden + r]);
-  }
+  // inference copy, fragment-major: A fragment (output tile t of 16 rows, K block b of 32) is
+  // 64 lanes x 8 BF16 contiguous, lane = row + 16 g holding K elements 8 g .. 8 g + 7, so that a
+  // wavefront's 16-byte LDS reads of one fragment fall on consecutive banks
+  const bool first = i < kBfW2;
+  const int  base  = first ? kBfW1 : (i < kBfW3 ? kBfW2 : (i < kBfW4 ? kBfW3 : kBfW4));
+  const int  moff  = first ? kOffW1 : (i < kBfW3 ? kOffW2 : (i < kBfW4 ? kOffW3 : kOffW4));
+  const int  nb    = first ? kMlpInPad / 32 : kMlpHidden / 32;
+  const int  j = i - base, e = j % 8, lane = (j / 8) % 64, frag = j / 512;
+  const int  t = frag / nb, b = frag % nb, g = lane / 16;
+  const int  r = 16 * t + lane % 16;
+  // K order: the input features as they are for layer 1; for layers 2..4 the order in which
+  // the previous layer's accumulators sit in a lane (rows 4 g .. 4 g + 3 of tiles 2b, 2b+1)
+  const int k = first ? 32 * b + 8 * g + e : 32 * b + (e < 4 ? 4 * g + e : 16 + 4 * g + (e - 4));
+  winfer[i]   = f2bf(master[moff + k * kMlpHidden + r]); // W^T[out r][in k]
 }
 
 __global__ void scale_kernel(float* p, int n, float s)

@@ -83,3 +83,22 @@ def test_split_packets_with_learned_bound(cuda):
     diag = float(np.linalg.norm(v.max(0).astype(np.float64) - v.min(0)))
     assert torch.equal(torch.sign(d0), torch.sign(d1))
     assert (d0 - d1).abs().max().item() <= 1e-6 * diag
+
+
+@pytest.mark.parametrize("name", ["tetrahedron.npz", "dodecahedron.npz"])
+def test_split_on_tiny_meshes(cuda, golden, name):
+    """Tiny meshes: the subtree frontier has fewer than 16 roots, some of them leaves. A query
+    set of 2^15 points (the golden queries tiled, their centroid-heavy part included) must give
+    the reference's golden distances whether or not packets are split."""
+    import torch
+    g = golden(name)
+    v, f = g["verts"], g["faces"]
+    m = TriMeshSDF(v, f)
+    reps = (1 << 15) // g["queries"].shape[0] + 1
+    q = np.tile(g["queries"].astype(np.float32), (reps, 1))
+    d = m.query(torch.from_numpy(q).to(cuda)).cpu().numpy()
+    diag = float(np.linalg.norm(v.max(0).astype(np.float64) - v.min(0)))
+    ref = np.tile(g["d_sdf"], reps)
+    assert np.abs(d - ref).max() <= 1e-5 * diag
+    nz = ref != 0
+    assert np.array_equal(np.sign(d[nz]), np.sign(ref[nz]))
