@@ -14,6 +14,7 @@
 //   3. batch64: packets that hold near-surface results were listed (one entry per packet with
 //      the lane mask); one wavefront per listed packet redoes those lanes in FP64.
 #include "kernels.hpp"
+#include "sfc_dev.hpp"
 #include "traverse_dev.hpp"
 
 #include <hip/hip_runtime.h>
@@ -24,20 +25,9 @@ namespace {
 
 using namespace dev;
 
-__device__ __forceinline__ uint64_t spread3b(uint64_t a)
-{
-  uint64_t b = a & 0x1fffffull;
-  b          = (b | b << 32) & 0x1f00000000ffffull;
-  b          = (b | b << 16) & 0x1f0000ff0000ffull;
-  b          = (b | b << 8) & 0x100f00f00f00f00full;
-  b          = (b | b << 4) & 0x10c30c30c30c30c3ull;
-  b          = (b | b << 2) & 0x1249249249249249ull;
-  return b;
-}
-
-// Sort key of flat query g: (item << 24) | top 24 bits of its 63-bit Morton code in the item's
-// box (bins clamped to 21 bits; 8 bits per axis order 64-query packets well at any item size).
-// Item found by binary search over the item bases.
+// Sort key of flat query g: (item << 24) | 24-bit Hilbert index of the query on a 256^3 grid of
+// the item's box (the mesh box grown by 25%; 8 bits per axis order 64-query packets well at any
+// item size). Item found by binary search over the item bases.
 __global__ __launch_bounds__(256) void batch_keys_kernel(const BatchItemDev* __restrict__ items, uint32_t num_items,
                                                          uint64_t total, uint64_t* __restrict__ keys)
 {
@@ -51,14 +41,11 @@ __global__ __launch_bounds__(256) void batch_keys_kernel(const BatchItemDev* __r
   }
   const BatchItemDev& it = items[lo];
   const uint64_t      j  = g - it.base;
-  uint64_t            code = 0;
+  uint32_t            x[3];
 #pragma unroll
-  for (int c = 0; c < 3; c++) {
-    float b = (it.pts[3 * j + c] - it.box_lo[c]) * it.box_inv[c];
-    b       = fminf(fmaxf(b, 0.0f), 2097151.0f); // NaN -> 0
-    code |= spread3b((uint64_t)b) << c;
-  }
-  keys[g] = ((uint64_t)lo << 24) | (code >> 39);
+  for (int c = 0; c < 3; c++)
+    x[c] = (uint32_t)fminf(fmaxf((it.pts[3 * j + c] - it.box_lo[c]) * it.box_inv[c], 0.0f), 255.0f); // NaN -> 0
+  keys[g] = ((uint64_t)lo << 24) | hilbert_index<8>(x[0], x[1], x[2]);
 }
 
 template <int K, int NB, bool COUNT>

@@ -378,7 +378,7 @@ ebg_status ebg_batch_plan_create(ebg_batch* b, const ebg_batch_item* items, uint
     for (int c = 0; c < 3; c++) {
       const double mid = 0.5 * (m->lo[c] + m->hi[c]), h = 0.625 * (m->hi[c] - m->lo[c]);
       r.box_lo[c]      = float(mid - h);
-      r.box_inv[c]     = h > 0 ? float(2097151.0 / (2.0 * h)) : 0.0f;
+      r.box_inv[c]     = h > 0 ? float(256.0 / (2.0 * h)) : 0.0f;
     }
     r.recheck_abs = o.fp64_all ? INFINITY : float(o.recheck_eps * m->diag);
     r.slack_abs   = float(1e-5 * m->diag);

@@ -93,7 +93,7 @@ struct Workspace
     if ((e = lists.reserve(8 * m + 256)) != hipSuccess) return e;
     if ((e = seeds.reserve(4 * m)) != hipSuccess) return e;
     // split packets: one flag byte per packet, the list of split packets, 64 result keys each
-    if ((e = sched.reserve(sched_keys_offset(m) + 512 * (size_t)heavy_cap(m))) != hipSuccess) return e;
+    if ((e = sched.reserve(sched_early_offset(m) + 4 * (size_t)early_cap(m))) != hipSuccess) return e;
     cap = m;
     return hipSuccess;
   }
@@ -101,6 +101,9 @@ struct Workspace
   static uint32_t heavy_cap(uint64_t n) { return (uint32_t)(packets(n) / 64 + 64); }
   static size_t   sched_list_offset(uint64_t n) { return (packets(n) + 255) & ~size_t(255); }
   static size_t   sched_keys_offset(uint64_t n) { return sched_list_offset(n) + ((4 * (size_t)heavy_cap(n) + 255) & ~size_t(255)); }
+  static uint32_t early_cap(uint64_t n) { return (uint32_t)(packets(n) / 8 + 64); }
+  static size_t   sched_early_offset(uint64_t n) { return sched_keys_offset(n) + 512 * (size_t)heavy_cap(n); }
+  uint32_t*       early_list() const { return reinterpret_cast<uint32_t*>(sched.as<uint8_t>() + sched_early_offset(cap)); }
   uint8_t*        heavy_flag() const { return sched.as<uint8_t>(); }
   uint32_t*       heavy_list() const { return reinterpret_cast<uint32_t*>(sched.as<uint8_t>() + sched_list_offset(cap)); }
   unsigned long long* heavy_keys() const
@@ -109,17 +112,17 @@ struct Workspace
   }
   uint32_t* recheck_list() const { return lists.as<uint32_t>(); }
   uint32_t* fallback_list() const { return lists.as<uint32_t>() + cap; }
-  uint32_t* counters() const { return lists.as<uint32_t>() + 2 * cap; } // [0] recheck [1] fallback [2] heavy
+  uint32_t* counters() const { return lists.as<uint32_t>() + 2 * cap; } // [0] recheck [1] fallback [2] split [3] early
 
-  // Morton presort of n device points: returns the permutation (valid until the next call).
+  // Hilbert presort of n device points: returns the permutation (valid until the next call).
   hipError_t presort(const float* d_pts, uint64_t n, hipStream_t s)
   {
     hipError_t e;
     if ((e = launch_bbox_f32(d_pts, n, box.as<float>(), box.as<float>() + 8, s)) != hipSuccess) return e;
-    // top 32 of the 63 code bits (about 10.7 per axis): 4 radix passes instead of 8
-    if ((e = launch_codes_f32(d_pts, n, box.as<float>(), codes.as<uint64_t>(), s, 31)) != hipSuccess) return e;
+    // 30-bit Hilbert keys on a 1024^3 grid: 4 radix passes
+    if ((e = launch_hilbert_f32(d_pts, n, box.as<float>(), codes.as<uint64_t>(), s)) != hipSuccess) return e;
     return radix_sort_pairs(codes.as<uint64_t>(), nullptr, codes_sorted.as<uint64_t>(), perm.as<uint32_t>(), n,
-                            32, sort_scratch.p, s);
+                            30, sort_scratch.p, s);
   }
 };
 

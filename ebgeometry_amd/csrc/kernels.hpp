@@ -37,6 +37,11 @@ struct QueryArgs
   const uint32_t*     split_count;
   unsigned long long* split_keys;
   uint32_t            split_nroots, split_cap, split_blocks;
+  // Early packets: the next early_blocks workgroups take early_list[0 .. min(*early_count,
+  // early_cap)) (flagged too), so that the costlier unsplit packets start first.
+  const uint32_t*     early_list;
+  const uint32_t*     early_count;
+  uint32_t            early_cap, early_blocks;
 };
 
 // Expensive packets: a packet whose most expensive query (cost-grid value of its cell, grid of
@@ -54,6 +59,10 @@ struct HeavyMarkArgs
   uint32_t*           count;
   uint8_t*            flag;
   unsigned long long* keys; // cap * 64
+  // packets below threshold but at least early_threshold go to the early list (up to early_cap)
+  uint32_t            early_threshold, early_cap;
+  uint32_t*           early_list;
+  uint32_t*           early_count;
 };
 hipError_t launch_heavy_mark(const HeavyMarkArgs& h, hipStream_t s);
 
@@ -82,7 +91,7 @@ struct alignas(16) BatchItemDev
   uint32_t*    tri;
   uint64_t     base;       // flat index of the item's first query
   uint32_t     n, mesh;
-  float        box_lo[3];  // Morton box of the item's mesh: bin = (p - lo) * inv, 21 bits
+  float        box_lo[3];  // key box of the item's mesh: bin = (p - lo) * inv, 8 bits per axis
   float        box_inv[3];
   float        recheck_abs; // |d| below this is redone in FP64 (recheck_eps * mesh diagonal)
   float        slack_abs;   // FP64 seed slack (1e-5 * mesh diagonal)
@@ -122,6 +131,8 @@ hipError_t launch_bbox_f64(const double* pts, uint64_t n, double* d_box6, void* 
 // top 32 bits, which order 64-query packets as well and sort in half the passes)
 hipError_t launch_codes_f32(const float* pts, uint64_t n, const float* d_box6, uint64_t* codes, hipStream_t s,
                             int shift = 0);
+// keys[i] = 30-bit Hilbert index of the point on a 1024^3 grid of d_box6 (query presort)
+hipError_t launch_hilbert_f32(const float* pts, uint64_t n, const float* d_box6, uint64_t* keys, hipStream_t s);
 hipError_t launch_codes_f64(const double* pts, uint64_t n, const double* d_box6, uint64_t* codes, hipStream_t s);
 // Bytes of scratch radix_sort needs for n keys.
 std::size_t radix_sort_scratch_bytes(uint64_t n);

@@ -9,6 +9,7 @@
 // The sort replaces the reference's std::sort on codes (:279-300) with a stable LSD radix sort,
 // so equal codes keep increasing index order.
 #include "kernels.hpp"
+#include "sfc_dev.hpp"
 
 #include <hip/hip_runtime.h>
 
@@ -101,6 +102,21 @@ __global__ __launch_bounds__(256) void codes_f32(const float* __restrict__ p, ui
     code |= spread3(b) << c;
   }
   codes[i] = code >> shift;
+}
+
+// Query presort key: position on a 3-D Hilbert curve over a 1024^3 grid of the query box (30
+// bits). Consecutive cells of the curve share a face, so a run of 64 sorted queries never jumps
+// across the box the way a Morton run can at a high-level boundary (C2: 10% fewer triangle
+// evaluations per wavefront).
+__global__ __launch_bounds__(256) void hilbert_f32(const float* __restrict__ p, uint64_t n,
+                                                   const float* __restrict__ box, uint64_t* __restrict__ keys)
+{
+  const uint64_t i = (uint64_t)blockIdx.x * 256 + threadIdx.x;
+  if (i >= n) return;
+  uint32_t x[3];
+#pragma unroll
+  for (int c = 0; c < 3; c++) x[c] = dev::hilbert_bin<10>(p[3 * i + c], box[c], box[3 + c] - box[c]);
+  keys[i] = dev::hilbert_index<10>(x[0], x[1], x[2]);
 }
 
 __global__ __launch_bounds__(256) void codes_f64(const double* __restrict__ p, uint64_t n, const double* __restrict__ box,
@@ -280,6 +296,12 @@ hipError_t launch_codes_f32(const float* pts, uint64_t n, const float* box, uint
   if (n == 0) return hipSuccess;
   if (shift < 0 || shift > 63) return hipErrorInvalidValue;
   hipLaunchKernelGGL(codes_f32, dim3((uint32_t)((n + 255) / 256)), dim3(256), 0, s, pts, n, box, codes, shift);
+  return hipGetLastError();
+}
+hipError_t launch_hilbert_f32(const float* pts, uint64_t n, const float* box, uint64_t* keys, hipStream_t s)
+{
+  if (n == 0) return hipSuccess;
+  hipLaunchKernelGGL(hilbert_f32, dim3((uint32_t)((n + 255) / 256)), dim3(256), 0, s, pts, n, box, keys);
   return hipGetLastError();
 }
 hipError_t launch_codes_f64(const double* pts, uint64_t n, const double* box, uint64_t* codes, hipStream_t s)

@@ -182,11 +182,20 @@ __global__ __launch_bounds__(256) void sdf32_kernel(const MeshDev m, const Query
     b -= a.split_blocks;
     nb -= a.split_blocks;
   }
+  if (a.early_blocks) {
+    if (b < a.early_blocks) {
+      const uint32_t j = b * 4u + wave;
+      if (j < min(*a.early_count, a.early_cap)) sdf32_packet<K, NB, COUNT, MODE>(m, a, a.n, a.early_list[j]);
+      return;
+    }
+    b -= a.early_blocks;
+    nb -= a.early_blocks;
+  }
   const uint32_t blk = xcd_remap(b, nb);
   const uint64_t n   = a.n;
   const uint64_t pk  = (uint64_t)blk * 4u + wave;
   if (pk * 64u >= n) return;                    // whole wavefront past the end
-  if (a.heavy_flag && a.heavy_flag[pk]) return; // whole wavefront: a split packet
+  if (a.heavy_flag && a.heavy_flag[pk]) return; // whole wavefront: a split or early packet
   sdf32_packet<K, NB, COUNT, MODE>(m, a, n, pk);
 }
 
@@ -269,10 +278,11 @@ void launch32m(const MeshDev& m, const QueryArgs& a, bool count, uint32_t blocks
 template <int K, int NB>
 hipError_t launch32(const MeshDev& m, const QueryArgs& a, bool count, hipStream_t s)
 {
-  const uint32_t blocks = (uint32_t)((a.n + 255) / 256) + a.split_blocks;
+  const uint32_t blocks = (uint32_t)((a.n + 255) / 256) + a.split_blocks + a.early_blocks;
   if (blocks == 0) return hipSuccess;
   if (a.count_ptr) return hipErrorInvalidValue;
   if (a.split_blocks && (!a.split_nroots || !a.split_count || !a.heavy_flag)) return hipErrorInvalidValue;
+  if (a.early_blocks && (!a.early_list || !a.early_count || !a.heavy_flag)) return hipErrorInvalidValue;
   if (a.seed_r) {
     if (!a.fallback_count) return hipErrorInvalidValue;
     launch32m<K, NB, kSeeded>(m, a, count, blocks, s);
@@ -327,14 +337,18 @@ __global__ __launch_bounds__(256) void heavy_mark_kernel(const HeavyMarkArgs h)
     c = h.grid[((size_t)cell[2] * h.g + cell[1]) * h.g + cell[0]];
   }
   const uint32_t est  = wave_max_u32(c);
-  uint32_t       slot = kInvalid;
+  uint32_t       slot = kInvalid, eslot = kInvalid;
   if (est >= h.threshold && __lane_id() == 0) slot = atomicAdd(h.count, 1u);
   slot = rdlane(slot, 0);
   const bool heavy = slot < h.cap;
   if (heavy) h.keys[64 * (size_t)slot + __lane_id()] = ~0ull;
+  if (!heavy && h.early_list && est >= h.early_threshold && __lane_id() == 0) eslot = atomicAdd(h.early_count, 1u);
+  eslot            = rdlane(eslot, 0);
+  const bool early = eslot < h.early_cap;
   if (__lane_id() == 0) {
     if (heavy) h.list[slot] = (uint32_t)pk;
-    h.flag[pk] = heavy ? 1 : 0;
+    if (early) h.early_list[eslot] = (uint32_t)pk;
+    h.flag[pk] = heavy ? 1 : (early ? 2 : 0);
   }
 }
 
