@@ -7,12 +7,12 @@
 //   * a node or triangle record is fetched once per wavefront with scalar loads into SGPRs
 //     and every lane tests its own query against it;
 //   * a child is pushed when any lane's box distance is within that lane's bound (ballot);
-//     children are ordered by the wave-minimum box distance (nearest popped first), which is
-//     the reference's descending-d^2 push order (Source/EBGeometry_BVHImplem.hpp:1178-1194)
-//     applied to the packet;
+//     children are ordered by the box distance seen by the packet's middle lane (nearest
+//     popped first), which is the reference's descending-d^2 push order
+//     (Source/EBGeometry_BVHImplem.hpp:1178-1194) applied to the packet;
 //   * on pop, each lane re-tests the entry box against its current bound (the reference's
 //     `entry.dist2 > curBest2` check, :1145-1149); the entry is skipped when no lane needs it.
-// Queries arrive presorted by Morton code, so the 64 queries of a wave are spatial
+// Queries arrive presorted along a Hilbert curve, so the 64 queries of a wave are spatial
 // neighbours and visit nearly the same nodes.
 //
 // The per-triangle math follows TriangleSoAT::signedDistanceLane
