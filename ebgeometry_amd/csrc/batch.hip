@@ -49,10 +49,11 @@ __global__ __launch_bounds__(256) void batch_keys_kernel(const BatchItemDev* __r
 }
 
 template <int K, int NB, bool COUNT>
-__global__ __launch_bounds__(256) void batch32_kernel(const BatchArgs a)
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(K == 4 ? 8 : 1))) void batch32_kernel(const BatchArgs a)
 {
   const uint32_t wpb = blockDim.x >> 6;
-  const uint32_t p   = xcd_remap(blockIdx.x, gridDim.x) * wpb + (threadIdx.x >> 6);
+  // wave-uniform, and known to be: the packet and mesh records below are then scalar loads
+  const uint32_t p   = uni(xcd_remap(blockIdx.x, gridDim.x) * wpb + (threadIdx.x >> 6));
   if (p >= a.num_packets) return; // whole wavefront (p is wave-uniform)
   const SWords<4>    pk = sload<4>(a.packets + 4 * (size_t)p);
   const uint32_t     item = pk.w[0], first = pk.w[1];

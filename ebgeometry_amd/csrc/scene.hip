@@ -58,6 +58,7 @@ __device__ __forceinline__ void scene_traverse(const SceneDev& sc, R px, R py, R
   st.init();
   const int  lane   = __lane_id();
   const bool active = bound >= R(0);
+  const int  rep    = 0; // the scene tree orders children by the wave minimum
   int        top    = 0;
   Entry      root{0u, 0u, sc.root_lo[0], sc.root_lo[1], sc.root_lo[2], sc.root_hi[0], sc.root_hi[1], sc.root_hi[2]};
   st.put(top++, root, lane);
@@ -70,7 +71,7 @@ __device__ __forceinline__ void scene_traverse(const SceneDev& sc, R px, R py, R
 
     if (e.cnt == 0u) {
       if (COUNT) { c.n_inner += need ? 1u : 0u; c.w_inner++; }
-      push_children<4, NB, R>(sload_record(nodes + uni(e.ref)), px, py, pz, bound, st, top, lane);
+      push_children<4, NB, R, true>(sload_record(nodes + uni(e.ref)), px, py, pz, bound, st, top, lane, rep);
       continue;
     }
     const uint32_t first = uni(e.ref), cnt = uni(e.cnt);
@@ -117,8 +118,8 @@ __device__ __forceinline__ void scene_traverse(const SceneDev& sc, R px, R py, R
         }
         R        best = INFINITY;
         uint32_t btri = kInvalid, bfeat = 0;
-        traverse<KM, NB, R, COUNT>(md, qx, qy, qz, pn ? R(INFINITY) : R(-1), best, btri, bfeat, c.n_inner, c.n_tris,
-                                   c.w_inner, c.w_tris);
+        traverse<KM, NB, R, COUNT, true>(md, qx, qy, qz, pn ? R(INFINITY) : R(-1), best, btri, bfeat, c.n_inner,
+                                         c.n_tris, c.w_inner, c.w_tris);
         // Lanes that did not ask for this instance (bound -1) still saw the triangles the
         // wavefront evaluated, so their btri names some triangle but not their nearest one.
         val = R(INFINITY);
@@ -138,7 +139,7 @@ __device__ __forceinline__ void scene_traverse(const SceneDev& sc, R px, R py, R
 
 // FP32 pass, one 64-query packet per wavefront, 4 per workgroup, XCD-aware block order.
 template <int KM, int NB, bool COUNT>
-__global__ __launch_bounds__(256) void scene32_kernel(const SceneDev sc, const QueryArgs a)
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(KM == 4 ? 8 : 1))) void scene32_kernel(const SceneDev sc, const QueryArgs a)
 {
   const uint32_t blk = xcd_remap(blockIdx.x, gridDim.x);
   const uint64_t n   = a.n;

@@ -166,9 +166,9 @@ __device__ __forceinline__ void split_item(const MeshDev& m, const QueryArgs& a,
 // The leading split_blocks workgroups (dispatched first) work through the split items instead,
 // beside the packets of the bulk; packets flagged in heavy_flag are skipped.
 template <int K, int NB, bool COUNT, int MODE>
-__global__ __launch_bounds__(256) void sdf32_kernel(const MeshDev m, const QueryArgs a)
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(K == 4 ? 8 : 1))) void sdf32_kernel(const MeshDev m, const QueryArgs a)
 {
-  const uint32_t wave = threadIdx.x >> 6;
+  const uint32_t wave = uni(threadIdx.x >> 6); // known uniform: list entries load as scalars
   uint32_t       b = blockIdx.x, nb = gridDim.x;
   if (a.split_blocks) {
     if (b < a.split_blocks) {
@@ -202,7 +202,7 @@ __global__ __launch_bounds__(256) void sdf32_kernel(const MeshDev m, const Query
 // One wavefront per listed packet: the winning triangle of each query, as the plain pass ends.
 __global__ __launch_bounds__(256) void sdf32_resolve_kernel(const MeshDev m, const QueryArgs a)
 {
-  const uint32_t j = blockIdx.x * 4u + (threadIdx.x >> 6);
+  const uint32_t j = uni(blockIdx.x * 4u + (threadIdx.x >> 6));
   if (j >= min(*a.split_count, a.split_cap)) return; // whole wavefront
   const uint64_t pk     = a.split_list[j];
   const uint64_t tid    = pk * 64u + __lane_id();
@@ -322,7 +322,7 @@ namespace {
 // largest cost-grid value over its queries' cells.
 __global__ __launch_bounds__(256) void heavy_mark_kernel(const HeavyMarkArgs h)
 {
-  const uint64_t pk     = (uint64_t)blockIdx.x * 4u + (threadIdx.x >> 6);
+  const uint64_t pk     = (uint64_t)uni(blockIdx.x * 4u + (threadIdx.x >> 6));
   const uint64_t tid    = pk * 64u + (uint64_t)__lane_id();
   if (pk * 64u >= h.n) return; // whole wavefront
   uint32_t c = 0;

@@ -26,8 +26,18 @@ __device__ __forceinline__ uint32_t uni(uint32_t v) { return (uint32_t)__builtin
 // Lane mask of a predicate, taken straight from the compare (no round trip through an int).
 __device__ __forceinline__ uint64_t ballot(bool p) { return __builtin_amdgcn_ballot_w64(p); }
 
-// Maximum over the 64 lanes of an unsigned value (all lanes active): DPP within rows of 16,
-// then the four row results combined in scalar registers.
+// Minimum / maximum over the 64 lanes of an unsigned value (all lanes active): DPP within rows
+// of 16, then the four row results combined in scalar registers.
+__device__ __forceinline__ uint32_t wave_min_u32(uint32_t v)
+{
+  v = min(v, (uint32_t)__builtin_amdgcn_mov_dpp((int)v, 0xB1, 0xF, 0xF, false));  // quad_perm 1,0,3,2
+  v = min(v, (uint32_t)__builtin_amdgcn_mov_dpp((int)v, 0x4E, 0xF, 0xF, false));  // quad_perm 2,3,0,1
+  v = min(v, (uint32_t)__builtin_amdgcn_mov_dpp((int)v, 0x141, 0xF, 0xF, false)); // row_half_mirror
+  v = min(v, (uint32_t)__builtin_amdgcn_mov_dpp((int)v, 0x140, 0xF, 0xF, false)); // row_mirror
+  const uint32_t a = rdlane(v, 0), b = rdlane(v, 16), c = rdlane(v, 32), d = rdlane(v, 48);
+  return min(min(a, b), min(c, d));
+}
+
 __device__ __forceinline__ uint32_t wave_max_u32(uint32_t v)
 {
   v = max(v, (uint32_t)__builtin_amdgcn_mov_dpp((int)v, 0xB1, 0xF, 0xF, false));
@@ -381,12 +391,14 @@ __device__ __forceinline__ void wave_append(bool pred, uint32_t idx, uint32_t* l
 
 // Interior node: test the K child boxes and push the ones some lane needs, ranked so that the
 // farthest lies deepest and the nearest is popped next. The order key is the box distance seen
-// by lane 32, the middle of the (curve-sorted) packet: one readlane instead of a wavefront
-// minimum (DPP and 4 readlanes per child), and as good an order (C2: 1.6% fewer triangle
-// evaluations, 5.7% less time).
-template <int K, int NB, class R>
+// by the representative lane `rep` (see rep_lane) for dense, curve-sorted mesh packets: one
+// readlane instead of a wavefront minimum (DPP and 4 readlanes per child), and as good an
+// order there (C2: 1.6% fewer triangle evaluations, 5.7% less time). With WMIN the key is the
+// minimum over the lanes that need the child: better for packets whose lanes are spread (the
+// FP64 passes over sparse lists, CSG scenes with primitives the size of a packet).
+template <int K, int NB, class R, bool WMIN>
 __device__ __forceinline__ void push_children(const Node<K>& nd, R px, R py, R pz, R bound, LaneStack<NB>& st,
-                                              int& top, int lane)
+                                              int& top, int lane, int rep)
 {
   uint32_t key[K];
   uint64_t msk[K];
@@ -398,7 +410,10 @@ __device__ __forceinline__ void push_children(const Node<K>& nd, R px, R py, R p
       const R    d2 = box_d2<R>(px, py, pz, nd.lox[k], nd.loy[k], nd.loz[k], nd.hix[k], nd.hiy[k], nd.hiz[k]);
       const bool nk = d2 <= bound;
       msk[k]        = ballot(nk);
-      if (msk[k]) key[k] = rdlane(__float_as_uint(float(d2)), 32);
+      if (msk[k]) {
+        if constexpr (WMIN) key[k] = wave_min_u32(nk ? __float_as_uint(float(d2)) : 0xFFFFFFFFu);
+        else key[k] = rdlane(__float_as_uint(float(d2)), rep);
+      }
     }
   }
   int pushed = 0;
@@ -418,9 +433,19 @@ __device__ __forceinline__ void push_children(const Node<K>& nd, R px, R py, R p
   top += pushed;
 }
 
+// Representative lane of a packet for the child order: lane 32, the middle of a curve-sorted
+// packet, when it is active (bound >= 0), else the first active lane (list-driven passes, where
+// the active lanes are sparse).
+template <class R>
+__device__ __forceinline__ int rep_lane(R bound)
+{
+  const uint64_t act = ballot(bound >= R(0));
+  return ((act >> 32) & 1ull) ? 32 : (act ? __builtin_ctzll(act) : 0);
+}
+
 // R = float: FP32 pass on TriRec records. R = double: FP64 recheck on raw positions.
 // The walk starts at `start` (the root, or a subtree root of a split packet).
-template <int K, int NB, class R, bool COUNT>
+template <int K, int NB, class R, bool COUNT, bool WMIN = (sizeof(R) == 8)>
 __device__ __forceinline__ void traverse_from(const MeshDev& m, const Entry& start, R px, R py, R pz, R bound, R& best,
                                               uint32_t& btri, uint32_t& bfeat, uint32_t& n_inner, uint32_t& n_tris,
                                               uint32_t& w_inner, uint32_t& w_tris)
@@ -429,6 +454,7 @@ __device__ __forceinline__ void traverse_from(const MeshDev& m, const Entry& sta
   LaneStack<NB>               st;
   st.init();
   const int lane = __lane_id();
+  const int rep  = rep_lane(bound);
   int       top  = 0;
   st.put(top++, start, lane);
 
@@ -440,7 +466,7 @@ __device__ __forceinline__ void traverse_from(const MeshDev& m, const Entry& sta
 
     if (e.cnt == 0u) {
       if (COUNT) { n_inner += need ? 1u : 0u; w_inner++; }
-      push_children<K, NB, R>(sload_record(nodes + uni(e.ref)), px, py, pz, bound, st, top, lane);
+      push_children<K, NB, R, WMIN>(sload_record(nodes + uni(e.ref)), px, py, pz, bound, st, top, lane, rep);
     }
     else {
       // leaf: a triangle is evaluated by the whole wavefront (keeping the minimum is exact for
@@ -518,12 +544,13 @@ __device__ __forceinline__ void traverse_from(const MeshDev& m, const Entry& sta
   }
 }
 
-template <int K, int NB, class R, bool COUNT>
+template <int K, int NB, class R, bool COUNT, bool WMIN = (sizeof(R) == 8)>
 __device__ __forceinline__ void traverse(const MeshDev& m, R px, R py, R pz, R bound, R& best, uint32_t& btri,
                                          uint32_t& bfeat, uint32_t& n_inner, uint32_t& n_tris, uint32_t& w_inner, uint32_t& w_tris)
 {
   const Entry root{0u, 0u, m.root_lo[0], m.root_lo[1], m.root_lo[2], m.root_hi[0], m.root_hi[1], m.root_hi[2]};
-  traverse_from<K, NB, R, COUNT>(m, root, px, py, pz, bound, best, btri, bfeat, n_inner, n_tris, w_inner, w_tris);
+  traverse_from<K, NB, R, COUNT, WMIN>(m, root, px, py, pz, bound, best, btri, bfeat, n_inner, n_tris, w_inner,
+                                       w_tris);
 }
 
 } // namespace dev
