@@ -209,7 +209,12 @@ def batched_bench(a):
     log(f"batch built in {time.perf_counter() - t0:.1f}s: K={info['branching']} leaf={info['leaf_size']}")
     items = [(i, torch.from_numpy(q).to("cuda:0")) for i, q in enumerate(w.queries)]
     plan = b.plan(items, capture=True)
-    dt, kms = timed(plan.launch, a.steps, a.warmup, plan.last_traversal_ms)
+    dt, replay_ms = timed(plan.launch, a.steps, a.warmup, plan.last_traversal_ms)
+    # the traversal kernel alone: the same items and workgroup through a plan without a graph
+    # (events inside a graph are not timed), timed the same way
+    eager = b.plan(items, capture=False, workgroup=plan.workgroup)
+    _, kms = timed(eager.launch, a.steps, a.warmup, eager.last_traversal_ms)
+    del eager
     wk = torch.zeros(8, dtype=torch.int64, device="cuda:0")
     b.query(items, work=wk.data_ptr())
     torch.cuda.synchronize()
@@ -223,6 +228,7 @@ def batched_bench(a):
            "config": {"workload": "batched-256-meshes", "meshes": 256, "faces": nt, "queries": nq, "seed": 3,
                       "branching": K, "leaf_size": info["leaf_size"], "workgroup": plan.workgroup,
                       "tune_ms": info["tune_ms"]},
+           "graph_replay_ms": round(replay_ms, 3),
            "roofline": {"bound": "hbm", "achieved": round(gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                         "frac": round(gbs / HBM_PEAK_GBS, 4), "traffic": None,
                         "kernel": f"batch32_kernel<{K},{info['stack_capacity'] // 64},false>",

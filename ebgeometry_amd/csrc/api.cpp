@@ -478,8 +478,10 @@ double ebg_mesh_last_traversal_ms(const ebg_mesh* cm)
   ebg_mesh* m = const_cast<ebg_mesh*>(cm);
   if (!m || !m->timed) return -1.0;
   float ms = -1.0f;
-  if (hipEventSynchronize(m->ev1) != hipSuccess) return -1.0;
-  if (hipEventElapsedTime(&ms, m->ev0, m->ev1) != hipSuccess) return -1.0;
+  if (hipEventSynchronize(m->ev1) != hipSuccess || hipEventElapsedTime(&ms, m->ev0, m->ev1) != hipSuccess) {
+    (void)hipGetLastError(); // keep the failure out of the caller's next HIP call
+    return -1.0;
+  }
   return ms;
 }
 

@@ -199,11 +199,15 @@ ebg_status enqueue(ebg_batch_plan* p, hipStream_t s, bool events)
   return EBG_OK;
 }
 
+// Milliseconds between two recorded events, or -1; a failure is cleared from the runtime's
+// last-error state so that it does not surface in the caller's next HIP call.
 double elapsed_ms(hipEvent_t a, hipEvent_t b)
 {
   float ms = -1.0f;
-  if (hipEventSynchronize(b) != hipSuccess) return -1.0;
-  if (hipEventElapsedTime(&ms, a, b) != hipSuccess) return -1.0;
+  if (hipEventSynchronize(b) != hipSuccess || hipEventElapsedTime(&ms, a, b) != hipSuccess) {
+    (void)hipGetLastError();
+    return -1.0;
+  }
   return ms;
 }
 
@@ -426,6 +430,7 @@ ebg_status ebg_batch_plan_create(ebg_batch* b, const ebg_batch_item* items, uint
     return bail(e, "events");
 
   hipStream_t s = static_cast<hipStream_t>(o.stream);
+
   if (!workgroup && p->num_packets > 0) {
     // pick the workgroup shape by timing the traversal on the items as they are
     double best = 1e300;
@@ -452,7 +457,7 @@ ebg_status ebg_batch_plan_create(ebg_batch* b, const ebg_batch_item* items, uint
     hipStream_t cs = nullptr;
     if ((e = hipStreamCreateWithFlags(&cs, hipStreamNonBlocking)) != hipSuccess) return bail(e, "capture stream");
     e = hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal);
-    ebg_status st = e == hipSuccess ? enqueue(p, cs, true) : EBG_HIP_ERROR;
+    ebg_status st = e == hipSuccess ? enqueue(p, cs, false) : EBG_HIP_ERROR;
     hipGraph_t g  = nullptr;
     const hipError_t e2 = hipStreamEndCapture(cs, &g);
     (void)hipStreamDestroy(cs);
@@ -475,7 +480,11 @@ ebg_status ebg_batch_plan_launch(ebg_batch_plan* p, void* stream)
   if (!p) return fail(EBG_INVALID_ARG, "plan is null");
   if (p->num_packets == 0) return EBG_OK;
   hipStream_t s = static_cast<hipStream_t>(stream);
-  if (p->exec) EBG_HIP(hipGraphLaunch(p->exec, s));
+  if (p->exec) {
+    EBG_HIP(hipEventRecord(p->ev0, s));
+    EBG_HIP(hipGraphLaunch(p->exec, s));
+    EBG_HIP(hipEventRecord(p->ev1, s));
+  }
   else {
     const ebg_status st = enqueue(p, s, true);
     if (st != EBG_OK) return st;

@@ -342,8 +342,10 @@ double ebg_scene_last_traversal_ms(const ebg_scene* csc)
   ebg_scene* sc = const_cast<ebg_scene*>(csc);
   if (!sc || !sc->timed) return -1.0;
   float ms = -1.0f;
-  if (hipEventSynchronize(sc->ev1) != hipSuccess) return -1.0;
-  if (hipEventElapsedTime(&ms, sc->ev0, sc->ev1) != hipSuccess) return -1.0;
+  if (hipEventSynchronize(sc->ev1) != hipSuccess || hipEventElapsedTime(&ms, sc->ev0, sc->ev1) != hipSuccess) {
+    (void)hipGetLastError(); // keep the failure out of the caller's next HIP call
+    return -1.0;
+  }
   return ms;
 }
 

@@ -273,8 +273,10 @@ ebg_status ebg_batch_plan_create(ebg_batch* batch, const ebg_batch_item* items, 
                                  ebg_batch_plan** out);
 ebg_status ebg_batch_plan_launch(ebg_batch_plan* plan, void* stream);
 ebg_status ebg_batch_plan_destroy(ebg_batch_plan* plan);
-/* Workgroup size the plan uses, and its traversal kernel time of the last launch (HIP events
- * around that launch on the launch stream; waits for it). */
+/* Workgroup size the plan uses, and the time of its last launch (HIP events on the launch
+ * stream; waits for it): for a plan without a graph, around the traversal kernel alone; for a
+ * captured plan, around the whole graph replay (keys, sort, traversal, FP64 pass) -- events
+ * recorded inside a graph are not timed. */
 int32_t    ebg_batch_plan_workgroup(const ebg_batch_plan* plan);
 double     ebg_batch_plan_last_traversal_ms(const ebg_batch_plan* plan);
 

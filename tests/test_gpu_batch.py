@@ -146,3 +146,20 @@ def test_batch_rejects_bad_items(small_batch, cuda):
     q = torch.zeros((10, 3), dtype=torch.float32, device=cuda)
     with pytest.raises(_lib.EbgError):
         b.query([(99, q)])
+
+
+def test_plan_with_fixed_workgroup_times_and_leaves_no_error(cuda):
+    """A captured plan with a caller-chosen workgroup (no timing launches before the capture)
+    still reports its traversal time, and a second plan can be made right after."""
+    import torch
+    w = wl.batched_meshes(3, 21, level_range=(2, 3), query_range=(1000, 2000))
+    b = MeshBatch(list(zip(w.verts, w.faces)))
+    items = [(i, torch.from_numpy(q).to(cuda)) for i, q in enumerate(w.queries)]
+    for wg in (64, 256):
+        plan = b.plan(items, capture=True, workgroup=wg)
+        plan.launch()
+        torch.cuda.synchronize()
+        assert plan.last_traversal_ms() >= 0.0
+        assert plan.workgroup == wg
+    torch.empty(16, device=cuda).fill_(1.0)  # no stale HIP error reported here
+    torch.cuda.synchronize()
