@@ -71,6 +71,7 @@ SIGNATURES = [
     ("ebg_query_sdf_host", C.c_int, [_vp, _vp, _u64, _vp, _vp, C.POINTER(QueryOpts)]),
     ("ebg_mesh_last_traversal_ms", C.c_double, [_vp]),
     ("ebg_mesh_last_split_packets", C.c_int64, [_vp]),
+    ("ebg_batch_plan_last_early_packets", C.c_int64, [_vp]),
     ("ebg_morton_codes_f32", C.c_int, [_vp, _u64, _vp, _vp]),
     ("ebg_morton_codes_f64", C.c_int, [_vp, _u64, _vp, _vp]),
     ("ebg_morton_sort", C.c_int, [_vp, _u64, _vp, _vp, _vp]),

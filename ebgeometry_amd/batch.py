@@ -140,3 +140,7 @@ class BatchPlan:
 
     def last_traversal_ms(self) -> float:
         return float(lib().ebg_batch_plan_last_traversal_ms(self._h))
+
+    def last_early_packets(self) -> int:
+        """Packets of the last launch that were started ahead of the table order (waits)."""
+        return int(lib().ebg_batch_plan_last_early_packets(self._h))

@@ -48,7 +48,6 @@ void set_last_error(const std::string& s) { g_last_error = s; }
 // Measured on the 327,680-face displaced sphere (K=4, leaf 16): kHeavyFactor 6 above 2^22
 // queries (the launch is full, and splitting costs extra work), 3 below (spare capacity); more
 // than 16 subtree roots only adds work.
-constexpr uint32_t kCostGrid    = 32;
 constexpr uint64_t kHeavyMinQ   = 1u << 14;
 constexpr uint32_t kSplitBlocks = 2048; // workgroups that stride over the split items
 constexpr std::size_t kSplitRoots = 16;
@@ -279,10 +278,13 @@ ebg_status ebg_mesh_get_info(const ebg_mesh* m, ebg_mesh_info* info)
 }
 
 
-static ebg_status ensure_cost_grid(ebg_mesh* m, hipStream_t s)
+} // extern "C"
+
+namespace ebgpu {
+ebg_status ensure_cost_grid(ebg_mesh* m, hipStream_t s)
 {
   if (m->grid_ready) return EBG_OK;
-  const uint32_t g  = kCostGrid;
+  const uint32_t g  = ebgpu::kCostGrid;
   const uint64_t ng = uint64_t(g) * g * g; // cells = packets of the measuring launch
   double         lo[3], step[3];
   for (int c = 0; c < 3; c++) {
@@ -321,6 +323,10 @@ static ebg_status ensure_cost_grid(ebg_mesh* m, hipStream_t s)
   m->grid_ready     = true;
   return EBG_OK;
 }
+} // namespace ebgpu
+using ebgpu::ensure_cost_grid;
+
+extern "C" {
 
 ebg_status ebg_query_sdf(const ebg_mesh* cm, const float* d_pts, uint64_t n, float* d_dist, uint32_t* d_tri,
                          const ebg_query_opts* opts)
@@ -386,7 +392,7 @@ ebg_status ebg_query_sdf(const ebg_mesh* cm, const float* d_pts, uint64_t n, flo
       h.lo[c]    = m->grid_lo[c];
       h.scale[c] = m->grid_scale[c];
     }
-    h.g         = kCostGrid;
+    h.g         = ebgpu::kCostGrid;
     h.threshold = uint32_t(std::max(1.0, heavy_factor(n) * m->grid_mean));
     h.cap       = ebgpu::Workspace::heavy_cap(n);
     h.list      = ws.heavy_list();

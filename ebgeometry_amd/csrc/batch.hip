@@ -57,7 +57,21 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(K == 4 ? 8 
   // packet table, so the meshes in flight at any time are few and stay in the Infinity Cache;
   // with a contiguous eighth of the table per XCD, eight stretches of meshes were in flight and
   // the launch took 2.5x as long (163 against 66 ms on the 256-mesh workload).
-  const uint32_t p   = uni(blockIdx.x * wpb + (threadIdx.x >> 6));
+  uint32_t w = uni(blockIdx.x * wpb + (threadIdx.x >> 6));
+  uint32_t p;
+  if (a.early_blocks) {
+    // the early packets first (they run longest), then the table in order without them
+    const uint32_t ew = a.early_blocks * wpb;
+    if (w < ew) {
+      if (w >= min(*a.early_count, a.early_cap)) return; // whole wavefront
+      p = uni(a.early_list[w]);
+    }
+    else {
+      p = w - ew;
+      if (p >= a.num_packets || a.flag[p]) return; // whole wavefront
+    }
+  }
+  else p = w;
   if (p >= a.num_packets) return; // whole wavefront (p is wave-uniform)
   const SWords<4>    pk = sload<4>(a.packets + 4 * (size_t)p);
   const uint32_t     item = pk.w[0], first = pk.w[1];
@@ -150,11 +164,45 @@ __global__ __launch_bounds__(64) void batch64_kernel(const BatchArgs a)
   for (uint32_t e = blockIdx.x; e < count; e += gridDim.x) batch64_entry<K, NB>(a, e);
 }
 
+// One wavefront per packet: the largest cost-grid value over its queries' cells in its mesh's
+// grid decides whether it is listed as early.
+__global__ __launch_bounds__(256) void batch_mark_kernel(const BatchArgs a)
+{
+  const uint32_t p = uni(blockIdx.x * 4u + (threadIdx.x >> 6));
+  if (p >= a.num_packets) return; // whole wavefront
+  const SWords<4>    pk   = sload<4>(a.packets + 4 * (size_t)p);
+  const uint32_t     item = pk.w[0], first = pk.w[1];
+  const BatchItemDev it   = sload_record(a.items + item);
+  const GridDev      gd   = sload_record(a.grids + it.mesh);
+  const uint32_t     lane = __lane_id();
+  const uint32_t     j    = first + lane;
+  uint32_t           c    = 0;
+  if (gd.threshold != kInvalid && j < it.n) {
+    const uint32_t qi = a.sorted ? (uint32_t)(a.sorted[it.base + j] - it.base) : j;
+    uint32_t       cell[3];
+#pragma unroll
+    for (int k = 0; k < 3; k++) {
+      const float x = (it.pts[3 * (size_t)qi + k] - gd.lo[k]) * gd.scale[k];
+      cell[k]       = (uint32_t)fminf(fmaxf(x, 0.0f), float(gd.g - 1)); // NaN -> 0
+    }
+    c = gd.cost[((size_t)cell[2] * gd.g + cell[1]) * gd.g + cell[0]];
+  }
+  const uint32_t est  = wave_max_u32(c);
+  uint32_t       slot = kInvalid;
+  if (gd.threshold != kInvalid && est >= gd.threshold && lane == 0) slot = atomicAdd(a.early_count, 1u);
+  slot             = rdlane(slot, 0);
+  const bool early = slot < a.early_cap;
+  if (lane == 0) {
+    if (early) a.early_list[slot] = p;
+    a.flag[p] = early ? 1 : 0;
+  }
+}
+
 template <int K, int NB>
 hipError_t launch32(const BatchArgs& a, int wg, bool count, hipStream_t s)
 {
   const uint32_t wpb    = (uint32_t)wg / 64u;
-  const uint32_t blocks = (a.num_packets + wpb - 1) / wpb;
+  const uint32_t blocks = (a.num_packets + wpb - 1) / wpb + a.early_blocks;
   if (blocks == 0) return hipSuccess;
   if (count) hipLaunchKernelGGL((batch32_kernel<K, NB, true>), dim3(blocks), dim3(wg), 0, s, a);
   else hipLaunchKernelGGL((batch32_kernel<K, NB, false>), dim3(blocks), dim3(wg), 0, s, a);
@@ -180,9 +228,18 @@ hipError_t launch_batch_keys(const BatchItemDev* items, uint32_t num_items, uint
   return hipGetLastError();
 }
 
+hipError_t launch_batch_mark(const BatchArgs& a, hipStream_t s)
+{
+  if (a.num_packets == 0) return hipSuccess;
+  if (!a.grids || !a.flag || !a.early_list || !a.early_count) return hipErrorInvalidValue;
+  hipLaunchKernelGGL(batch_mark_kernel, dim3((a.num_packets + 3) / 4), dim3(256), 0, s, a);
+  return hipGetLastError();
+}
+
 hipError_t launch_batch32(const BatchArgs& a, int K, int banks, int wg, bool count, hipStream_t s)
 {
   if (wg != 64 && wg != 128 && wg != 256) return hipErrorInvalidValue;
+  if (a.early_blocks && (!a.flag || !a.early_list || !a.early_count)) return hipErrorInvalidValue;
   if (K == 4) {
     if (banks <= 1) return launch32<4, 1>(a, wg, count, s);
     if (banks == 2) return launch32<4, 2>(a, wg, count, s);

@@ -39,6 +39,8 @@ struct ebg_batch_plan
   uint64_t        total = 0;
   uint32_t        num_items = 0, num_packets = 0;
   DevBuf          items, packets, keys, keys_sorted, sorted, scratch, recheck, counter;
+  DevBuf          grids, flags, early; // early packets: per-mesh grid records, packet flags, list
+  uint32_t        early_cap = 0;       // 0: no early packets
   int             wg = 256;
   bool            presort = true, fp64 = true;
   unsigned long long* work = nullptr;
@@ -183,7 +185,7 @@ ebg_status enqueue(ebg_batch_plan* p, hipStream_t s, bool events)
   a.recheck       = p->fp64 ? p->recheck.as<uint4>() : nullptr;
   a.recheck_count = p->counter.as<uint32_t>();
   a.work          = p->work;
-  if (p->fp64) EBG_HIP(ebgpu::launch_fill_u32(a.recheck_count, 1, 0u, s));
+  if (p->fp64 || p->early_cap) EBG_HIP(ebgpu::launch_fill_u32(a.recheck_count, 2, 0u, s)); // recheck, early
   if (p->presort && p->total > 1) {
     EBG_HIP(ebgpu::launch_batch_keys(a.items, p->num_items, p->total, p->keys.as<uint64_t>(), s));
     int item_bits = 1;
@@ -191,6 +193,15 @@ ebg_status enqueue(ebg_batch_plan* p, hipStream_t s, bool events)
     EBG_HIP(ebgpu::radix_sort_pairs(p->keys.as<uint64_t>(), nullptr, p->keys_sorted.as<uint64_t>(),
                                     p->sorted.as<uint32_t>(), p->total, 24 + item_bits, p->scratch.p, s));
     a.sorted = p->sorted.as<uint32_t>();
+  }
+  if (p->early_cap) {
+    a.grids        = p->grids.as<ebgpu::GridDev>();
+    a.flag         = p->flags.as<uint8_t>();
+    a.early_list   = p->early.as<uint32_t>();
+    a.early_count  = p->counter.as<uint32_t>() + 1;
+    a.early_cap    = p->early_cap;
+    a.early_blocks = (p->early_cap + uint32_t(p->wg / 64) - 1) / uint32_t(p->wg / 64);
+    EBG_HIP(ebgpu::launch_batch_mark(a, s));
   }
   if (events) EBG_HIP(hipEventRecord(p->ev0, s));
   EBG_HIP(ebgpu::launch_batch32(a, b->K, b->banks, p->wg, p->work != nullptr, s));
@@ -201,6 +212,11 @@ ebg_status enqueue(ebg_batch_plan* p, hipStream_t s, bool events)
 
 // Milliseconds between two recorded events, or -1; a failure is cleared from the runtime's
 // last-error state so that it does not surface in the caller's next HIP call.
+// Early packets of a plan: meshes from 16,384 faces get a cost grid; packets at >= 2x their
+// mesh's grid mean start first (256-mesh workload: 66.8 -> 55.4 ms; 1.5x: 56.3, 3x: 55.4).
+constexpr uint64_t kEarlyMinFaces = 16384;
+constexpr double   kEarlyFactor   = 2.0;
+
 double elapsed_ms(hipEvent_t a, hipEvent_t b)
 {
   float ms = -1.0f;
@@ -430,6 +446,44 @@ ebg_status ebg_batch_plan_create(ebg_batch* b, const ebg_batch_item* items, uint
     return bail(e, "events");
 
   hipStream_t s = static_cast<hipStream_t>(o.stream);
+  // Early packets: queries near a mesh's medial axis cost a packet tens of times the median, and
+  // one that starts late ends the launch. Meshes of at least kEarlyMinFaces get their packet
+  // cost grid (built once per mesh), and packets at >= kEarlyFactor x their mesh's grid mean
+  // start first.
+  if (p->num_packets > 0) {
+    std::vector<ebgpu::GridDev> gd(b->meshes.size());
+    std::vector<char>           used(b->meshes.size(), 0);
+    for (uint64_t i = 0; i < num_items; i++) used[items[i].mesh] = 1;
+    bool any = false;
+    for (std::size_t k = 0; k < b->meshes.size(); k++) {
+      ebgpu::GridDev& g = gd[k];
+      g.threshold       = ebgpu::kInvalid;
+      ebg_mesh* m       = b->meshes[k];
+      if (!used[k] || m->nt < kEarlyMinFaces) continue;
+      {
+        std::lock_guard<std::recursive_mutex> lock(m->mu);
+        const ebg_status gs = ebgpu::ensure_cost_grid(m, s);
+        if (gs != EBG_OK) {
+          delete p;
+          return gs;
+        }
+      }
+      g.cost = m->cost_grid.as<uint32_t>();
+      for (int c = 0; c < 3; c++) {
+        g.lo[c]    = m->grid_lo[c];
+        g.scale[c] = m->grid_scale[c];
+      }
+      g.g         = ebgpu::kCostGrid;
+      g.threshold = uint32_t(std::max(1.0, kEarlyFactor * m->grid_mean));
+      any         = true;
+    }
+    if (any) {
+      p->early_cap = p->num_packets / 8 + 64;
+      if ((e = upload(p->grids, gd)) != hipSuccess) return bail(e, "grids");
+      if ((e = p->flags.reserve(p->num_packets)) != hipSuccess) return bail(e, "flags");
+      if ((e = p->early.reserve(4 * (std::size_t)p->early_cap)) != hipSuccess) return bail(e, "early list");
+    }
+  }
 
   if (!workgroup && p->num_packets > 0) {
     // pick the workgroup shape by timing the traversal on the items as they are
@@ -500,6 +554,16 @@ ebg_status ebg_batch_plan_destroy(ebg_batch_plan* p)
 }
 
 int32_t ebg_batch_plan_workgroup(const ebg_batch_plan* p) { return p ? p->wg : 0; }
+
+int64_t ebg_batch_plan_last_early_packets(const ebg_batch_plan* p)
+{
+  if (!p) return -1;
+  if (!p->early_cap || !p->timed) return 0;
+  uint32_t c = 0;
+  if (hipDeviceSynchronize() != hipSuccess) return -1;
+  if (hipMemcpy(&c, p->counter.as<uint32_t>() + 1, 4, hipMemcpyDeviceToHost) != hipSuccess) return -1;
+  return std::min<int64_t>(c, p->early_cap);
+}
 
 double ebg_batch_plan_last_traversal_ms(const ebg_batch_plan* p)
 {

@@ -173,6 +173,10 @@ struct ebg_mesh
 };
 
 namespace ebgpu {
+constexpr uint32_t kCostGrid = 32; // cells per axis of a mesh's packet cost grid
+// Build the mesh's packet cost grid once (a counting launch of 64 lattice queries per cell on
+// stream s, then a wait); defined with the query pipeline (api.cpp).
+ebg_status ensure_cost_grid(ebg_mesh* m, hipStream_t s);
 inline void mesh_retain(ebg_mesh* m) { m->refs.fetch_add(1); }
 inline void mesh_release(ebg_mesh* m)
 {

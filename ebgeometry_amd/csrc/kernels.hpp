@@ -99,6 +99,15 @@ struct alignas(16) BatchItemDev
 };
 static_assert(sizeof(BatchItemDev) == 80, "BatchItemDev layout");
 
+// A mesh's packet cost grid as the batch marking reads it (threshold 0xFFFFFFFF: no grid).
+struct GridDev
+{
+  const uint32_t* cost; // g^3 cells, x fastest
+  float           lo[3], scale[3];
+  uint32_t        g, threshold, pad[2];
+};
+static_assert(sizeof(GridDev) == 48, "GridDev: whole 16-byte units (scalar loads)");
+
 struct BatchArgs
 {
   const BatchItemDev* items;
@@ -109,7 +118,17 @@ struct BatchArgs
   uint4*              recheck;     // per listed packet: packet, 0, lane mask lo, hi
   uint32_t*           recheck_count;
   unsigned long long* work;        // [0] inner, [1] triangles, [4]/[5] per wavefront (null = off)
+  // Early packets (early_blocks = 0: none): batch_mark lists the packets whose most expensive
+  // query (the cost-grid value of its cell in its mesh's grid) reaches the mesh's threshold;
+  // the first early_blocks workgroups of batch32 take them, the rest skip flagged packets.
+  const GridDev*      grids;       // per mesh
+  uint8_t*            flag;        // per packet
+  uint32_t*           early_list;
+  uint32_t*           early_count;
+  uint32_t            early_cap, early_blocks;
 };
+
+hipError_t launch_batch_mark(const BatchArgs& a, hipStream_t s);
 
 hipError_t launch_batch_keys(const BatchItemDev* items, uint32_t num_items, uint64_t total, uint64_t* keys,
                              hipStream_t s);

@@ -279,6 +279,11 @@ ebg_status ebg_batch_plan_destroy(ebg_batch_plan* plan);
  * recorded inside a graph are not timed. */
 int32_t    ebg_batch_plan_workgroup(const ebg_batch_plan* plan);
 double     ebg_batch_plan_last_traversal_ms(const ebg_batch_plan* plan);
+/* Early packets of the last launch: packets of meshes with at least 16,384 faces whose
+ * estimated cost (the mesh's packet cost grid, built once per mesh at plan creation) is >= 2x
+ * the mesh's grid mean are started first. Diagnostic with no counterpart in the reference;
+ * waits for the device; 0 if none, <0 on error. */
+int64_t    ebg_batch_plan_last_early_packets(const ebg_batch_plan* plan);
 
 /* One-shot convenience: plan (no tuning, no graph), launch on opts->stream, release. */
 ebg_status ebg_batch_query(ebg_batch* batch, const ebg_batch_item* items, uint64_t num_items,

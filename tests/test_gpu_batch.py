@@ -163,3 +163,35 @@ def test_plan_with_fixed_workgroup_times_and_leaves_no_error(cuda):
         assert plan.workgroup == wg
     torch.empty(16, device=cuda).fill_(1.0)  # no stale HIP error reported here
     torch.cuda.synchronize()
+
+
+def test_early_packets_in_a_batch(cuda):
+    """Meshes of >= 16,384 faces get a cost grid; packets of queries deep inside them start
+    first. The results must equal the single-mesh queries and the oracle."""
+    import torch
+    rng = np.random.default_rng(11)
+    meshes, queries = [], []
+    for k in range(3):
+        v, f = wl.displaced_icosphere(5, 40 + k)  # 20,480 faces
+        meshes.append((v, f))
+        d = rng.normal(size=(6000, 3))
+        d /= np.linalg.norm(d, axis=1, keepdims=True)
+        deep = (d * (0.3 * rng.random(6000) ** (1 / 3))[:, None]).astype(np.float32)
+        queries.append(np.concatenate([deep, wl.mixed_queries(6000, v, f, 50 + k)]))
+    b = MeshBatch(meshes, branching=4, leaf_size=16)
+    items = [(k, torch.from_numpy(q).to(cuda)) for k, q in enumerate(queries)]
+    plan = b.plan(items, capture=True)
+    outs = plan.launch()
+    torch.cuda.synchronize()
+    assert plan.last_early_packets() > 0
+    for k, (v, f) in enumerate(meshes):
+        diag = diag_of(v)
+        single = TriMeshSDF(v, f, branching=4, leaf_size=16)
+        d1 = single.query(items[k][1]).cpu().numpy()
+        d0 = outs[k].cpu().numpy()
+        assert np.abs(d0 - d1).max() <= 1e-6 * diag, k
+        assert np.array_equal(np.sign(d0), np.sign(d1)), k
+        sub = np.arange(0, queries[k].shape[0], 20)
+        ref, _ = port.Mesh(v.astype(np.float64), f).sdf(queries[k][sub].astype(np.float64))
+        assert np.abs(d0[sub] - ref).max() <= D_TOL * diag, k
+        assert np.array_equal(np.sign(d0[sub]), np.sign(ref)), k
