@@ -489,17 +489,20 @@ __device__ __forceinline__ void traverse_from(const MeshDev& m, const Entry& sta
             const R    sx  = px - R(__uint_as_float(sp.w[8 * i])), sy = py - R(__uint_as_float(sp.w[8 * i + 1])),
                        sz  = pz - R(__uint_as_float(sp.w[8 * i + 2]));
             const R    lim = rb + R(__uint_as_float(sp.w[8 * i + 3]));
-            bool       tn  = sx * sx + sy * sy + sz * sz <= lim * lim;
+            const bool in_s = sx * sx + sy * sy + sz * sz <= lim * lim;
+            bool       in_p = true;
             if constexpr (sizeof(R) == 4) {
               // the FP32 plane (rounded normal and offset) may misjudge by a few ulps of the
               // coordinates; the FP64 pass, whose seed only just contains the minimum, skips it
               const float pd = fmaf(__uint_as_float(sp.w[8 * i + 4]), px,
                                     fmaf(__uint_as_float(sp.w[8 * i + 5]), py,
                                          fmaf(__uint_as_float(sp.w[8 * i + 6]), pz, __uint_as_float(sp.w[8 * i + 7]))));
-              tn = tn && fabsf(pd) <= rb;
+              in_p = fabsf(pd) <= rb;
             }
-            if (COUNT) n_tris += tn ? 1u : 0u;
-            need |= ballot(tn) ? (1u << i) : 0u;
+            if (COUNT) n_tris += (in_s && in_p) ? 1u : 0u;
+            // two ballots straight from the compares (a ballot of their conjunction went
+            // through a VGPR)
+            need |= (ballot(in_s) & ballot(in_p)) ? (1u << i) : 0u;
           }
         }
         while (need) {
