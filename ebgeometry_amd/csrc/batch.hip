@@ -28,8 +28,9 @@ using namespace dev;
 // Sort key of flat query g: (item << 24) | 24-bit Hilbert index of the query on a 256^3 grid of
 // the item's box (the mesh box grown by 25%; 8 bits per axis order 64-query packets well at any
 // item size). Item found by binary search over the item bases.
+template <class KeyT>
 __global__ __launch_bounds__(256) void batch_keys_kernel(const BatchItemDev* __restrict__ items, uint32_t num_items,
-                                                         uint64_t total, uint64_t* __restrict__ keys)
+                                                         uint64_t total, KeyT* __restrict__ keys)
 {
   const uint64_t g = (uint64_t)blockIdx.x * 256u + threadIdx.x;
   if (g >= total) return;
@@ -45,7 +46,7 @@ __global__ __launch_bounds__(256) void batch_keys_kernel(const BatchItemDev* __r
 #pragma unroll
   for (int c = 0; c < 3; c++)
     x[c] = (uint32_t)fminf(fmaxf((it.pts[3 * j + c] - it.box_lo[c]) * it.box_inv[c], 0.0f), 255.0f); // NaN -> 0
-  keys[g] = ((uint64_t)lo << 24) | hilbert_index<8>(x[0], x[1], x[2]);
+  keys[g] = KeyT(((uint64_t)lo << 24) | hilbert_index<8>(x[0], x[1], x[2]));
 }
 
 template <int K, int NB, bool COUNT>
@@ -219,12 +220,18 @@ hipError_t launch64(const BatchArgs& a, hipStream_t s)
 
 } // namespace
 
-hipError_t launch_batch_keys(const BatchItemDev* items, uint32_t num_items, uint64_t total, uint64_t* keys,
+hipError_t launch_batch_keys(const BatchItemDev* items, uint32_t num_items, uint64_t total, void* keys, bool narrow,
                              hipStream_t s)
 {
   if (total == 0) return hipSuccess;
-  hipLaunchKernelGGL(batch_keys_kernel, dim3((uint32_t)((total + 255) / 256)), dim3(256), 0, s, items, num_items,
-                     total, keys);
+  if (narrow && num_items > 256) return hipErrorInvalidValue;
+  const dim3 grid((uint32_t)((total + 255) / 256));
+  if (narrow)
+    hipLaunchKernelGGL((batch_keys_kernel<uint32_t>), grid, dim3(256), 0, s, items, num_items, total,
+                       static_cast<uint32_t*>(keys));
+  else
+    hipLaunchKernelGGL((batch_keys_kernel<uint64_t>), grid, dim3(256), 0, s, items, num_items, total,
+                       static_cast<uint64_t*>(keys));
   return hipGetLastError();
 }
 

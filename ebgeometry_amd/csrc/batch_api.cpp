@@ -187,11 +187,16 @@ ebg_status enqueue(ebg_batch_plan* p, hipStream_t s, bool events)
   a.work          = p->work;
   if (p->fp64 || p->early_cap) EBG_HIP(ebgpu::launch_fill_u32(a.recheck_count, 2, 0u, s)); // recheck, early
   if (p->presort && p->total > 1) {
-    EBG_HIP(ebgpu::launch_batch_keys(a.items, p->num_items, p->total, p->keys.as<uint64_t>(), s));
     int item_bits = 1;
     while ((1ull << item_bits) < p->num_items) item_bits++;
-    EBG_HIP(ebgpu::radix_sort_pairs(p->keys.as<uint64_t>(), nullptr, p->keys_sorted.as<uint64_t>(),
-                                    p->sorted.as<uint32_t>(), p->total, 24 + item_bits, p->scratch.p, s));
+    const bool narrow = 24 + item_bits <= 32; // 32-bit keys: half the key traffic per pass
+    EBG_HIP(ebgpu::launch_batch_keys(a.items, p->num_items, p->total, p->keys.p, narrow, s));
+    if (narrow)
+      EBG_HIP(ebgpu::radix_sort_pairs32(p->keys.as<uint32_t>(), nullptr, p->keys_sorted.as<uint32_t>(),
+                                        p->sorted.as<uint32_t>(), p->total, 24 + item_bits, p->scratch.p, false, s));
+    else
+      EBG_HIP(ebgpu::radix_sort_pairs(p->keys.as<uint64_t>(), nullptr, p->keys_sorted.as<uint64_t>(),
+                                      p->sorted.as<uint32_t>(), p->total, 24 + item_bits, p->scratch.p, s));
     a.sorted = p->sorted.as<uint32_t>();
   }
   if (p->early_cap) {

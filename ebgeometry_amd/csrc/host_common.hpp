@@ -119,10 +119,11 @@ struct Workspace
   {
     hipError_t e;
     if ((e = launch_bbox_f32(d_pts, n, box.as<float>(), box.as<float>() + 8, s)) != hipSuccess) return e;
-    // 30-bit Hilbert keys on a 1024^3 grid: 4 radix passes
-    if ((e = launch_hilbert_f32(d_pts, n, box.as<float>(), codes.as<uint64_t>(), s)) != hipSuccess) return e;
-    return radix_sort_pairs(codes.as<uint64_t>(), nullptr, codes_sorted.as<uint64_t>(), perm.as<uint32_t>(), n,
-                            30, sort_scratch.p, s);
+    // 30-bit Hilbert keys on a 1024^3 grid, sorted as 32-bit keys in 4 radix passes; only the
+    // permutation is kept
+    if ((e = launch_hilbert_f32(d_pts, n, box.as<float>(), codes.as<uint32_t>(), s)) != hipSuccess) return e;
+    return radix_sort_pairs32(codes.as<uint32_t>(), nullptr, codes_sorted.as<uint32_t>(), perm.as<uint32_t>(), n, 30,
+                              sort_scratch.p, false, s);
   }
 };
 

@@ -130,7 +130,8 @@ struct BatchArgs
 
 hipError_t launch_batch_mark(const BatchArgs& a, hipStream_t s);
 
-hipError_t launch_batch_keys(const BatchItemDev* items, uint32_t num_items, uint64_t total, uint64_t* keys,
+// keys: uint32 when narrow (at most 256 items: 8 item bits + 24 curve bits), else uint64
+hipError_t launch_batch_keys(const BatchItemDev* items, uint32_t num_items, uint64_t total, void* keys, bool narrow,
                              hipStream_t s);
 // wg = 64, 128 or 256 lanes per workgroup (1, 2 or 4 packets).
 hipError_t launch_batch32(const BatchArgs& a, int K, int banks, int wg, bool count, hipStream_t s);
@@ -151,7 +152,7 @@ hipError_t launch_bbox_f64(const double* pts, uint64_t n, double* d_box6, void* 
 hipError_t launch_codes_f32(const float* pts, uint64_t n, const float* d_box6, uint64_t* codes, hipStream_t s,
                             int shift = 0);
 // keys[i] = 30-bit Hilbert index of the point on a 1024^3 grid of d_box6 (query presort)
-hipError_t launch_hilbert_f32(const float* pts, uint64_t n, const float* d_box6, uint64_t* keys, hipStream_t s);
+hipError_t launch_hilbert_f32(const float* pts, uint64_t n, const float* d_box6, uint32_t* keys, hipStream_t s);
 hipError_t launch_codes_f64(const double* pts, uint64_t n, const double* d_box6, uint64_t* codes, hipStream_t s);
 // Bytes of scratch radix_sort needs for n keys.
 std::size_t radix_sort_scratch_bytes(uint64_t n);
@@ -159,5 +160,10 @@ std::size_t radix_sort_scratch_bytes(uint64_t n);
 // keys_out / values_out. 'bits' low bits of the keys are significant (<= 64).
 hipError_t radix_sort_pairs(const uint64_t* keys_in, const uint32_t* values_in, uint64_t* keys_out,
                             uint32_t* values_out, uint64_t n, int bits, void* scratch, hipStream_t s);
+// The same for 32-bit keys (bits <= 32); with want_keys false the sorted keys are not written
+// (keys_out still serves as a ping-pong buffer).
+hipError_t radix_sort_pairs32(const uint32_t* keys_in, const uint32_t* values_in, uint32_t* keys_out,
+                              uint32_t* values_out, uint64_t n, int bits, void* scratch, bool want_keys,
+                              hipStream_t s);
 
 } // namespace ebgpu

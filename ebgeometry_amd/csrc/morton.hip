@@ -64,14 +64,30 @@ __global__ __launch_bounds__(256) void bbox_partial(const T* __restrict__ p, uin
   if (threadIdx.x < 6) part[6 * blockIdx.x + threadIdx.x] = sh[threadIdx.x][0];
 }
 
+// One block of 256 threads: strided partial minima / maxima, then a tree in LDS (min and max
+// are exact, so the result does not depend on the order).
 template <class T>
 __global__ __launch_bounds__(256) void bbox_final(const T* __restrict__ part, int nparts, T* __restrict__ box)
 {
-  if (threadIdx.x >= 6) return;
-  const int c = threadIdx.x;
-  T         v = c < 3 ? T(INFINITY) : T(-INFINITY);
-  for (int i = 0; i < nparts; i++) v = c < 3 ? tmin(v, part[6 * i + c]) : tmax(v, part[6 * i + c]);
-  box[c] = v;
+  __shared__ T sh[6][256];
+  T            v[6];
+#pragma unroll
+  for (int c = 0; c < 6; c++) v[c] = c < 3 ? T(INFINITY) : T(-INFINITY);
+  for (int i = threadIdx.x; i < nparts; i += 256)
+#pragma unroll
+    for (int c = 0; c < 6; c++) v[c] = c < 3 ? tmin(v[c], part[6 * i + c]) : tmax(v[c], part[6 * i + c]);
+#pragma unroll
+  for (int c = 0; c < 6; c++) sh[c][threadIdx.x] = v[c];
+  __syncthreads();
+  for (int o = 128; o > 0; o >>= 1) {
+    if (threadIdx.x < (unsigned)o)
+#pragma unroll
+      for (int c = 0; c < 6; c++)
+        sh[c][threadIdx.x] = c < 3 ? tmin(sh[c][threadIdx.x], sh[c][threadIdx.x + o])
+                                   : tmax(sh[c][threadIdx.x], sh[c][threadIdx.x + o]);
+    __syncthreads();
+  }
+  if (threadIdx.x < 6) box[threadIdx.x] = sh[threadIdx.x][0];
 }
 
 template <class T>
@@ -79,7 +95,7 @@ hipError_t bbox_impl(const T* pts, uint64_t n, T* box, T* part, hipStream_t s)
 {
   const int blocks = (int)std::min<uint64_t>(kBboxParts, std::max<uint64_t>(1, (n + 255) / 256));
   hipLaunchKernelGGL(bbox_partial<T>, dim3(blocks), dim3(256), 0, s, pts, n, part);
-  hipLaunchKernelGGL(bbox_final<T>, dim3(1), dim3(64), 0, s, part, blocks, box);
+  hipLaunchKernelGGL(bbox_final<T>, dim3(1), dim3(256), 0, s, part, blocks, box);
   return hipGetLastError();
 }
 
@@ -109,14 +125,14 @@ __global__ __launch_bounds__(256) void codes_f32(const float* __restrict__ p, ui
 // across the box the way a Morton run can at a high-level boundary (C2: 10% fewer triangle
 // evaluations per wavefront).
 __global__ __launch_bounds__(256) void hilbert_f32(const float* __restrict__ p, uint64_t n,
-                                                   const float* __restrict__ box, uint64_t* __restrict__ keys)
+                                                   const float* __restrict__ box, uint32_t* __restrict__ keys)
 {
   const uint64_t i = (uint64_t)blockIdx.x * 256 + threadIdx.x;
   if (i >= n) return;
   uint32_t x[3];
 #pragma unroll
   for (int c = 0; c < 3; c++) x[c] = dev::hilbert_bin<10>(p[3 * i + c], box[c], box[3 + c] - box[c]);
-  keys[i] = dev::hilbert_index<10>(x[0], x[1], x[2]);
+  keys[i] = (uint32_t)dev::hilbert_index<10>(x[0], x[1], x[2]);
 }
 
 __global__ __launch_bounds__(256) void codes_f64(const double* __restrict__ p, uint64_t n, const double* __restrict__ box,
@@ -144,7 +160,8 @@ constexpr int kSortThreads = 256;
 constexpr int kSortItems   = 16;
 constexpr int kTile        = kSortThreads * kSortItems; // 4096 keys per workgroup
 
-__global__ __launch_bounds__(kSortThreads) void radix_hist(const uint64_t* __restrict__ keys, uint64_t n, int shift,
+template <class KeyT>
+__global__ __launch_bounds__(kSortThreads) void radix_hist(const KeyT* __restrict__ keys, uint64_t n, int shift,
                                                            uint32_t* __restrict__ hist, uint32_t nblocks)
 {
   __shared__ uint32_t h[256];
@@ -231,11 +248,13 @@ __global__ __launch_bounds__(256) void scan_apply(uint32_t* __restrict__ a, uint
 }
 
 // Stable scatter: keys of the tile are ranked in index order, 256 at a time; within a
-// wavefront the rank among equal digits comes from 8 ballots (one per digit bit).
-__global__ __launch_bounds__(kSortThreads) void radix_scatter(const uint64_t* __restrict__ kin,
+// wavefront the rank among equal digits comes from 8 ballots (one per digit bit). kout may be
+// null (last pass of a sort whose caller only wants the permutation).
+template <class KeyT>
+__global__ __launch_bounds__(kSortThreads) void radix_scatter(const KeyT* __restrict__ kin,
                                                               const uint32_t* __restrict__ vin, uint64_t n, int shift,
                                                               const uint32_t* __restrict__ offs, uint32_t nblocks,
-                                                              uint64_t* __restrict__ kout, uint32_t* __restrict__ vout)
+                                                              KeyT* __restrict__ kout, uint32_t* __restrict__ vout)
 {
   constexpr int       W = kSortThreads / 64;
   __shared__ uint32_t base[256];
@@ -249,7 +268,7 @@ __global__ __launch_bounds__(kSortThreads) void radix_scatter(const uint64_t* __
   for (int r = 0; r < kSortItems; r++) {
     const uint64_t i     = tile + (uint64_t)r * kSortThreads + threadIdx.x;
     const bool     valid = i < n;
-    uint64_t       key   = valid ? kin[i] : 0ull;
+    const KeyT     key   = valid ? kin[i] : KeyT(0);
     uint32_t       val   = valid ? (vin ? vin[i] : (uint32_t)i) : 0u;
     const uint32_t d     = (uint32_t)(key >> shift) & 255u;
     uint64_t       peers = __ballot(valid);
@@ -266,8 +285,8 @@ __global__ __launch_bounds__(kSortThreads) void radix_scatter(const uint64_t* __
       uint32_t pre = 0;
       for (int w = 0; w < wave; w++) pre += wcnt[w][d];
       const uint32_t pos = base[d] + pre + rank;
-      kout[pos]          = key;
-      vout[pos]          = val;
+      if (kout) kout[pos] = key;
+      vout[pos] = val;
     }
     __syncthreads();
     uint32_t tot = 0;
@@ -298,7 +317,7 @@ hipError_t launch_codes_f32(const float* pts, uint64_t n, const float* box, uint
   hipLaunchKernelGGL(codes_f32, dim3((uint32_t)((n + 255) / 256)), dim3(256), 0, s, pts, n, box, codes, shift);
   return hipGetLastError();
 }
-hipError_t launch_hilbert_f32(const float* pts, uint64_t n, const float* box, uint64_t* keys, hipStream_t s)
+hipError_t launch_hilbert_f32(const float* pts, uint64_t n, const float* box, uint32_t* keys, hipStream_t s)
 {
   if (n == 0) return hipSuccess;
   hipLaunchKernelGGL(hilbert_f32, dim3((uint32_t)((n + 255) / 256)), dim3(256), 0, s, pts, n, box, keys);
@@ -321,16 +340,20 @@ std::size_t radix_sort_scratch_bytes(uint64_t n)
   return round_up(n * 8, 256) + round_up(n * 4, 256) + round_up(m * 4, 256) + round_up(nsum * 4, 256) + 256;
 }
 
-hipError_t radix_sort_pairs(const uint64_t* keys_in, const uint32_t* values_in, uint64_t* keys_out,
-                            uint32_t* values_out, uint64_t n, int bits, void* scratch, hipStream_t s)
+namespace {
+// keys_out is also the ping-pong partner of the scratch keys; with want_keys false the last
+// pass writes only the values (the permutation).
+template <class KeyT>
+hipError_t radix_sort_impl(const KeyT* keys_in, const uint32_t* values_in, KeyT* keys_out, uint32_t* values_out,
+                           uint64_t n, int bits, void* scratch, bool want_keys, hipStream_t s)
 {
   if (n == 0) return hipSuccess;
-  if (n > 0xFFFFFFFFull) return hipErrorInvalidValue;
+  if (n > 0xFFFFFFFFull || bits < 1 || bits > 8 * (int)sizeof(KeyT)) return hipErrorInvalidValue;
   const uint32_t nb    = (uint32_t)((n + kTile - 1) / kTile);
   const uint64_t m     = 256ull * nb;
   const uint32_t nsum  = (uint32_t)((m + kScanChunk - 1) / kScanChunk);
   char*          p     = static_cast<char*>(scratch);
-  uint64_t*      ktmp  = reinterpret_cast<uint64_t*>(p);
+  KeyT*          ktmp  = reinterpret_cast<KeyT*>(p);
   p += round_up(n * 8, 256);
   uint32_t* vtmp = reinterpret_cast<uint32_t*>(p);
   p += round_up(n * 4, 256);
@@ -339,23 +362,40 @@ hipError_t radix_sort_pairs(const uint64_t* keys_in, const uint32_t* values_in, 
   uint32_t* sums = reinterpret_cast<uint32_t*>(p);
 
   const int passes = (bits + 7) / 8;
-  const uint64_t* ksrc = keys_in;
+  const KeyT*     ksrc = keys_in;
   const uint32_t* vsrc = values_in;
   for (int pass = 0; pass < passes; pass++) {
     // the last pass must land in the output buffers
     const bool to_out = ((passes - 1 - pass) % 2) == 0;
-    uint64_t*  kdst   = to_out ? keys_out : ktmp;
+    KeyT*      kdst   = to_out ? (pass == passes - 1 && !want_keys ? nullptr : keys_out) : ktmp;
     uint32_t*  vdst   = to_out ? values_out : vtmp;
     const int  shift  = 8 * pass;
-    hipLaunchKernelGGL(radix_hist, dim3(nb), dim3(kSortThreads), 0, s, ksrc, n, shift, hist, nb);
+    hipLaunchKernelGGL((radix_hist<KeyT>), dim3(nb), dim3(kSortThreads), 0, s, ksrc, n, shift, hist, nb);
     hipLaunchKernelGGL(scan_reduce, dim3(nsum), dim3(256), 0, s, hist, m, sums);
     hipLaunchKernelGGL(scan_sums, dim3(1), dim3(1024), 0, s, sums, nsum);
     hipLaunchKernelGGL(scan_apply, dim3(nsum), dim3(256), 0, s, hist, m, sums);
-    hipLaunchKernelGGL(radix_scatter, dim3(nb), dim3(kSortThreads), 0, s, ksrc, vsrc, n, shift, hist, nb, kdst, vdst);
+    hipLaunchKernelGGL((radix_scatter<KeyT>), dim3(nb), dim3(kSortThreads), 0, s, ksrc, vsrc, n, shift, hist, nb,
+                       kdst, vdst);
     ksrc = kdst;
     vsrc = vdst;
   }
   return hipGetLastError();
+}
+} // namespace
+
+hipError_t radix_sort_pairs(const uint64_t* keys_in, const uint32_t* values_in, uint64_t* keys_out,
+                            uint32_t* values_out, uint64_t n, int bits, void* scratch, hipStream_t s)
+{
+  if (!keys_out) return hipErrorInvalidValue;
+  return radix_sort_impl<uint64_t>(keys_in, values_in, keys_out, values_out, n, bits, scratch, true, s);
+}
+
+hipError_t radix_sort_pairs32(const uint32_t* keys_in, const uint32_t* values_in, uint32_t* keys_out,
+                              uint32_t* values_out, uint64_t n, int bits, void* scratch, bool want_keys,
+                              hipStream_t s)
+{
+  if (!keys_out) return hipErrorInvalidValue;
+  return radix_sort_impl<uint32_t>(keys_in, values_in, keys_out, values_out, n, bits, scratch, want_keys, s);
 }
 
 } // namespace ebgpu
