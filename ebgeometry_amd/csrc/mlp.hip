@@ -28,6 +28,19 @@ typedef uint16_t u16x8 __attribute__((ext_vector_type(8)));
 // FP32 -> BF16, round to nearest even (v_cvt_pk_bf16_f32 on gfx950)
 __device__ __forceinline__ uint16_t f2bf(float x) { return __builtin_bit_cast(uint16_t, (__bf16)x); }
 __device__ __forceinline__ float bf2f(uint16_t h) { return __uint_as_float((uint32_t)h << 16); }
+
+// relu(a), relu(b) as two BF16 in one word: one v_cvt_pk_bf16_f32 and one packed 16-bit integer
+// max with 0 (a BF16 bit pattern is negative as an int16 exactly when the value is, -0 included)
+typedef float  f32x2 __attribute__((ext_vector_type(2)));
+typedef __bf16 bf16x2 __attribute__((ext_vector_type(2)));
+typedef short  i16x2 __attribute__((ext_vector_type(2)));
+typedef unsigned int u32x4v __attribute__((ext_vector_type(4)));
+__device__ __forceinline__ uint32_t relu_pk_bf16(float a, float b)
+{
+  const bf16x2 h = __builtin_convertvector((f32x2){a, b}, bf16x2);
+  const i16x2  r = __builtin_elementwise_max(__builtin_bit_cast(i16x2, h), (i16x2){0, 0});
+  return __builtin_bit_cast(uint32_t, r);
+}
 __device__ __forceinline__ f32x4 mfma(u16x8 a, u16x8 b, f32x4 c)
 {
   return __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(bf16x8, a), __builtin_bit_cast(bf16x8, b), c, 0,
@@ -90,11 +103,17 @@ __device__ __forceinline__ void infer_tile(const uint16_t* __restrict__ sw, cons
       for (int e = 0; e < 8; e++) bq[q][b][e] = f2bf(feature(32 * b + 8 * g + e, x[0], x[1], x[2]));
   }
 
+  // each layer's accumulators start from its bias (output rows 16 t + 4 g .. + 3 of this lane)
   f32x4 acc[kInferQT][8];
+  {
+    const float* bias = p.master + kOffB1;
 #pragma unroll
-  for (int q = 0; q < kInferQT; q++)
+    for (int t = 0; t < 8; t++) {
+      const f32x4 bv = *reinterpret_cast<const f32x4*>(bias + 16 * t + 4 * g);
 #pragma unroll
-    for (int t = 0; t < 8; t++) acc[q][t] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
+      for (int q = 0; q < kInferQT; q++) acc[q][t] = bv;
+    }
+  }
   // layer 1 (K = 64)
 #pragma unroll
   for (int t = 0; t < 8; t++)
@@ -106,30 +125,27 @@ __device__ __forceinline__ void infer_tile(const uint16_t* __restrict__ sw, cons
     }
   // layers 2..4 (K = 128)
   const int wofs[3] = {kBfW2, kBfW3, kBfW4};
-  const int bofs[3] = {kOffB1, kOffB2, kOffB3};
+  const int bofs[3] = {kOffB2, kOffB3, kOffB4}; // bias of each layer's output
 #pragma unroll
   for (int L = 0; L < 3; L++) {
+    // the previous layer's output, relu'd and rounded to BF16, is this layer's B operand
+#pragma unroll
+    for (int b = 0; b < 4; b++)
+#pragma unroll
+      for (int q = 0; q < kInferQT; q++) {
+        const u32x4v w = {relu_pk_bf16(acc[q][2 * b][0], acc[q][2 * b][1]),
+                          relu_pk_bf16(acc[q][2 * b][2], acc[q][2 * b][3]),
+                          relu_pk_bf16(acc[q][2 * b + 1][0], acc[q][2 * b + 1][1]),
+                          relu_pk_bf16(acc[q][2 * b + 1][2], acc[q][2 * b + 1][3])};
+        bq[q][b] = __builtin_bit_cast(u16x8, w);
+      }
     const float* bias = p.master + bofs[L];
 #pragma unroll
-    for (int b = 0; b < 4; b++) {
-      float bl[8];
+    for (int t = 0; t < 8; t++) {
+      const f32x4 bv = *reinterpret_cast<const f32x4*>(bias + 16 * t + 4 * g);
 #pragma unroll
-      for (int e = 0; e < 4; e++) {
-        bl[e]     = bias[32 * b + 4 * g + e];
-        bl[e + 4] = bias[32 * b + 16 + 4 * g + e];
-      }
-#pragma unroll
-      for (int q = 0; q < kInferQT; q++)
-#pragma unroll
-        for (int e = 0; e < 8; e++) {
-          const float z = (e < 4 ? acc[q][2 * b][e] : acc[q][2 * b + 1][e - 4]) + bl[e];
-          bq[q][b][e]   = f2bf(fmaxf(z, 0.0f));
-        }
+      for (int q = 0; q < kInferQT; q++) acc[q][t] = bv;
     }
-#pragma unroll
-    for (int q = 0; q < kInferQT; q++)
-#pragma unroll
-      for (int t = 0; t < 8; t++) acc[q][t] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
 #pragma unroll
     for (int t = 0; t < 8; t++)
 #pragma unroll
@@ -139,8 +155,8 @@ __device__ __forceinline__ void infer_tile(const uint16_t* __restrict__ sw, cons
         for (int q = 0; q < kInferQT; q++) acc[q][t] = mfma(a, bq[q][b], acc[q][t]);
       }
   }
-  // output layer: y = relu(h4 + b4) . w5 + b5 (h4 rounded to bf16 as in training)
-  const float* b4 = p.master + kOffB4;
+  // output layer: y = relu(h4 + b4) . w5 + b5 (b4 already in the accumulators; h4 rounded to
+  // bf16 as in training)
   const float* w5 = p.master + kOffW5;
   const float  b5 = p.master[kOffB5];
 #pragma unroll
@@ -151,7 +167,7 @@ __device__ __forceinline__ void infer_tile(const uint16_t* __restrict__ sw, cons
 #pragma unroll
       for (int e = 0; e < 4; e++) {
         const int f = 16 * t + 4 * g + e;
-        s += bf2f(f2bf(fmaxf(acc[q][t][e] + b4[f], 0.0f))) * w5[f];
+        s += bf2f(f2bf(fmaxf(acc[q][t][e], 0.0f))) * w5[f];
       }
     s += __shfl_xor(s, 16, 64);
     s += __shfl_xor(s, 32, 64);
