@@ -195,3 +195,25 @@ def test_early_packets_in_a_batch(cuda):
         ref, _ = port.Mesh(v.astype(np.float64), f).sdf(queries[k][sub].astype(np.float64))
         assert np.abs(d0[sub] - ref).max() <= D_TOL * diag, k
         assert np.array_equal(np.sign(d0[sub]), np.sign(ref)), k
+
+
+def test_more_than_256_items_use_wide_keys(cuda):
+    """Plans with more than 256 items sort 64-bit keys (item bits above 8): results must still
+    equal the single-mesh queries."""
+    import torch
+    w = wl.batched_meshes(3, 23, level_range=(2, 3), query_range=(100, 300))
+    b = MeshBatch(list(zip(w.verts, w.faces)), branching=4, leaf_size=8)
+    rng = np.random.default_rng(4)
+    items = []
+    for k in range(300):
+        mesh = int(rng.integers(0, 3))
+        q = w.queries[mesh][: int(rng.integers(1, w.queries[mesh].shape[0] + 1))]
+        items.append((mesh, torch.from_numpy(np.ascontiguousarray(q)).to(cuda)))
+    outs = b.query(items)
+    torch.cuda.synchronize()
+    singles = [TriMeshSDF(w.verts[k], w.faces[k], branching=4, leaf_size=8) for k in range(3)]
+    for (mesh, q), d0 in zip(items, outs):
+        d1 = singles[mesh].query(q).cpu().numpy()
+        diag = diag_of(w.verts[mesh])
+        assert np.abs(d0.cpu().numpy() - d1).max() <= 1e-6 * diag
+        assert np.array_equal(np.sign(d0.cpu().numpy()), np.sign(d1))
