@@ -26,6 +26,7 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
+BF16_PEAK_FLOPS = 2.5e15  # MI355X dense BF16 MFMA
 
 
 def log(*a):
@@ -298,6 +299,22 @@ def main():
     train_s = time.perf_counter() - t0
     log(f"bound trained in {train_s:.1f}s, final loss {loss:.4f}")
 
+    # MFMA utilisation of the bound network: inference over the 16M queries alone (HIP events on
+    # torch's stream, which the library uses), 2 * (51*128 + 3*128*128 + 128) FLOP per query
+    mlp_flop_q = 2 * (51 * 128 + 3 * 128 * 128 + 128)
+    for _ in range(2):
+        mesh.predict_bound(qt)
+    torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(5):
+        mesh.predict_bound(qt)
+    e1.record()
+    torch.cuda.synchronize()
+    infer_ms = e0.elapsed_time(e1) / 5
+    mfma_util = mlp_flop_q * n / (infer_ms * 1e-3) / BF16_PEAK_FLOPS
+    log(f"bound inference {infer_ms:.2f} ms for {n} queries ({mfma_util:.3f} of the BF16 dense peak)")
+
     t_plain, trav_plain = timed(plain, a.steps, a.warmup, mesh.last_traversal_ms)
     t_learn, trav_learn = timed(learned, a.steps, a.warmup, mesh.last_traversal_ms)
     log(f"plain {t_plain * 1e3:.2f} ms/step (traversal {trav_plain:.2f} ms), learned {t_learn * 1e3:.2f} ms/step "
@@ -357,7 +374,11 @@ def main():
                    "fp64_recheck_below": "1e-4*diag"},
         "plain": {"value": round(n / t_plain, 1), "ms_per_step": round(t_plain * 1e3, 3)},
         "speedup_learned_vs_plain": round(t_plain / t_learn, 3),
-        "learned_bound": {"train_samples": a.train_samples, "epochs": a.epochs, "train_seconds": round(train_s, 2),
+        "learned_bound": {"inference_ms": round(infer_ms, 3),
+                          "inference_mfma_utilisation": round(mfma_util, 4),
+                          "train_mfma_utilisation_incl_labelling": round(
+                              3 * mlp_flop_q * a.train_samples * a.epochs / train_s / BF16_PEAK_FLOPS, 4),
+                          "train_samples": a.train_samples, "epochs": a.epochs, "train_seconds": round(train_s, 2),
                           "final_loss": round(loss, 5), "max_abs_diff_vs_plain_over_diag": dmax / diag,
                           "sign_agreement_vs_plain": sign_ok},
         "roofline": rl,
