@@ -70,18 +70,66 @@ __device__ __forceinline__ float feature(int f, float x0, float x1, float x2)
 // ------------------------------------------------------------------------------------------
 // fused inference: one wavefront = kInferQT tiles of 16 queries, each weight fragment read from
 // LDS once per kInferQT MFMAs; kInferWaves waves per workgroup.
-// The BF16 weight matrices (112 KB, the inference K order) are copied once per workgroup into
+// The BF16 weight fragments (116 KB) and the FP32 biases are copied once per workgroup into
 // LDS and the workgroup then strides over query tiles (one workgroup per CU), so every A
 // fragment comes from LDS instead of an L2 round trip per wavefront.
 // ------------------------------------------------------------------------------------------
 
 constexpr int kInferWaves = 8; // 2 per SIMD
-constexpr int kInferQT    = 3; // 16-query tiles per wavefront (48 queries; 4 spill at 2 waves/SIMD)
+constexpr int kInferQT    = 3; // 16-query tiles per wavefront (48 queries)
+constexpr int kInferBias  = 4 * kMlpHidden; // b1..b4 in LDS
 
-__device__ __forceinline__ void infer_tile(const uint16_t* __restrict__ sw, const float* __restrict__ pts,
-                                           const uint32_t* __restrict__ idx, uint64_t n, uint64_t base,
-                                           const MlpDev& p, float scale, float offset, int seed_mode,
-                                           float* __restrict__ out)
+// Layer-1 K order of the fused kernel: K index 32 b + 8 g + e holds, for g < 3, the Fourier
+// feature sin(w_j x_g) with j = 8 b + e, and for g = 3 the raw coordinate x_e (b = 0, e < 3)
+// or zero padding (-1). A lane's frequencies are then compile-time constants and its coordinate
+// is one select per query; features are bitwise those of features_kernel.
+__host__ __device__ constexpr int infer_feature(int b, int g, int e)
+{
+  return g < 3 ? 3 + 3 * (8 * b + e) + g : (b == 0 && e < 3 ? e : -1);
+}
+
+__device__ __forceinline__ uint32_t pk_bf16(float a, float b)
+{
+  return __builtin_bit_cast(uint32_t, __builtin_convertvector((f32x2){a, b}, bf16x2));
+}
+
+// relu'd BF16 copy of a layer's accumulators as the next layer's four B operands (K order phi)
+__device__ __forceinline__ void relu_to_b(const f32x4 (&acc)[8], u16x8 (&bq)[4])
+{
+#pragma unroll
+  for (int b = 0; b < 4; b++) {
+    const u32x4v w = {relu_pk_bf16(acc[2 * b][0], acc[2 * b][1]), relu_pk_bf16(acc[2 * b][2], acc[2 * b][3]),
+                      relu_pk_bf16(acc[2 * b + 1][0], acc[2 * b + 1][1]),
+                      relu_pk_bf16(acc[2 * b + 1][2], acc[2 * b + 1][3])};
+    bq[b] = __builtin_bit_cast(u16x8, w);
+  }
+}
+
+// acc[q][t] += A(t, b) B_q(b) over the 8 NB fragments of one layer, K block b outermost so that
+// consecutive MFMAs never chain on one accumulator; the A fragments stream from LDS through a
+// ring of three registers, two loads ahead of the MFMAs that use them.
+template <int NB>
+__device__ __forceinline__ void layer_mma(const uint16_t* __restrict__ w, int lane, f32x4 (&acc)[kInferQT][8],
+                                          const u16x8 (&bq)[kInferQT][4])
+{
+  constexpr int F = 8 * NB;
+  auto frag = [&](int f) { return *reinterpret_cast<const u16x8*>(w + (((f % 8) * NB + f / 8) * 64 + lane) * 8); };
+  u16x8 a[3];
+  a[0] = frag(0);
+  a[1] = frag(1);
+#pragma unroll
+  for (int f = 0; f < F; f++) {
+    if (f + 2 < F) a[(f + 2) % 3] = frag(f + 2);
+    const int t = f % 8, b = f / 8;
+#pragma unroll
+    for (int q = 0; q < kInferQT; q++) acc[q][t] = mfma(a[f % 3], bq[q][b], acc[q][t]);
+  }
+}
+
+__device__ __forceinline__ void infer_tile(const uint16_t* __restrict__ sw, const float* __restrict__ sb,
+                                           const float* __restrict__ pts, const uint32_t* __restrict__ idx,
+                                           uint64_t n, uint64_t base, const MlpDev& p, float b5, float scale,
+                                           float offset, int seed_mode, float* __restrict__ out)
 {
   const int lane = threadIdx.x & 63;
   const int nn = lane & 15, g = lane >> 4;
@@ -97,83 +145,65 @@ __device__ __forceinline__ void infer_tile(const uint16_t* __restrict__ sw, cons
     float x[3]       = {0.0f, 0.0f, 0.0f};
     if (valid[q])
       for (int c = 0; c < 3; c++) x[c] = (pts[3 * (size_t)qid[q] + c] - p.lo[c]) * p.sc[c] - 1.0f;
+    // g = 3 lanes take 0 here, so their sin() terms are exactly the zero padding
+    const float xg = g == 0 ? x[0] : (g == 1 ? x[1] : (g == 2 ? x[2] : 0.0f));
 #pragma unroll
-    for (int b = 0; b < 2; b++)
+    for (int b = 0; b < 2; b++) {
+      float v[8];
 #pragma unroll
-      for (int e = 0; e < 8; e++) bq[q][b][e] = f2bf(feature(32 * b + 8 * g + e, x[0], x[1], x[2]));
+      for (int e = 0; e < 8; e++) {
+        v[e] = __builtin_amdgcn_sinf(c_freq[8 * b + e] * xg);
+        if (b == 0 && e < 3) v[e] = g == 3 ? x[e] : v[e];
+      }
+      const u32x4v w = {pk_bf16(v[0], v[1]), pk_bf16(v[2], v[3]), pk_bf16(v[4], v[5]), pk_bf16(v[6], v[7])};
+      bq[q][b] = __builtin_bit_cast(u16x8, w);
+    }
   }
 
   // each layer's accumulators start from its bias (output rows 16 t + 4 g .. + 3 of this lane)
   f32x4 acc[kInferQT][8];
-  {
-    const float* bias = p.master + kOffB1;
 #pragma unroll
-    for (int t = 0; t < 8; t++) {
-      const f32x4 bv = *reinterpret_cast<const f32x4*>(bias + 16 * t + 4 * g);
+  for (int t = 0; t < 8; t++) {
+    const f32x4 bv = *reinterpret_cast<const f32x4*>(sb + 16 * t + 4 * g);
 #pragma unroll
-      for (int q = 0; q < kInferQT; q++) acc[q][t] = bv;
-    }
+    for (int q = 0; q < kInferQT; q++) acc[q][t] = bv;
   }
   // layer 1 (K = 64)
-#pragma unroll
-  for (int t = 0; t < 8; t++)
-#pragma unroll
-    for (int b = 0; b < 2; b++) {
-      const u16x8 a = *reinterpret_cast<const u16x8*>(sw + kBfW1 + ((2 * t + b) * 64 + lane) * 8);
-#pragma unroll
-      for (int q = 0; q < kInferQT; q++) acc[q][t] = mfma(a, bq[q][b], acc[q][t]);
-    }
+  layer_mma<2>(sw + kBfW1, lane, acc, bq);
   // layers 2..4 (K = 128)
   const int wofs[3] = {kBfW2, kBfW3, kBfW4};
-  const int bofs[3] = {kOffB2, kOffB3, kOffB4}; // bias of each layer's output
 #pragma unroll
   for (int L = 0; L < 3; L++) {
     // the previous layer's output, relu'd and rounded to BF16, is this layer's B operand
 #pragma unroll
-    for (int b = 0; b < 4; b++)
-#pragma unroll
-      for (int q = 0; q < kInferQT; q++) {
-        const u32x4v w = {relu_pk_bf16(acc[q][2 * b][0], acc[q][2 * b][1]),
-                          relu_pk_bf16(acc[q][2 * b][2], acc[q][2 * b][3]),
-                          relu_pk_bf16(acc[q][2 * b + 1][0], acc[q][2 * b + 1][1]),
-                          relu_pk_bf16(acc[q][2 * b + 1][2], acc[q][2 * b + 1][3])};
-        bq[q][b] = __builtin_bit_cast(u16x8, w);
-      }
-    const float* bias = p.master + bofs[L];
+    for (int q = 0; q < kInferQT; q++) relu_to_b(acc[q], bq[q]);
 #pragma unroll
     for (int t = 0; t < 8; t++) {
-      const f32x4 bv = *reinterpret_cast<const f32x4*>(bias + 16 * t + 4 * g);
+      const f32x4 bv = *reinterpret_cast<const f32x4*>(sb + (L + 1) * kMlpHidden + 16 * t + 4 * g);
 #pragma unroll
       for (int q = 0; q < kInferQT; q++) acc[q][t] = bv;
     }
-#pragma unroll
-    for (int t = 0; t < 8; t++)
-#pragma unroll
-      for (int b = 0; b < 4; b++) {
-        const u16x8 a = *reinterpret_cast<const u16x8*>(sw + wofs[L] + ((4 * t + b) * 64 + lane) * 8);
-#pragma unroll
-        for (int q = 0; q < kInferQT; q++) acc[q][t] = mfma(a, bq[q][b], acc[q][t]);
-      }
+    layer_mma<4>(sw + wofs[L], lane, acc, bq);
   }
-  // output layer: y = relu(h4 + b4) . w5 + b5 (b4 already in the accumulators; h4 rounded to
-  // bf16 as in training)
-  const float* w5 = p.master + kOffW5;
-  const float  b5 = p.master[kOffB5];
+  // output layer y = relu(h4) . w5 + b5 on MFMA: h4 rounded to BF16 as in training; rows 0..2 of
+  // the W5 fragments are w5's three BF16 parts, so lanes g = 0 receive b5 + hi.h, mid.h, lo.h
+#pragma unroll
+  for (int q = 0; q < kInferQT; q++) relu_to_b(acc[q], bq[q]);
+  f32x4 y[kInferQT];
+#pragma unroll
+  for (int q = 0; q < kInferQT; q++) y[q] = (f32x4){b5, 0.0f, 0.0f, 0.0f};
+#pragma unroll
+  for (int b = 0; b < 4; b++) {
+    const u16x8 a = *reinterpret_cast<const u16x8*>(sw + kBfW5 + (b * 64 + lane) * 8);
+#pragma unroll
+    for (int q = 0; q < kInferQT; q++) y[q] = mfma(a, bq[q][b], y[q]);
+  }
 #pragma unroll
   for (int q = 0; q < kInferQT; q++) {
-    float s = 0.0f;
-#pragma unroll
-    for (int t = 0; t < 8; t++)
-#pragma unroll
-      for (int e = 0; e < 4; e++) {
-        const int f = 16 * t + 4 * g + e;
-        s += bf2f(f2bf(fmaxf(acc[q][t][e], 0.0f))) * w5[f];
-      }
-    s += __shfl_xor(s, 16, 64);
-    s += __shfl_xor(s, 32, 64);
     if (g == 0 && valid[q]) {
-      const float y = (expf(s + b5) - kMlpLogC) * p.diag;
-      out[qid[q]]   = seed_mode ? scale * fmaxf(y, 0.0f) + offset : y;
+      const float s = y[q][0] + y[q][1] + y[q][2];
+      const float r = (expf(s) - kMlpLogC) * p.diag;
+      out[qid[q]]   = seed_mode ? scale * fmaxf(r, 0.0f) + offset : r;
     }
   }
 }
@@ -183,15 +213,21 @@ __global__ __launch_bounds__(64 * kInferWaves) void infer_kernel(const float* __
                                                                  float scale, float offset, int seed_mode,
                                                                  float* __restrict__ out)
 {
-  __shared__ __attribute__((aligned(16))) uint16_t sw[kBfTotal];
-  for (int i = 8 * threadIdx.x; i < kBfTotal; i += 8 * 64 * kInferWaves)
+  __shared__ __attribute__((aligned(16))) uint16_t sw[kBfInfer];
+  __shared__ __attribute__((aligned(16))) float    sb[kInferBias];
+  for (int i = 8 * threadIdx.x; i < kBfInfer; i += 8 * 64 * kInferWaves)
     *reinterpret_cast<u16x8*>(sw + i) = *reinterpret_cast<const u16x8*>(p.winfer + i);
+  {
+    const int bofs[4] = {kOffB1, kOffB2, kOffB3, kOffB4};
+    for (int i = threadIdx.x; i < kInferBias; i += 64 * kInferWaves) sb[i] = p.master[bofs[i / kMlpHidden] + i % kMlpHidden];
+  }
   __syncthreads();
+  const float b5 = p.master[kOffB5];
   // every wavefront runs the same number of iterations of this loop shape (no barrier inside)
   const uint64_t per = 16u * kInferQT * kInferWaves;
   for (uint64_t tile = blockIdx.x; tile * per < n; tile += gridDim.x) {
     const uint64_t base = tile * per + 16u * kInferQT * (threadIdx.x >> 6);
-    if (base < n) infer_tile(sw, pts, idx, n, base, p, scale, offset, seed_mode, out);
+    if (base < n) infer_tile(sw, sb, pts, idx, n, base, p, b5, scale, offset, seed_mode, out);
   }
 }
 
@@ -347,7 +383,23 @@ __global__ __launch_bounds__(256) void sync_bf16_kernel(const float* __restrict_
                                                         uint16_t* __restrict__ winfer)
 {
   const int i = blockIdx.x * 256 + threadIdx.x;
-  if (i >= kBfTotal) return;
+  if (i >= kBfInfer) return;
+  // inference copy, fragment-major: A fragment (output tile t of 16 rows, K block b of 32) is
+  // 64 lanes x 8 BF16 contiguous, lane = row + 16 g holding K elements 8 g .. 8 g + 7, so that a
+  // wavefront's 16-byte LDS reads of one fragment fall on consecutive banks
+  if (i >= kBfW5) {
+    // output layer: row r < 3 of fragment b holds part r of w5 in the K order of layers 2..4
+    const int j = i - kBfW5, e = j % 8, lane = (j / 8) % 64, b = j / 512, g = lane / 16, r = lane % 16;
+    const int k = 32 * b + (e < 4 ? 4 * g + e : 16 + 4 * g + (e - 4));
+    const float  w  = master[kOffW5 + k];
+    const __bf16 hi = (__bf16)w;
+    const float  r1 = w - (float)hi;
+    const __bf16 mid = (__bf16)r1;
+    const __bf16 lo  = (__bf16)(r1 - (float)mid);
+    const __bf16 v   = r == 0 ? hi : (r == 1 ? mid : (r == 2 ? lo : (__bf16)0.0f));
+    winfer[i]        = __builtin_bit_cast(uint16_t, v);
+    return;
+  }
   // training copy: same [in][out] layout as the master
   int src;
   if (i < kBfW2) src = kOffW1 + i;
@@ -355,9 +407,6 @@ __global__ __launch_bounds__(256) void sync_bf16_kernel(const float* __restrict_
   else if (i < kBfW4) src = kOffW3 + (i - kBfW3);
   else src = kOffW4 + (i - kBfW4);
   wtrain[i] = f2bf(master[src]);
-  // inference copy, fragment-major: A fragment (output tile t of 16 rows, K block b of 32) is
-  // 64 lanes x 8 BF16 contiguous, lane = row + 16 g holding K elements 8 g .. 8 g + 7, so that a
-  // wavefront's 16-byte LDS reads of one fragment fall on consecutive banks
   const bool first = i < kBfW2;
   const int  base  = first ? kBfW1 : (i < kBfW3 ? kBfW2 : (i < kBfW4 ? kBfW3 : kBfW4));
   const int  moff  = first ? kOffW1 : (i < kBfW3 ? kOffW2 : (i < kBfW4 ? kOffW3 : kOffW4));
@@ -365,10 +414,11 @@ __global__ __launch_bounds__(256) void sync_bf16_kernel(const float* __restrict_
   const int  j = i - base, e = j % 8, lane = (j / 8) % 64, frag = j / 512;
   const int  t = frag / nb, b = frag % nb, g = lane / 16;
   const int  r = 16 * t + lane % 16;
-  // K order: the input features as they are for layer 1; for layers 2..4 the order in which
-  // the previous layer's accumulators sit in a lane (rows 4 g .. 4 g + 3 of tiles 2b, 2b+1)
-  const int k = first ? 32 * b + 8 * g + e : 32 * b + (e < 4 ? 4 * g + e : 16 + 4 * g + (e - 4));
-  winfer[i]   = f2bf(master[moff + k * kMlpHidden + r]); // W^T[out r][in k]
+  // K order: for layer 1 the feature order of infer_feature (padding -> zero weight); for
+  // layers 2..4 the order in which the previous layer's accumulators sit in a lane (rows
+  // 4 g .. 4 g + 3 of tiles 2b, 2b+1)
+  const int k = first ? infer_feature(b, g, e) : 32 * b + (e < 4 ? 4 * g + e : 16 + 4 * g + (e - 4));
+  winfer[i]   = k < 0 ? (uint16_t)0 : f2bf(master[moff + k * kMlpHidden + r]); // W^T[out r][in k]
 }
 
 __global__ void scale_kernel(float* p, int n, float s)
@@ -503,7 +553,7 @@ hipError_t mlp_alloc(Mlp& m)
   if ((e = hipMalloc(&m.adam_m, sizeof(float) * kMaster)) != hipSuccess) return e;
   if ((e = hipMalloc(&m.adam_v, sizeof(float) * kMaster)) != hipSuccess) return e;
   if ((e = hipMalloc(&m.wtrain, sizeof(uint16_t) * kBfTotal)) != hipSuccess) return e;
-  if ((e = hipMalloc(&m.winfer, sizeof(uint16_t) * kBfTotal)) != hipSuccess) return e;
+  if ((e = hipMalloc(&m.winfer, sizeof(uint16_t) * kBfInfer)) != hipSuccess) return e;
   if ((e = hipMemset(m.grad, 0, sizeof(float) * kMaster)) != hipSuccess) return e;
   return hipSuccess;
 }
@@ -575,7 +625,7 @@ hipError_t mlp_init(Mlp& m, uint64_t seed)
 
 hipError_t mlp_sync_bf16(Mlp& m, hipStream_t s)
 {
-  hipLaunchKernelGGL(sync_bf16_kernel, dim3((kBfTotal + 255) / 256), dim3(256), 0, s, m.master, m.wtrain, m.winfer);
+  hipLaunchKernelGGL(sync_bf16_kernel, dim3((kBfInfer + 255) / 256), dim3(256), 0, s, m.master, m.wtrain, m.winfer);
   return hipGetLastError();
 }
 

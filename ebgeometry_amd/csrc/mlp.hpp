@@ -51,6 +51,10 @@ constexpr int kBfW2 = kBfW1 + kMlpInPad * kMlpHidden;
 constexpr int kBfW3 = kBfW2 + kMlpHidden * kMlpHidden;
 constexpr int kBfW4 = kBfW3 + kMlpHidden * kMlpHidden;
 constexpr int kBfTotal = kBfW4 + kMlpHidden * kMlpHidden;
+// The inference copy also holds the output layer as four 16 x 32 A fragments whose rows 0..2
+// are w5 split into three BF16 parts (hi + mid + lo = w5 to FP32 precision), rows 3..15 zero.
+constexpr int kBfW5    = kBfTotal;
+constexpr int kBfInfer = kBfW5 + 4 * 512;
 
 struct Mlp
 {
@@ -64,7 +68,7 @@ struct Mlp
   float*    adam_m  = nullptr;
   float*    adam_v  = nullptr;
   uint16_t* wtrain  = nullptr;   // kBfTotal BF16, row-major [in][out]
-  uint16_t* winfer  = nullptr;   // kBfTotal BF16, inference fragment layout
+  uint16_t* winfer  = nullptr;   // kBfInfer BF16, inference fragment layout
   int       step    = 0;
   ~Mlp();
 };

@@ -247,55 +247,116 @@ __global__ __launch_bounds__(256) void scan_apply(uint32_t* __restrict__ a, uint
   }
 }
 
-// Stable scatter: keys of the tile are ranked in index order, 256 at a time; within a
-// wavefront the rank among equal digits comes from 8 ballots (one per digit bit). kout may be
-// null (last pass of a sort whose caller only wants the permutation).
+// Stable scatter of one tile of kTile keys, in three phases:
+//  1. wavefront w ranks its own contiguous kTile / W keys in index order, 64 at a time (rank
+//     among equal digits from 8 ballots), against a per-wavefront digit counter in LDS: no
+//     workgroup barrier inside the loop;
+//  2. per digit: tile-local start (exclusive scan of the tile's digit counts) plus the counts
+//     of the lower wavefronts gives each wavefront's base; keys and values are placed at their
+//     tile-local sorted positions in LDS;
+//  3. the tile is read back in sorted order and written out, so each digit's run of the tile
+//     goes to consecutive addresses (coalesced stores instead of one scattered store per key).
+// kout may be null (last pass of a sort whose caller only wants the permutation).
 template <class KeyT>
 __global__ __launch_bounds__(kSortThreads) void radix_scatter(const KeyT* __restrict__ kin,
                                                               const uint32_t* __restrict__ vin, uint64_t n, int shift,
                                                               const uint32_t* __restrict__ offs, uint32_t nblocks,
                                                               KeyT* __restrict__ kout, uint32_t* __restrict__ vout)
 {
-  constexpr int       W = kSortThreads / 64;
-  __shared__ uint32_t base[256];
-  __shared__ uint32_t wcnt[W][256];
+  constexpr int       W   = kSortThreads / 64;
+  constexpr int       PER = kSortItems; // keys per lane
+  __shared__ uint32_t cnt[W][256];      // phase 1: running counts; phase 2: wavefront bases
+  __shared__ uint32_t lstart[256];      // tile-local start of each digit
+  __shared__ uint32_t goff[256];        // global start of each digit's run of this tile
+  __shared__ uint32_t wsum[W];
+  __shared__ KeyT     sk[kTile];
+  __shared__ uint32_t sv[kTile];
   const int           lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  base[threadIdx.x] = offs[(uint64_t)threadIdx.x * nblocks + blockIdx.x];
-  for (int w = 0; w < W; w++) wcnt[w][threadIdx.x] = 0;
+#pragma unroll
+  for (int w = 0; w < W; w++) cnt[w][threadIdx.x] = 0;
+  goff[threadIdx.x] = offs[(uint64_t)threadIdx.x * nblocks + blockIdx.x];
   __syncthreads();
-  const uint64_t tile = (uint64_t)blockIdx.x * kTile;
-  const uint64_t lt   = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
-  for (int r = 0; r < kSortItems; r++) {
-    const uint64_t i     = tile + (uint64_t)r * kSortThreads + threadIdx.x;
+
+  const uint64_t tile  = (uint64_t)blockIdx.x * kTile;
+  const uint64_t first = tile + (uint64_t)wave * (64 * PER);
+  const uint64_t lt    = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+  KeyT           key[PER];
+  uint32_t       val[PER], rk[PER];
+#pragma unroll
+  for (int r = 0; r < PER; r++) {
+    const uint64_t i     = first + (uint64_t)r * 64 + lane;
     const bool     valid = i < n;
-    const KeyT     key   = valid ? kin[i] : KeyT(0);
-    uint32_t       val   = valid ? (vin ? vin[i] : (uint32_t)i) : 0u;
-    const uint32_t d     = (uint32_t)(key >> shift) & 255u;
+    key[r]               = valid ? kin[i] : KeyT(0);
+    val[r]               = valid ? (vin ? vin[i] : (uint32_t)i) : 0u;
+  }
+#pragma unroll
+  for (int r = 0; r < PER; r++) {
+    const uint64_t i     = first + (uint64_t)r * 64 + lane;
+    const bool     valid = i < n;
+    const uint32_t d     = (uint32_t)(key[r] >> shift) & 255u;
     uint64_t       peers = __ballot(valid);
 #pragma unroll
     for (int b = 0; b < 8; b++) {
       const uint64_t bal = __ballot((d >> b) & 1u);
       peers &= ((d >> b) & 1u) ? bal : ~bal;
     }
-    const uint32_t rank   = (uint32_t)__popcll(peers & lt);
-    const uint32_t ncount = (uint32_t)__popcll(peers);
-    if (valid && rank == 0) wcnt[wave][d] = ncount;
-    __syncthreads();
-    if (valid) {
-      uint32_t pre = 0;
-      for (int w = 0; w < wave; w++) pre += wcnt[w][d];
-      const uint32_t pos = base[d] + pre + rank;
-      if (kout) kout[pos] = key;
-      vout[pos] = val;
-    }
-    __syncthreads();
-    uint32_t tot = 0;
+    const uint32_t rank = (uint32_t)__popcll(peers & lt);
+    const uint32_t prev = valid ? cnt[wave][d] : 0u;
+    rk[r]               = valid ? prev + rank : 0xFFFFFFFFu;
+    if (valid && rank == 0) cnt[wave][d] = prev + (uint32_t)__popcll(peers);
+  }
+  __syncthreads();
+
+  // phase 2: thread t owns digit t
+  {
+    uint32_t c[W], tot = 0;
+#pragma unroll
     for (int w = 0; w < W; w++) {
-      tot += wcnt[w][threadIdx.x];
-      wcnt[w][threadIdx.x] = 0;
+      c[w] = cnt[w][threadIdx.x];
+      tot += c[w];
     }
-    base[threadIdx.x] += tot;
+    // exclusive scan of tot over the 256 digits: inclusive within each wavefront, then the
+    // wavefront totals
+    uint32_t inc = tot;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+      const uint32_t t = __shfl_up(inc, o, 64);
+      if (lane >= o) inc += t;
+    }
+    if (lane == 63) wsum[wave] = inc;
     __syncthreads();
+    uint32_t start = inc - tot;
+    for (int w = 0; w < wave; w++) start += wsum[w];
+    lstart[threadIdx.x] = start;
+#pragma unroll
+    for (int w = 0; w < W; w++) {
+      cnt[w][threadIdx.x] = start;
+      start += c[w];
+    }
+  }
+  __syncthreads();
+#pragma unroll
+  for (int r = 0; r < PER; r++) {
+    if (rk[r] == 0xFFFFFFFFu) continue;
+    const uint32_t d   = (uint32_t)(key[r] >> shift) & 255u;
+    const uint32_t pos = cnt[wave][d] + rk[r];
+    sk[pos]            = key[r];
+    sv[pos]            = val[r];
+  }
+  __syncthreads();
+
+  // phase 3: sorted read-out
+  const uint32_t valid_n = (uint32_t)((n - tile) < (uint64_t)kTile ? (n - tile) : (uint64_t)kTile);
+#pragma unroll
+  for (int r = 0; r < PER; r++) {
+    const uint32_t j = (uint32_t)r * kSortThreads + threadIdx.x;
+    if (j < valid_n) {
+      const KeyT     k   = sk[j];
+      const uint32_t d   = (uint32_t)(k >> shift) & 255u;
+      const uint32_t pos = goff[d] + (j - lstart[d]);
+      if (kout) kout[pos] = k;
+      vout[pos] = sv[j];
+    }
   }
 }
 
