@@ -377,32 +377,36 @@ ebg_status ebg_query_sdf(const ebg_mesh* cm, const float* d_pts, uint64_t n, flo
     seeds = ws.seeds.as<float>();
   }
 
+  // the presort keys carry each query's cost flags (cost-grid value against the heavy and early
+  // thresholds) to the packet marking
+  ebgpu::CostFlags cf{};
+  if (heavy_first) {
+    cf.grid = m->cost_grid.as<uint32_t>();
+    for (int c = 0; c < 3; c++) {
+      cf.lo[c]    = m->grid_lo[c];
+      cf.scale[c] = m->grid_scale[c];
+    }
+    cf.g     = ebgpu::kCostGrid;
+    cf.heavy = uint32_t(std::max(1.0, heavy_factor(n) * m->grid_mean));
+    cf.early = uint32_t(std::max(1.0, kEarlyFactor * m->grid_mean));
+  }
   const uint32_t* idx = nullptr;
   if (o.presort && n > 1) {
-    EBG_HIP(ws.presort(d_pts, n, s));
+    EBG_HIP(ws.presort(d_pts, n, s, heavy_first ? &cf : nullptr));
     idx = ws.perm.as<uint32_t>();
   }
   ebgpu::HeavyMarkArgs h{};
   if (heavy_first) {
-    h.pts = d_pts;
-    h.idx = idx;
-    h.n   = n;
-    h.grid = m->cost_grid.as<uint32_t>();
-    for (int c = 0; c < 3; c++) {
-      h.lo[c]    = m->grid_lo[c];
-      h.scale[c] = m->grid_scale[c];
-    }
-    h.g         = ebgpu::kCostGrid;
-    h.threshold = uint32_t(std::max(1.0, heavy_factor(n) * m->grid_mean));
-    h.cap       = ebgpu::Workspace::heavy_cap(n);
-    h.list      = ws.heavy_list();
-    h.count     = cnt + 2;
-    h.flag      = ws.heavy_flag();
-    h.keys      = ws.heavy_keys();
-    h.early_threshold = uint32_t(std::max(1.0, kEarlyFactor * m->grid_mean));
-    h.early_cap       = ebgpu::Workspace::early_cap(n);
-    h.early_list      = ws.early_list();
-    h.early_count     = cnt + 3;
+    h.skeys       = ws.codes_sorted.as<uint32_t>();
+    h.n           = n;
+    h.cap         = ebgpu::Workspace::heavy_cap(n);
+    h.list        = ws.heavy_list();
+    h.count       = cnt + 2;
+    h.flag        = ws.heavy_flag();
+    h.keys        = ws.heavy_keys();
+    h.early_cap   = ebgpu::Workspace::early_cap(n);
+    h.early_list  = ws.early_list();
+    h.early_count = cnt + 3;
     EBG_HIP(ebgpu::launch_heavy_mark(h, s));
   }
   if (o.learned_bound) EBG_HIP(hipStreamWaitEvent(s, m->ev_join, 0));

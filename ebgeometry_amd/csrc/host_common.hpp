@@ -114,16 +114,16 @@ struct Workspace
   uint32_t* fallback_list() const { return lists.as<uint32_t>() + cap; }
   uint32_t* counters() const { return lists.as<uint32_t>() + 2 * cap; } // [0] recheck [1] fallback [2] split [3] early
 
-  // Hilbert presort of n device points: returns the permutation (valid until the next call).
-  hipError_t presort(const float* d_pts, uint64_t n, hipStream_t s)
+  // Hilbert presort of n device points: the permutation lands in perm (valid until the next
+  // call). With cost flags the sorted keys, flags in bits 30 and 31, land in codes_sorted.
+  hipError_t presort(const float* d_pts, uint64_t n, hipStream_t s, const CostFlags* cf = nullptr)
   {
     hipError_t e;
     if ((e = launch_bbox_f32(d_pts, n, box.as<float>(), box.as<float>() + 8, s)) != hipSuccess) return e;
-    // 30-bit Hilbert keys on a 1024^3 grid, sorted as 32-bit keys in 4 radix passes; only the
-    // permutation is kept
-    if ((e = launch_hilbert_f32(d_pts, n, box.as<float>(), codes.as<uint32_t>(), s)) != hipSuccess) return e;
+    // 30-bit Hilbert keys on a 1024^3 grid, sorted as 32-bit keys in 4 radix passes
+    if ((e = launch_hilbert_f32(d_pts, n, box.as<float>(), codes.as<uint32_t>(), s, cf)) != hipSuccess) return e;
     return radix_sort_pairs32(codes.as<uint32_t>(), nullptr, codes_sorted.as<uint32_t>(), perm.as<uint32_t>(), n, 30,
-                              sort_scratch.p, false, s);
+                              sort_scratch.p, cf != nullptr, s);
   }
 };
 

@@ -44,23 +44,30 @@ struct QueryArgs
   uint32_t            early_cap, early_blocks;
 };
 
-// Expensive packets: a packet whose most expensive query (cost-grid value of its cell, grid of
-// g^3 cells over [lo, lo + g/scale)) reaches threshold is listed (up to cap), flagged, and its
-// 64 result keys are reset. count must be zero on entry.
+// Cost flags of the split schedule, set while the presort keys are made: a query whose
+// cost-grid value (grid of g^3 cells over [lo, lo + g/scale)) reaches heavy / early sets bit 31
+// / bit 30 of its 30-bit presort key. The sort ignores both bits and carries them to the
+// sorted keys, so the packet marking reads them in sorted order without gathering points.
+struct CostFlags
+{
+  const uint32_t* grid;
+  float           lo[3], scale[3];
+  uint32_t        g, heavy, early;
+};
+
+// Expensive packets: a packet with a heavy-flagged query is listed (up to cap), flagged, and its
+// 64 result keys are reset; otherwise a packet with an early-flagged query goes to the early
+// list (up to early_cap). skeys are the sorted presort keys; count must be zero on entry.
 struct HeavyMarkArgs
 {
-  const float*        pts;
-  const uint32_t*     idx;
+  const uint32_t*     skeys;
   uint64_t            n;
-  const uint32_t*     grid;
-  float               lo[3], scale[3];
-  uint32_t            g, threshold, cap;
+  uint32_t            cap;
   uint32_t*           list;
   uint32_t*           count;
   uint8_t*            flag;
   unsigned long long* keys; // cap * 64
-  // packets below threshold but at least early_threshold go to the early list (up to early_cap)
-  uint32_t            early_threshold, early_cap;
+  uint32_t            early_cap;
   uint32_t*           early_list;
   uint32_t*           early_count;
 };
@@ -151,13 +158,16 @@ hipError_t launch_bbox_f64(const double* pts, uint64_t n, double* d_box6, void* 
 // top 32 bits, which order 64-query packets as well and sort in half the passes)
 hipError_t launch_codes_f32(const float* pts, uint64_t n, const float* d_box6, uint64_t* codes, hipStream_t s,
                             int shift = 0);
-// keys[i] = 30-bit Hilbert index of the point on a 1024^3 grid of d_box6 (query presort)
-hipError_t launch_hilbert_f32(const float* pts, uint64_t n, const float* d_box6, uint32_t* keys, hipStream_t s);
+// keys[i] = 30-bit Hilbert index of the point on a 1024^3 grid of d_box6 (query presort), with
+// the cost flags in bits 30 and 31 when cf is given
+hipError_t launch_hilbert_f32(const float* pts, uint64_t n, const float* d_box6, uint32_t* keys, hipStream_t s,
+                              const CostFlags* cf = nullptr);
 hipError_t launch_codes_f64(const double* pts, uint64_t n, const double* d_box6, uint64_t* codes, hipStream_t s);
 // Bytes of scratch radix_sort needs for n keys.
 std::size_t radix_sort_scratch_bytes(uint64_t n);
 // Stable LSD sort of (keys, values); values_in may be null (identity). Results land in
-// keys_out / values_out. 'bits' low bits of the keys are significant (<= 64).
+// keys_out / values_out. The 'bits' low bits of the keys are the sort key (<= 64); higher bits
+// ride along unsorted.
 hipError_t radix_sort_pairs(const uint64_t* keys_in, const uint32_t* values_in, uint64_t* keys_out,
                             uint32_t* values_out, uint64_t n, int bits, void* scratch, hipStream_t s);
 // The same for 32-bit keys (bits <= 32); with want_keys false the sorted keys are not written

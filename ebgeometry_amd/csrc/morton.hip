@@ -124,15 +124,31 @@ __global__ __launch_bounds__(256) void codes_f32(const float* __restrict__ p, ui
 // bits). Consecutive cells of the curve share a face, so a run of 64 sorted queries never jumps
 // across the box the way a Morton run can at a high-level boundary (C2: 10% fewer triangle
 // evaluations per wavefront).
+// With FLAGS, bits 31 / 30 carry the query's cost flags (see CostFlags).
+template <bool FLAGS>
 __global__ __launch_bounds__(256) void hilbert_f32(const float* __restrict__ p, uint64_t n,
-                                                   const float* __restrict__ box, uint32_t* __restrict__ keys)
+                                                   const float* __restrict__ box, uint32_t* __restrict__ keys,
+                                                   const CostFlags cf)
 {
   const uint64_t i = (uint64_t)blockIdx.x * 256 + threadIdx.x;
   if (i >= n) return;
+  float    q[3];
   uint32_t x[3];
 #pragma unroll
-  for (int c = 0; c < 3; c++) x[c] = dev::hilbert_bin<10>(p[3 * i + c], box[c], box[3 + c] - box[c]);
-  keys[i] = (uint32_t)dev::hilbert_index<10>(x[0], x[1], x[2]);
+  for (int c = 0; c < 3; c++) {
+    q[c] = p[3 * i + c];
+    x[c] = dev::hilbert_bin<10>(q[c], box[c], box[3 + c] - box[c]);
+  }
+  uint32_t key = (uint32_t)dev::hilbert_index<10>(x[0], x[1], x[2]);
+  if (FLAGS) {
+    uint32_t cell[3];
+#pragma unroll
+    for (int c = 0; c < 3; c++)
+      cell[c] = (uint32_t)fminf(fmaxf((q[c] - cf.lo[c]) * cf.scale[c], 0.0f), float(cf.g - 1)); // NaN -> 0
+    const uint32_t cost = cf.grid[((size_t)cell[2] * cf.g + cell[1]) * cf.g + cell[0]];
+    key |= (cost >= cf.heavy ? 1u << 31 : 0u) | (cost >= cf.early ? 1u << 30 : 0u);
+  }
+  keys[i] = key;
 }
 
 __global__ __launch_bounds__(256) void codes_f64(const double* __restrict__ p, uint64_t n, const double* __restrict__ box,
@@ -162,7 +178,8 @@ constexpr int kTile        = kSortThreads * kSortItems; // 4096 keys per workgro
 
 template <class KeyT>
 __global__ __launch_bounds__(kSortThreads) void radix_hist(const KeyT* __restrict__ keys, uint64_t n, int shift,
-                                                           uint32_t* __restrict__ hist, uint32_t nblocks)
+                                                           uint32_t dmask, uint32_t* __restrict__ hist,
+                                                           uint32_t nblocks)
 {
   __shared__ uint32_t h[256];
   h[threadIdx.x] = 0;
@@ -171,7 +188,7 @@ __global__ __launch_bounds__(kSortThreads) void radix_hist(const KeyT* __restric
 #pragma unroll 4
   for (int r = 0; r < kSortItems; r++) {
     const uint64_t i = base + (uint64_t)r * kSortThreads + threadIdx.x;
-    if (i < n) atomicAdd(&h[(uint32_t)(keys[i] >> shift) & 255u], 1u);
+    if (i < n) atomicAdd(&h[(uint32_t)(keys[i] >> shift) & dmask], 1u);
   }
   __syncthreads();
   hist[(uint64_t)threadIdx.x * nblocks + blockIdx.x] = h[threadIdx.x];
@@ -260,8 +277,9 @@ __global__ __launch_bounds__(256) void scan_apply(uint32_t* __restrict__ a, uint
 template <class KeyT>
 __global__ __launch_bounds__(kSortThreads) void radix_scatter(const KeyT* __restrict__ kin,
                                                               const uint32_t* __restrict__ vin, uint64_t n, int shift,
-                                                              const uint32_t* __restrict__ offs, uint32_t nblocks,
-                                                              KeyT* __restrict__ kout, uint32_t* __restrict__ vout)
+                                                              uint32_t dmask, const uint32_t* __restrict__ offs,
+                                                              uint32_t nblocks, KeyT* __restrict__ kout,
+                                                              uint32_t* __restrict__ vout)
 {
   constexpr int       W   = kSortThreads / 64;
   constexpr int       PER = kSortItems; // keys per lane
@@ -293,7 +311,7 @@ __global__ __launch_bounds__(kSortThreads) void radix_scatter(const KeyT* __rest
   for (int r = 0; r < PER; r++) {
     const uint64_t i     = first + (uint64_t)r * 64 + lane;
     const bool     valid = i < n;
-    const uint32_t d     = (uint32_t)(key[r] >> shift) & 255u;
+    const uint32_t d     = (uint32_t)(key[r] >> shift) & dmask;
     uint64_t       peers = __ballot(valid);
 #pragma unroll
     for (int b = 0; b < 8; b++) {
@@ -338,7 +356,7 @@ __global__ __launch_bounds__(kSortThreads) void radix_scatter(const KeyT* __rest
 #pragma unroll
   for (int r = 0; r < PER; r++) {
     if (rk[r] == 0xFFFFFFFFu) continue;
-    const uint32_t d   = (uint32_t)(key[r] >> shift) & 255u;
+    const uint32_t d   = (uint32_t)(key[r] >> shift) & dmask;
     const uint32_t pos = cnt[wave][d] + rk[r];
     sk[pos]            = key[r];
     sv[pos]            = val[r];
@@ -352,7 +370,7 @@ __global__ __launch_bounds__(kSortThreads) void radix_scatter(const KeyT* __rest
     const uint32_t j = (uint32_t)r * kSortThreads + threadIdx.x;
     if (j < valid_n) {
       const KeyT     k   = sk[j];
-      const uint32_t d   = (uint32_t)(k >> shift) & 255u;
+      const uint32_t d   = (uint32_t)(k >> shift) & dmask;
       const uint32_t pos = goff[d] + (j - lstart[d]);
       if (kout) kout[pos] = k;
       vout[pos] = sv[j];
@@ -378,10 +396,13 @@ hipError_t launch_codes_f32(const float* pts, uint64_t n, const float* box, uint
   hipLaunchKernelGGL(codes_f32, dim3((uint32_t)((n + 255) / 256)), dim3(256), 0, s, pts, n, box, codes, shift);
   return hipGetLastError();
 }
-hipError_t launch_hilbert_f32(const float* pts, uint64_t n, const float* box, uint32_t* keys, hipStream_t s)
+hipError_t launch_hilbert_f32(const float* pts, uint64_t n, const float* box, uint32_t* keys, hipStream_t s,
+                              const CostFlags* cf)
 {
   if (n == 0) return hipSuccess;
-  hipLaunchKernelGGL(hilbert_f32, dim3((uint32_t)((n + 255) / 256)), dim3(256), 0, s, pts, n, box, keys);
+  const dim3 grid((uint32_t)((n + 255) / 256));
+  if (cf) hipLaunchKernelGGL(hilbert_f32<true>, grid, dim3(256), 0, s, pts, n, box, keys, *cf);
+  else hipLaunchKernelGGL(hilbert_f32<false>, grid, dim3(256), 0, s, pts, n, box, keys, CostFlags{});
   return hipGetLastError();
 }
 hipError_t launch_codes_f64(const double* pts, uint64_t n, const double* box, uint64_t* codes, hipStream_t s)
@@ -431,12 +452,14 @@ hipError_t radix_sort_impl(const KeyT* keys_in, const uint32_t* values_in, KeyT*
     KeyT*      kdst   = to_out ? (pass == passes - 1 && !want_keys ? nullptr : keys_out) : ktmp;
     uint32_t*  vdst   = to_out ? values_out : vtmp;
     const int  shift  = 8 * pass;
-    hipLaunchKernelGGL((radix_hist<KeyT>), dim3(nb), dim3(kSortThreads), 0, s, ksrc, n, shift, hist, nb);
+    // bits at and above 'bits' are not part of the digit
+    const uint32_t dmask = bits - shift >= 8 ? 255u : (1u << (bits - shift)) - 1u;
+    hipLaunchKernelGGL((radix_hist<KeyT>), dim3(nb), dim3(kSortThreads), 0, s, ksrc, n, shift, dmask, hist, nb);
     hipLaunchKernelGGL(scan_reduce, dim3(nsum), dim3(256), 0, s, hist, m, sums);
     hipLaunchKernelGGL(scan_sums, dim3(1), dim3(1024), 0, s, sums, nsum);
     hipLaunchKernelGGL(scan_apply, dim3(nsum), dim3(256), 0, s, hist, m, sums);
-    hipLaunchKernelGGL((radix_scatter<KeyT>), dim3(nb), dim3(kSortThreads), 0, s, ksrc, vsrc, n, shift, hist, nb,
-                       kdst, vdst);
+    hipLaunchKernelGGL((radix_scatter<KeyT>), dim3(nb), dim3(kSortThreads), 0, s, ksrc, vsrc, n, shift, dmask, hist,
+                       nb, kdst, vdst);
     ksrc = kdst;
     vsrc = vdst;
   }

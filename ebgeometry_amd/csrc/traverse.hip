@@ -318,31 +318,22 @@ hipError_t launch_sdf32(const MeshDev& m, int K, int banks, const QueryArgs& a, 
 
 namespace {
 
-// One wavefront per packet (the 64 presorted queries pk*64 ..): the packet's cost estimate is the
-// largest cost-grid value over its queries' cells.
+// One wavefront per packet (the 64 presorted queries pk*64 ..): the packet is heavy (early) when
+// any of its queries carries the heavy (early) cost flag in its sorted presort key.
 __global__ __launch_bounds__(256) void heavy_mark_kernel(const HeavyMarkArgs h)
 {
-  const uint64_t pk     = (uint64_t)uni(blockIdx.x * 4u + (threadIdx.x >> 6));
-  const uint64_t tid    = pk * 64u + (uint64_t)__lane_id();
+  const uint64_t pk  = (uint64_t)uni(blockIdx.x * 4u + (threadIdx.x >> 6));
+  const uint64_t tid = pk * 64u + (uint64_t)__lane_id();
   if (pk * 64u >= h.n) return; // whole wavefront
-  uint32_t c = 0;
-  if (tid < h.n) {
-    const uint32_t qi = h.idx ? h.idx[tid] : (uint32_t)tid;
-    uint32_t       cell[3];
-#pragma unroll
-    for (int k = 0; k < 3; k++) {
-      const float x = (h.pts[3 * (size_t)qi + k] - h.lo[k]) * h.scale[k];
-      cell[k]       = (uint32_t)fminf(fmaxf(x, 0.0f), float(h.g - 1)); // NaN -> 0
-    }
-    c = h.grid[((size_t)cell[2] * h.g + cell[1]) * h.g + cell[0]];
-  }
-  const uint32_t est  = wave_max_u32(c);
+  const uint32_t key     = tid < h.n ? h.skeys[tid] : 0u;
+  const bool     any_hvy = ballot((key >> 31) != 0u) != 0ull;
+  const bool     any_erl = ballot(((key >> 30) & 1u) != 0u) != 0ull;
   uint32_t       slot = kInvalid, eslot = kInvalid;
-  if (est >= h.threshold && __lane_id() == 0) slot = atomicAdd(h.count, 1u);
+  if (any_hvy && __lane_id() == 0) slot = atomicAdd(h.count, 1u);
   slot = rdlane(slot, 0);
   const bool heavy = slot < h.cap;
   if (heavy) h.keys[64 * (size_t)slot + __lane_id()] = ~0ull;
-  if (!heavy && h.early_list && est >= h.early_threshold && __lane_id() == 0) eslot = atomicAdd(h.early_count, 1u);
+  if (!heavy && h.early_list && any_erl && __lane_id() == 0) eslot = atomicAdd(h.early_count, 1u);
   eslot            = rdlane(eslot, 0);
   const bool early = eslot < h.early_cap;
   if (__lane_id() == 0) {
