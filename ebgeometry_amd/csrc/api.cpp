@@ -422,6 +422,7 @@ ebg_status ebg_query_sdf(const ebg_mesh* cm, const float* d_pts, uint64_t n, flo
   a.recheck_abs    = o.fp64_all ? INFINITY : (recheck ? float(o.recheck_eps * m->diag) : 0.0f);
   a.recheck_list   = recheck ? ws.recheck_list() : nullptr;
   a.recheck_count  = cnt;
+  a.recheck_mask   = recheck ? ws.rmask.as<uint64_t>() : nullptr;
   a.fallback_list  = nullptr;
   a.fallback_count = cnt + 1;
   a.work           = reinterpret_cast<unsigned long long*>(o.work);
@@ -454,11 +455,14 @@ ebg_status ebg_query_sdf(const ebg_mesh* cm, const float* d_pts, uint64_t n, flo
   m->timed = true;
 
   if (recheck) {
+    // the list follows the presort order, so FP64 packets hold neighbouring queries
+    EBG_HIP(ebgpu::launch_recheck_compact(a.recheck_mask, n, idx, ws.recheck_list(), cnt, s));
     ebgpu::QueryArgs r = a;
     r.idx              = ws.recheck_list();
     r.count_ptr        = cnt;
     r.seed_r           = nullptr;
     r.recheck_list     = nullptr;
+    r.recheck_mask     = nullptr;
     r.fallback_list    = nullptr;
     r.work             = nullptr;
     // bound (|d32| (1 + rel) + abs)^2 always contains the FP64 minimum: FP32 distances are

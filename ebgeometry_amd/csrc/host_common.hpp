@@ -76,7 +76,7 @@ hipError_t upload(DevBuf& b, const std::vector<T>& v)
 // Query workspace, grown on demand.
 struct Workspace
 {
-  DevBuf   codes, codes_sorted, perm, sort_scratch, box, lists, seeds, sched;
+  DevBuf   codes, codes_sorted, perm, sort_scratch, box, lists, seeds, sched, rmask;
   uint64_t cap = 0;
 
   hipError_t ensure(uint64_t n)
@@ -92,6 +92,7 @@ struct Workspace
     // recheck list, fallback list, 2 counters (+pad)
     if ((e = lists.reserve(8 * m + 256)) != hipSuccess) return e;
     if ((e = seeds.reserve(4 * m)) != hipSuccess) return e;
+    if ((e = rmask.reserve(8 * packets(m))) != hipSuccess) return e; // recheck lane mask per packet
     // split packets: one flag byte per packet, the list of split packets, 64 result keys each
     if ((e = sched.reserve(sched_early_offset(m) + 4 * (size_t)early_cap(m))) != hipSuccess) return e;
     cap = m;

@@ -20,6 +20,9 @@ struct QueryArgs
   float               recheck_abs; // |d| below this goes to the recheck list (0 = off)
   uint32_t*           recheck_list;
   uint32_t*           recheck_count;
+  // non-null: instead of appending to recheck_list, each packet pk (threads 64 pk ..) writes the
+  // lane mask of its recheck queries here; launch_recheck_compact then builds the list
+  uint64_t*           recheck_mask;
   uint32_t*           fallback_list;  // queries whose seeded radius held no triangle
   uint32_t*           fallback_count;
   unsigned long long* work;      // [inner nodes, triangles] accumulated (null = not counted)
@@ -72,6 +75,12 @@ struct HeavyMarkArgs
   uint32_t*           early_count;
 };
 hipError_t launch_heavy_mark(const HeavyMarkArgs& h, hipStream_t s);
+
+// Recheck list from the per-packet masks of n threads: one wavefront per 64 consecutive packets
+// appends their flagged queries (idx[thread], or the thread) with one atomic on *count, in packet
+// order, so each run of the list covers a compact stretch of the presort curve.
+hipError_t launch_recheck_compact(const uint64_t* mask, uint64_t n, const uint32_t* idx, uint32_t* list,
+                                  uint32_t* count, hipStream_t s);
 
 // After the FP32 launch: the split packets' winning triangles classified, signed and written like
 // the plain pass (outputs, recheck list).

@@ -110,7 +110,11 @@ __device__ __forceinline__ void sdf32_packet(const MeshDev& m, const QueryArgs& 
     a.out_d[qi] = d;
     if (a.out_tri) a.out_tri[qi] = m.tris[btri].face;
   }
-  if (a.recheck_list) wave_append(found && fabsf(d) < a.recheck_abs, qi, a.recheck_list, a.recheck_count);
+  if (a.recheck_mask) {
+    const uint64_t rm = ballot(found && fabsf(d) < a.recheck_abs);
+    if (__lane_id() == 0) a.recheck_mask[pk] = rm;
+  }
+  else if (a.recheck_list) wave_append(found && fabsf(d) < a.recheck_abs, qi, a.recheck_list, a.recheck_count);
   if (COUNT && a.packet_cost && __lane_id() == 0) a.packet_cost[pk] = w_inner + w_tris;
   if (COUNT && a.work) {
     const uint64_t si = wave_sum_u64(active ? (uint64_t)n_inner : 0ull);
@@ -225,7 +229,39 @@ __global__ __launch_bounds__(256) void sdf32_resolve_kernel(const MeshDev m, con
       found = true;
     }
   }
-  if (a.recheck_list) wave_append(found && fabsf(d) < a.recheck_abs, qi, a.recheck_list, a.recheck_count);
+  if (a.recheck_mask) {
+    const uint64_t rm = ballot(found && fabsf(d) < a.recheck_abs);
+    if (__lane_id() == 0) a.recheck_mask[pk] = rm;
+  }
+  else if (a.recheck_list) wave_append(found && fabsf(d) < a.recheck_abs, qi, a.recheck_list, a.recheck_count);
+}
+
+// one wavefront per 64 consecutive packets
+__global__ __launch_bounds__(256) void recheck_compact_kernel(const uint64_t* __restrict__ mask, uint64_t npk,
+                                                              const uint32_t* __restrict__ idx, uint32_t* __restrict__ list,
+                                                              uint32_t* __restrict__ count)
+{
+  const uint64_t chunk = (uint64_t)uni(blockIdx.x * 4u + (threadIdx.x >> 6));
+  const int      lane  = __lane_id();
+  const uint64_t pk    = chunk * 64u + (uint64_t)lane;
+  uint64_t       mk    = pk < npk ? mask[pk] : 0ull;
+  const uint32_t c     = (uint32_t)__popcll(mk);
+  uint32_t       inc   = c;
+#pragma unroll
+  for (int o = 1; o < 64; o <<= 1) {
+    const uint32_t t = __shfl_up(inc, o, 64);
+    if (lane >= o) inc += t;
+  }
+  const uint32_t total = rdlane(inc, 63);
+  if (total == 0) return; // whole wavefront
+  uint32_t base = 0;
+  if (lane == 0) base = atomicAdd(count, total);
+  uint32_t pos = rdlane(base, 0) + inc - c;
+  while (mk) {
+    const uint64_t tid = pk * 64u + (uint64_t)(__ffsll((unsigned long long)mk) - 1);
+    mk &= mk - 1ull;
+    list[pos++] = idx ? idx[tid] : (uint32_t)tid;
+  }
 }
 
 template <int K, int NB>
@@ -350,6 +386,17 @@ hipError_t launch_heavy_mark(const HeavyMarkArgs& h, hipStream_t s)
   const uint64_t packets = (h.n + 63) / 64;
   if (packets == 0) return hipSuccess;
   hipLaunchKernelGGL(heavy_mark_kernel, dim3((uint32_t)((packets + 3) / 4)), dim3(256), 0, s, h);
+  return hipGetLastError();
+}
+
+hipError_t launch_recheck_compact(const uint64_t* mask, uint64_t n, const uint32_t* idx, uint32_t* list,
+                                  uint32_t* count, hipStream_t s)
+{
+  const uint64_t npk    = (n + 63) / 64;
+  const uint64_t chunks = (npk + 63) / 64;
+  if (chunks == 0) return hipSuccess;
+  hipLaunchKernelGGL(recheck_compact_kernel, dim3((uint32_t)((chunks + 3) / 4)), dim3(256), 0, s, mask, npk, idx, list,
+                     count);
   return hipGetLastError();
 }
 
