@@ -25,21 +25,37 @@ namespace {
 
 using namespace dev;
 
-// Sort key of flat query g: (item << 24) | 24-bit Hilbert index of the query on a 256^3 grid of
-// the item's box (the mesh box grown by 25%; 8 bits per axis order 64-query packets well at any
-// item size). Item found by binary search over the item bases.
-template <class KeyT>
-__global__ __launch_bounds__(256) void batch_keys_kernel(const BatchItemDev* __restrict__ items, uint32_t num_items,
-                                                         uint64_t total, KeyT* __restrict__ keys)
+// Item of flat query g (items[lo].base <= g < items[lo + 1].base): binary search over the item
+// bases; with a wave-uniform g the probes are scalar loads.
+__device__ __forceinline__ uint32_t item_of(const BatchItemDev* __restrict__ items, uint32_t num_items, uint64_t g)
 {
-  const uint64_t g = (uint64_t)blockIdx.x * 256u + threadIdx.x;
-  if (g >= total) return;
-  uint32_t lo = 0, hi = num_items; // items[lo].base <= g < items[hi].base
+  uint32_t lo = 0, hi = num_items;
   while (hi - lo > 1) {
     const uint32_t mid = (lo + hi) >> 1;
     if (items[mid].base <= g) lo = mid;
     else hi = mid;
   }
+  return lo;
+}
+
+// Sort key of flat query g: (item << 24) | 24-bit Hilbert index of the query on a 256^3 grid of
+// the item's box (the mesh box grown by 25%; 8 bits per axis order 64-query packets well at any
+// item size). The item of the wavefront's first and last query is searched once per
+// wavefront; lanes of a wavefront that straddles items step forward from the first.
+template <class KeyT>
+__global__ __launch_bounds__(256) void batch_keys_kernel(const BatchItemDev* __restrict__ items, uint32_t num_items,
+                                                         uint64_t total, KeyT* __restrict__ keys)
+{
+  const uint64_t g0 = (uint64_t)blockIdx.x * 256u + uni(threadIdx.x & ~63u);
+  if (g0 >= total) return; // whole wavefront
+  const uint64_t g     = g0 + __lane_id();
+  const uint64_t glast = min(g0 + 63u, total - 1);
+  const uint32_t lo0   = uni(item_of(items, num_items, g0));
+  const uint32_t lo1   = uni(item_of(items, num_items, glast));
+  uint32_t       lo    = lo0;
+  if (lo1 != lo0)
+    while (lo < lo1 && items[lo + 1].base <= g) lo++;
+  if (g >= total) return;
   const BatchItemDev& it = items[lo];
   const uint64_t      j  = g - it.base;
   uint32_t            x[3];
@@ -101,12 +117,9 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(K == 4 ? 8 
     it.out[qi]   = d;
     if (it.tri) it.tri[qi] = m.tris[btri].face;
   }
-  if (a.recheck) {
+  if (a.recheck_mask) {
     const uint64_t mask = ballot(found && fabsf(d) < it.recheck_abs);
-    if (mask && lane == 0) {
-      const uint32_t slot = atomicAdd(a.recheck_count, 1u);
-      a.recheck[slot]     = make_uint4(p, 0u, (uint32_t)mask, (uint32_t)(mask >> 32));
-    }
+    if (lane == 0) a.recheck_mask[p] = mask;
   }
   if (COUNT && a.work) {
     const uint64_t si = wave_sum_u64(active ? (uint64_t)n_inner : 0ull);
@@ -120,49 +133,59 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(K == 4 ? 8 
   }
 }
 
-// One wavefront per listed packet; the listed lanes redo their query in FP64, seeded with
-// (|d32| (1 + 1e-4) + slack_abs)^2, which contains the FP64 minimum.
+// One wavefront per 64 consecutive entries of the recheck list (packet << 6 | lane, in packet
+// order: neighbouring queries, mostly of one item). The wavefront takes the distinct items of its
+// entries in turn; each listed query is redone in FP64, seeded with (|d32| (1 + 1e-4) +
+// slack_abs)^2, which contains the FP64 minimum.
 template <int K, int NB>
-__device__ __forceinline__ void batch64_entry(const BatchArgs& a, uint32_t entry)
+__device__ __forceinline__ void batch64_group(const BatchArgs& a, uint32_t count, uint32_t g)
 {
-  const SWords<4>    e    = sload<4>(a.recheck + entry);
-  const uint32_t     p    = e.w[0];
-  const uint64_t     mask = (uint64_t)e.w[2] | ((uint64_t)e.w[3] << 32);
-  const SWords<4>    pk   = sload<4>(a.packets + 4 * (size_t)p);
-  const uint32_t     item = pk.w[0], first = pk.w[1];
-  const BatchItemDev it   = sload_record(a.items + item);
-  const MeshDev      m    = sload_record(a.meshes + it.mesh);
-
   const uint32_t lane   = __lane_id();
-  const bool     active = (mask >> lane) & 1ull;
-  const uint32_t j      = first + lane;
-  uint32_t       qi     = 0;
-  double         px = 0.0, py = 0.0, pz = 0.0, bound = -1.0;
-  if (active) {
-    qi               = a.sorted ? (uint32_t)(a.sorted[it.base + j] - it.base) : j;
-    px               = it.pts[3 * (size_t)qi];
-    py               = it.pts[3 * (size_t)qi + 1];
-    pz               = it.pts[3 * (size_t)qi + 2];
-    const double r   = fabs((double)it.out[qi]) * (1.0 + 1e-4) + (double)it.slack_abs;
-    bound            = r * r;
+  const uint32_t e      = g * 64u + lane;
+  const bool     listed = e < count;
+  uint32_t       item = kInvalid, j = 0;
+  if (listed) {
+    const uint32_t en  = a.recheck_list[e];
+    const size_t   pkt = en >> 6;
+    item               = a.packets[4 * pkt];
+    j                  = a.packets[4 * pkt + 1] + (en & 63u);
   }
-  double   best = INFINITY;
-  uint32_t btri = kInvalid, bfeat = 0, n_inner = 0, n_tris = 0, w_inner = 0, w_tris = 0;
-  const double seed2 = bound;
-  traverse<K, NB, double, false>(m, px, py, pz, bound, best, btri, bfeat, n_inner, n_tris, w_inner, w_tris);
-  if (active && btri != kInvalid && best <= seed2) {
-    it.out[qi] = (float)resolve64(m, btri, bfeat, best, px, py, pz);
-    if (it.tri) it.tri[qi] = m.feat[8 * (size_t)btri + 6];
+  uint64_t pending = ballot(listed);
+  while (pending) {
+    const uint32_t cur  = rdlane(item, __ffsll((unsigned long long)pending) - 1);
+    const bool     mine = listed && item == cur;
+    pending &= ~ballot(mine);
+    const BatchItemDev it = sload_record(a.items + cur);
+    const MeshDev      m  = sload_record(a.meshes + it.mesh);
+    uint32_t           qi = 0;
+    double             px = 0.0, py = 0.0, pz = 0.0, bound = -1.0;
+    if (mine) {
+      qi             = a.sorted ? (uint32_t)(a.sorted[it.base + j] - it.base) : j;
+      px             = it.pts[3 * (size_t)qi];
+      py             = it.pts[3 * (size_t)qi + 1];
+      pz             = it.pts[3 * (size_t)qi + 2];
+      const double r = fabs((double)it.out[qi]) * (1.0 + 1e-4) + (double)it.slack_abs;
+      bound          = r * r;
+    }
+    double   best = INFINITY;
+    uint32_t btri = kInvalid, bfeat = 0, n_inner = 0, n_tris = 0, w_inner = 0, w_tris = 0;
+    const double seed2 = bound;
+    traverse<K, NB, double, false>(m, px, py, pz, bound, best, btri, bfeat, n_inner, n_tris, w_inner, w_tris);
+    if (mine && btri != kInvalid && best <= seed2) {
+      it.out[qi] = (float)resolve64(m, btri, bfeat, best, px, py, pz);
+      if (it.tri) it.tri[qi] = m.feat[8 * (size_t)btri + 6];
+    }
   }
 }
 
-// One wavefront per workgroup; a fixed grid strides over the listed packets (their number is
-// only known on the device).
+// A fixed grid of wavefronts strides over the groups (their number is only known on the device).
 template <int K, int NB>
-__global__ __launch_bounds__(64) void batch64_kernel(const BatchArgs a)
+__global__ __launch_bounds__(256) void batch64_kernel(const BatchArgs a)
 {
-  const uint32_t count = *a.recheck_count;
-  for (uint32_t e = blockIdx.x; e < count; e += gridDim.x) batch64_entry<K, NB>(a, e);
+  const uint32_t count  = *a.recheck_count;
+  const uint32_t groups = (count + 63u) / 64u;
+  const uint32_t waves  = gridDim.x * 4u;
+  for (uint32_t g = uni(blockIdx.x * 4u + (threadIdx.x >> 6)); g < groups; g += waves) batch64_group<K, NB>(a, count, g);
 }
 
 // One wavefront per packet: the largest cost-grid value over its queries' cells in its mesh's
@@ -213,8 +236,9 @@ hipError_t launch32(const BatchArgs& a, int wg, bool count, hipStream_t s)
 template <int K, int NB>
 hipError_t launch64(const BatchArgs& a, hipStream_t s)
 {
-  if (a.num_packets == 0) return hipSuccess;
-  hipLaunchKernelGGL((batch64_kernel<K, NB>), dim3(std::min<uint32_t>(a.num_packets, 8192u)), dim3(64), 0, s, a);
+  if (a.num_packets == 0 || !a.recheck_list) return hipSuccess;
+  hipLaunchKernelGGL((batch64_kernel<K, NB>), dim3(std::min<uint32_t>((a.num_packets + 3) / 4, 2048u)), dim3(256), 0, s,
+                     a);
   return hipGetLastError();
 }
 

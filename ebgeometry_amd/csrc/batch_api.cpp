@@ -38,7 +38,7 @@ struct ebg_batch_plan
   ebg_batch*      b = nullptr;
   uint64_t        total = 0;
   uint32_t        num_items = 0, num_packets = 0;
-  DevBuf          items, packets, keys, keys_sorted, sorted, scratch, recheck, counter;
+  DevBuf          items, packets, keys, keys_sorted, sorted, scratch, rmask, recheck, counter;
   DevBuf          grids, flags, early; // early packets: per-mesh grid records, packet flags, list
   uint32_t        early_cap = 0;       // 0: no early packets
   int             wg = 256;
@@ -182,7 +182,8 @@ ebg_status enqueue(ebg_batch_plan* p, hipStream_t s, bool events)
   a.num_packets   = p->num_packets;
   a.meshes        = b->meshdev.as<ebgpu::MeshDev>();
   a.sorted        = nullptr;
-  a.recheck       = p->fp64 ? p->recheck.as<uint4>() : nullptr;
+  a.recheck_mask  = p->fp64 ? p->rmask.as<uint64_t>() : nullptr;
+  a.recheck_list  = p->fp64 ? p->recheck.as<uint32_t>() : nullptr;
   a.recheck_count = p->counter.as<uint32_t>();
   a.work          = p->work;
   if (p->fp64 || p->early_cap) EBG_HIP(ebgpu::launch_fill_u32(a.recheck_count, 2, 0u, s)); // recheck, early
@@ -211,7 +212,13 @@ ebg_status enqueue(ebg_batch_plan* p, hipStream_t s, bool events)
   if (events) EBG_HIP(hipEventRecord(p->ev0, s));
   EBG_HIP(ebgpu::launch_batch32(a, b->K, b->banks, p->wg, p->work != nullptr, s));
   if (events) EBG_HIP(hipEventRecord(p->ev1, s));
-  if (p->fp64) EBG_HIP(ebgpu::launch_batch64(a, b->K, b->banks, s));
+  if (p->fp64) {
+    // the list follows the packet table (presort order), so FP64 wavefronts hold neighbouring
+    // queries
+    EBG_HIP(ebgpu::launch_recheck_compact(a.recheck_mask, uint64_t(p->num_packets) * 64, nullptr, a.recheck_list,
+                                          a.recheck_count, s));
+    EBG_HIP(ebgpu::launch_batch64(a, b->K, b->banks, s));
+  }
   return EBG_OK;
 }
 
@@ -445,7 +452,13 @@ ebg_status ebg_batch_plan_create(ebg_batch* b, const ebg_batch_item* items, uint
   if ((e = p->keys_sorted.reserve(8 * cap)) != hipSuccess) return bail(e, "keys");
   if ((e = p->sorted.reserve(4 * cap)) != hipSuccess) return bail(e, "sort");
   if ((e = p->scratch.reserve(ebgpu::radix_sort_scratch_bytes(cap))) != hipSuccess) return bail(e, "sort scratch");
-  if ((e = p->recheck.reserve(16 * std::max<uint64_t>(p->num_packets, 1))) != hipSuccess) return bail(e, "recheck");
+  // recheck entries are packet << 6 | lane in 32 bits; at most one per query
+  if (p->fp64 && p->num_packets > (1u << 26)) {
+    delete p;
+    return fail(EBG_INVALID_ARG, "FP64 recheck of more than 2^26 packets in one plan");
+  }
+  if ((e = p->rmask.reserve(8 * std::max<uint64_t>(p->num_packets, 1))) != hipSuccess) return bail(e, "recheck");
+  if ((e = p->recheck.reserve(4 * (p->fp64 ? cap : 1))) != hipSuccess) return bail(e, "recheck");
   if ((e = p->counter.reserve(256)) != hipSuccess) return bail(e, "counter");
   if ((e = hipEventCreate(&p->ev0)) != hipSuccess || (e = hipEventCreate(&p->ev1)) != hipSuccess)
     return bail(e, "events");

@@ -131,7 +131,8 @@ struct BatchArgs
   uint32_t            num_packets;
   const MeshDev*      meshes;
   const uint32_t*     sorted;      // flat query indices in sort order (null: no presort)
-  uint4*              recheck;     // per listed packet: packet, 0, lane mask lo, hi
+  uint64_t*           recheck_mask; // per packet: lane mask of its queries to redo in FP64 (null = off)
+  uint32_t*           recheck_list; // packet << 6 | lane, in packet order (launch_recheck_compact)
   uint32_t*           recheck_count;
   unsigned long long* work;        // [0] inner, [1] triangles, [4]/[5] per wavefront (null = off)
   // Early packets (early_blocks = 0: none): batch_mark lists the packets whose most expensive
@@ -151,7 +152,8 @@ hipError_t launch_batch_keys(const BatchItemDev* items, uint32_t num_items, uint
                              hipStream_t s);
 // wg = 64, 128 or 256 lanes per workgroup (1, 2 or 4 packets).
 hipError_t launch_batch32(const BatchArgs& a, int K, int banks, int wg, bool count, hipStream_t s);
-// a.recheck == null skips; grid = num_packets one-wavefront blocks, those past the count exit.
+// FP64 pass over the compacted recheck list (a.recheck_list == null skips); a fixed grid
+// strides over groups of 64 entries.
 hipError_t launch_batch64(const BatchArgs& a, int K, int banks, hipStream_t s);
 
 // util.hip
