@@ -248,14 +248,16 @@ struct GemmArgs
   float*          Cf;    // nullable
   uint16_t*       Cb;    // nullable
   int64_t         ldc;
+  int             kchunk; // K per blockIdx.z (set by gemm(); more than one chunk adds into Cf)
 };
 
 __global__ __launch_bounds__(64) void gemm_kernel(GemmArgs a)
 {
   const int lane = threadIdx.x, nn = lane & 15, g = lane >> 4;
   const int i0 = blockIdx.x * 16, j0 = blockIdx.y * 16;
+  const int kb = blockIdx.z * a.kchunk, ke = min(a.K, kb + a.kchunk);
   f32x4     acc{0.0f, 0.0f, 0.0f, 0.0f};
-  for (int k0 = 0; k0 < a.K; k0 += 32) {
+  for (int k0 = kb; k0 < ke; k0 += 32) {
     u16x8 va, vb;
 #pragma unroll
     for (int e = 0; e < 8; e++) {
@@ -268,7 +270,11 @@ __global__ __launch_bounds__(64) void gemm_kernel(GemmArgs a)
 #pragma unroll
   for (int e = 0; e < 4; e++) {
     const int i = i0 + 4 * g + e, j = j0 + nn;
-    float     v = acc[e] + (a.bias ? a.bias[j] : 0.0f);
+    if (gridDim.z > 1) { // split K: plain partial products, summed into the zeroed Cf
+      atomicAdd(a.Cf + (int64_t)i * a.ldc + j, acc[e]);
+      continue;
+    }
+    float v = acc[e] + (a.bias ? a.bias[j] : 0.0f);
     if (a.mask) {
       const uint16_t h = a.mask[(int64_t)i * a.ms0 + j];
       if (h == 0 || (h & 0x8000u)) v = 0.0f;
@@ -279,9 +285,25 @@ __global__ __launch_bounds__(64) void gemm_kernel(GemmArgs a)
   }
 }
 
-hipError_t gemm(const GemmArgs& a, hipStream_t s)
+__global__ __launch_bounds__(256) void zero_rows_kernel(float* __restrict__ c, int rows, int cols, int64_t ld)
 {
-  hipLaunchKernelGGL(gemm_kernel, dim3(a.M / 16, a.N / 16), dim3(64), 0, s, a);
+  const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  if (i < (int64_t)rows * cols) c[(i / cols) * ld + i % cols] = 0.0f;
+}
+
+// Products with a long K and a plain FP32 output (the weight gradients, K = batch rows) split K
+// over blockIdx.z in chunks of kSplitK: a 128 x 128 gradient is only 64 output tiles.
+constexpr int kSplitK = 512;
+
+hipError_t gemm(const GemmArgs& a0, hipStream_t s)
+{
+  GemmArgs   a     = a0;
+  const bool plain = a.Cf && !a.Cb && !a.bias && !a.mask && !a.relu;
+  a.kchunk         = (plain && a.K >= 4 * kSplitK) ? kSplitK : a.K;
+  const int splits = (a.K + a.kchunk - 1) / a.kchunk;
+  if (splits > 1)
+    hipLaunchKernelGGL(zero_rows_kernel, dim3((a.M * a.N + 255) / 256), dim3(256), 0, s, a.Cf, a.M, a.N, a.ldc);
+  hipLaunchKernelGGL(gemm_kernel, dim3(a.M / 16, a.N / 16, splits), dim3(64), 0, s, a);
   return hipGetLastError();
 }
 
