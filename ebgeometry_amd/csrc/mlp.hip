@@ -75,8 +75,8 @@ __device__ __forceinline__ float feature(int f, float x0, float x1, float x2)
 // fragment comes from LDS instead of an L2 round trip per wavefront.
 // ------------------------------------------------------------------------------------------
 
-constexpr int kInferWaves = 8; // 2 per SIMD
-constexpr int kInferQT    = 3; // 16-query tiles per wavefront (48 queries)
+constexpr int kInferWaves = 12; // 3 per SIMD
+constexpr int kInferQT    = 2;  // 16-query tiles per wavefront (32 queries; 148 VGPRs)
 constexpr int kInferBias  = 4 * kMlpHidden; // b1..b4 in LDS
 
 // Layer-1 K order of the fused kernel: K index 32 b + 8 g + e holds, for g < 3, the Fourier
@@ -91,18 +91,6 @@ __host__ __device__ constexpr int infer_feature(int b, int g, int e)
 __device__ __forceinline__ uint32_t pk_bf16(float a, float b)
 {
   return __builtin_bit_cast(uint32_t, __builtin_convertvector((f32x2){a, b}, bf16x2));
-}
-
-// relu'd BF16 copy of a layer's accumulators as the next layer's four B operands (K order phi)
-__device__ __forceinline__ void relu_to_b(const f32x4 (&acc)[8], u16x8 (&bq)[4])
-{
-#pragma unroll
-  for (int b = 0; b < 4; b++) {
-    const u32x4v w = {relu_pk_bf16(acc[2 * b][0], acc[2 * b][1]), relu_pk_bf16(acc[2 * b][2], acc[2 * b][3]),
-                      relu_pk_bf16(acc[2 * b + 1][0], acc[2 * b + 1][1]),
-                      relu_pk_bf16(acc[2 * b + 1][2], acc[2 * b + 1][3])};
-    bq[b] = __builtin_bit_cast(u16x8, w);
-  }
 }
 
 // acc[q][t] += A(t, b) B_q(b) over the 8 NB fragments of one layer, K block b outermost so that
@@ -123,6 +111,43 @@ __device__ __forceinline__ void layer_mma(const uint16_t* __restrict__ w, int la
     const int t = f % 8, b = f / 8;
 #pragma unroll
     for (int q = 0; q < kInferQT; q++) acc[q][t] = mfma(a[f % 3], bq[q][b], acc[q][t]);
+  }
+}
+
+// One hidden layer (K = 128) from the previous layer's accumulators `prev` into `acc` (separate
+// registers): K block b's B operands (relu'd BF16 of prev tiles 2b, 2b+1) are packed right
+// before the MFMAs that read them, so that the packing of block b+1 can be scheduled under the
+// MFMAs of block b.
+__device__ __forceinline__ void dense_layer(const uint16_t* __restrict__ w, const float* __restrict__ bias, int lane,
+                                            const f32x4 (&prev)[kInferQT][8], f32x4 (&acc)[kInferQT][8])
+{
+  const int g = lane >> 4;
+#pragma unroll
+  for (int t = 0; t < 8; t++) {
+    const f32x4 bv = *reinterpret_cast<const f32x4*>(bias + 16 * t + 4 * g);
+#pragma unroll
+    for (int q = 0; q < kInferQT; q++) acc[q][t] = bv;
+  }
+  auto  frag = [&](int f) { return *reinterpret_cast<const u16x8*>(w + (((f % 8) * 4 + f / 8) * 64 + lane) * 8); };
+  u16x8 a[3], bq[kInferQT];
+  a[0] = frag(0);
+  a[1] = frag(1);
+#pragma unroll
+  for (int f = 0; f < 32; f++) {
+    const int t = f % 8, b = f / 8;
+    if (t == 0) {
+#pragma unroll
+      for (int q = 0; q < kInferQT; q++) {
+        const u32x4v v = {relu_pk_bf16(prev[q][2 * b][0], prev[q][2 * b][1]),
+                          relu_pk_bf16(prev[q][2 * b][2], prev[q][2 * b][3]),
+                          relu_pk_bf16(prev[q][2 * b + 1][0], prev[q][2 * b + 1][1]),
+                          relu_pk_bf16(prev[q][2 * b + 1][2], prev[q][2 * b + 1][3])};
+        bq[q]          = __builtin_bit_cast(u16x8, v);
+      }
+    }
+    if (f + 2 < 32) a[(f + 2) % 3] = frag(f + 2);
+#pragma unroll
+    for (int q = 0; q < kInferQT; q++) acc[q][t] = mfma(a[f % 3], bq[q], acc[q][t]);
   }
 }
 
@@ -160,35 +185,23 @@ __device__ __forceinline__ void infer_tile(const uint16_t* __restrict__ sw, cons
     }
   }
 
-  // each layer's accumulators start from its bias (output rows 16 t + 4 g .. + 3 of this lane)
-  f32x4 acc[kInferQT][8];
+  // each layer's accumulators start from its bias (output rows 16 t + 4 g .. + 3 of this lane);
+  // layers alternate between two accumulator sets
+  f32x4 accA[kInferQT][8], accB[kInferQT][8];
 #pragma unroll
   for (int t = 0; t < 8; t++) {
     const f32x4 bv = *reinterpret_cast<const f32x4*>(sb + 16 * t + 4 * g);
 #pragma unroll
-    for (int q = 0; q < kInferQT; q++) acc[q][t] = bv;
+    for (int q = 0; q < kInferQT; q++) accA[q][t] = bv;
   }
   // layer 1 (K = 64)
-  layer_mma<2>(sw + kBfW1, lane, acc, bq);
+  layer_mma<2>(sw + kBfW1, lane, accA, bq);
   // layers 2..4 (K = 128)
-  const int wofs[3] = {kBfW2, kBfW3, kBfW4};
-#pragma unroll
-  for (int L = 0; L < 3; L++) {
-    // the previous layer's output, relu'd and rounded to BF16, is this layer's B operand
-#pragma unroll
-    for (int q = 0; q < kInferQT; q++) relu_to_b(acc[q], bq[q]);
-#pragma unroll
-    for (int t = 0; t < 8; t++) {
-      const f32x4 bv = *reinterpret_cast<const f32x4*>(sb + (L + 1) * kMlpHidden + 16 * t + 4 * g);
-#pragma unroll
-      for (int q = 0; q < kInferQT; q++) acc[q][t] = bv;
-    }
-    layer_mma<4>(sw + wofs[L], lane, acc, bq);
-  }
+  dense_layer(sw + kBfW2, sb + 1 * kMlpHidden, lane, accA, accB);
+  dense_layer(sw + kBfW3, sb + 2 * kMlpHidden, lane, accB, accA);
+  dense_layer(sw + kBfW4, sb + 3 * kMlpHidden, lane, accA, accB);
   // output layer y = relu(h4) . w5 + b5 on MFMA: h4 rounded to BF16 as in training; rows 0..2 of
   // the W5 fragments are w5's three BF16 parts, so lanes g = 0 receive b5 + hi.h, mid.h, lo.h
-#pragma unroll
-  for (int q = 0; q < kInferQT; q++) relu_to_b(acc[q], bq[q]);
   f32x4 y[kInferQT];
 #pragma unroll
   for (int q = 0; q < kInferQT; q++) y[q] = (f32x4){b5, 0.0f, 0.0f, 0.0f};
@@ -196,7 +209,13 @@ __device__ __forceinline__ void infer_tile(const uint16_t* __restrict__ sw, cons
   for (int b = 0; b < 4; b++) {
     const u16x8 a = *reinterpret_cast<const u16x8*>(sw + kBfW5 + (b * 64 + lane) * 8);
 #pragma unroll
-    for (int q = 0; q < kInferQT; q++) y[q] = mfma(a, bq[q][b], y[q]);
+    for (int q = 0; q < kInferQT; q++) {
+      const u32x4v v = {relu_pk_bf16(accB[q][2 * b][0], accB[q][2 * b][1]),
+                        relu_pk_bf16(accB[q][2 * b][2], accB[q][2 * b][3]),
+                        relu_pk_bf16(accB[q][2 * b + 1][0], accB[q][2 * b + 1][1]),
+                        relu_pk_bf16(accB[q][2 * b + 1][2], accB[q][2 * b + 1][3])};
+      y[q]           = mfma(a, __builtin_bit_cast(u16x8, v), y[q]);
+    }
   }
 #pragma unroll
   for (int q = 0; q < kInferQT; q++) {
