@@ -137,12 +137,13 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(K == 4 ? 8 
 // order: neighbouring queries, mostly of one item). The wavefront takes the distinct items of its
 // entries in turn; each listed query is redone in FP64, seeded with (|d32| (1 + 1e-4) +
 // slack_abs)^2, which contains the FP64 minimum.
+constexpr uint32_t kBatch64Lanes = 64; // list entries per wavefront (32 measured the same on C3)
 template <int K, int NB>
 __device__ __forceinline__ void batch64_group(const BatchArgs& a, uint32_t count, uint32_t g)
 {
   const uint32_t lane   = __lane_id();
-  const uint32_t e      = g * 64u + lane;
-  const bool     listed = e < count;
+  const uint32_t e      = g * kBatch64Lanes + lane;
+  const bool     listed = lane < kBatch64Lanes && e < count;
   uint32_t       item = kInvalid, j = 0;
   if (listed) {
     const uint32_t en  = a.recheck_list[e];
@@ -183,7 +184,7 @@ template <int K, int NB>
 __global__ __launch_bounds__(256) void batch64_kernel(const BatchArgs a)
 {
   const uint32_t count  = *a.recheck_count;
-  const uint32_t groups = (count + 63u) / 64u;
+  const uint32_t groups = (count + kBatch64Lanes - 1u) / kBatch64Lanes;
   const uint32_t waves  = gridDim.x * 4u;
   for (uint32_t g = uni(blockIdx.x * 4u + (threadIdx.x >> 6)); g < groups; g += waves) batch64_group<K, NB>(a, count, g);
 }

@@ -197,12 +197,22 @@ __device__ __forceinline__ void scene64_block(const SceneDev& sc, const QueryArg
   if (active && v < seed) a.out_d[qi] = (float)v;
 }
 
-// The list's length is only known on the device: a fixed grid strides over it.
+// The list's length is only known on the device: a fixed grid of wavefronts strides over it,
+// kScene64Lanes entries per wavefront (the other lanes sit out). The list holds near-surface
+// queries from all over the scene, so a wavefront walks about one path per entry; with 32
+// entries the slowest wavefront, which ends the launch, ends earlier (C5: 0.98 -> 0.79 ms; 16
+// entries 0.81 ms).
+constexpr uint32_t kScene64Lanes = 32;
 template <int KM, int NB>
 __global__ __launch_bounds__(256) void scene64_kernel(const SceneDev sc, const QueryArgs a)
 {
-  const uint64_t n = a.count_ptr ? (uint64_t)*a.count_ptr : a.n;
-  for (uint64_t b = blockIdx.x; b * 256u < n; b += gridDim.x) scene64_block<KM, NB>(sc, a, n, b * 256u + threadIdx.x);
+  const uint64_t n     = a.count_ptr ? (uint64_t)*a.count_ptr : a.n;
+  const uint32_t lane  = __lane_id();
+  const uint64_t waves = (uint64_t)gridDim.x * 4u;
+  for (uint64_t w = uni(blockIdx.x * 4u + (threadIdx.x >> 6)); w * kScene64Lanes < n; w += waves) {
+    const uint64_t tid = lane < kScene64Lanes ? w * kScene64Lanes + lane : n; // n: inactive
+    scene64_block<KM, NB>(sc, a, n, tid);
+  }
 }
 
 template <int KM, int NB>

@@ -296,12 +296,18 @@ __device__ __forceinline__ void sdf64_block(const MeshDev& m, const QueryArgs& a
 }
 
 // FP64 pass over a device-side list (a.idx, count *a.count_ptr) or over all a.n queries; a fixed
-// grid strides over the entries.
+// grid of wavefronts strides over the entries, kSdf64Lanes per wavefront (the other lanes sit
+// out). Near-surface queries are sparse even in presort order, so a wavefront walks several
+// paths; with 32 entries its slowest wavefronts end earlier (C2: 0.31 -> 0.23 ms).
+constexpr uint32_t kSdf64Lanes = 32;
 template <int K, int NB>
 __global__ __launch_bounds__(256) void sdf64_kernel(const MeshDev m, const QueryArgs a)
 {
-  const uint64_t n = a.count_ptr ? (uint64_t)*a.count_ptr : a.n;
-  for (uint64_t b = blockIdx.x; b * 256u < n; b += gridDim.x) sdf64_block<K, NB>(m, a, n, b * 256u + threadIdx.x);
+  const uint64_t n     = a.count_ptr ? (uint64_t)*a.count_ptr : a.n;
+  const uint32_t lane  = __lane_id();
+  const uint64_t waves = (uint64_t)gridDim.x * 4u;
+  for (uint64_t w = uni(blockIdx.x * 4u + (threadIdx.x >> 6)); w * kSdf64Lanes < n; w += waves)
+    sdf64_block<K, NB>(m, a, n, lane < kSdf64Lanes ? w * kSdf64Lanes + lane : n);
 }
 
 template <int K, int NB, int MODE>
