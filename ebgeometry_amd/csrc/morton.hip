@@ -126,19 +126,11 @@ __global__ __launch_bounds__(256) void codes_f32(const float* __restrict__ p, ui
 // evaluations per wavefront).
 // With FLAGS, bits 31 / 30 carry the query's cost flags (see CostFlags).
 template <bool FLAGS>
-__global__ __launch_bounds__(256) void hilbert_f32(const float* __restrict__ p, uint64_t n,
-                                                   const float* __restrict__ box, uint32_t* __restrict__ keys,
-                                                   const CostFlags cf)
+__device__ __forceinline__ uint32_t hilbert_key(const float (&q)[3], const float* __restrict__ box, const CostFlags& cf)
 {
-  const uint64_t i = (uint64_t)blockIdx.x * 256 + threadIdx.x;
-  if (i >= n) return;
-  float    q[3];
   uint32_t x[3];
 #pragma unroll
-  for (int c = 0; c < 3; c++) {
-    q[c] = p[3 * i + c];
-    x[c] = dev::hilbert_bin<10>(q[c], box[c], box[3 + c] - box[c]);
-  }
+  for (int c = 0; c < 3; c++) x[c] = dev::hilbert_bin<10>(q[c], box[c], box[3 + c] - box[c]);
   uint32_t key = (uint32_t)dev::hilbert_index<10>(x[0], x[1], x[2]);
   if (FLAGS) {
     uint32_t cell[3];
@@ -148,7 +140,30 @@ __global__ __launch_bounds__(256) void hilbert_f32(const float* __restrict__ p, 
     const uint32_t cost = cf.grid[((size_t)cell[2] * cf.g + cell[1]) * cf.g + cell[0]];
     key |= (cost >= cf.heavy ? 1u << 31 : 0u) | (cost >= cf.early ? 1u << 30 : 0u);
   }
-  keys[i] = key;
+  return key;
+}
+
+// Four points per thread: 48 contiguous bytes read as three 16-byte loads (when the points are
+// 16-byte aligned) and four keys stored at once.
+template <bool FLAGS>
+__global__ __launch_bounds__(256) void hilbert_f32(const float* __restrict__ p, uint64_t n,
+                                                   const float* __restrict__ box, uint32_t* __restrict__ keys,
+                                                   const CostFlags cf)
+{
+  const uint64_t i0 = ((uint64_t)blockIdx.x * 256 + threadIdx.x) * 4u;
+  if (i0 >= n) return;
+  if (i0 + 4u <= n && (reinterpret_cast<uintptr_t>(p) & 15u) == 0) {
+    const float4* v = reinterpret_cast<const float4*>(p + 3 * i0);
+    const float4  a = v[0], b = v[1], c = v[2];
+    const float   q0[3] = {a.x, a.y, a.z}, q1[3] = {a.w, b.x, b.y}, q2[3] = {b.z, b.w, c.x}, q3[3] = {c.y, c.z, c.w};
+    *reinterpret_cast<uint4*>(keys + i0) = make_uint4(hilbert_key<FLAGS>(q0, box, cf), hilbert_key<FLAGS>(q1, box, cf),
+                                                      hilbert_key<FLAGS>(q2, box, cf), hilbert_key<FLAGS>(q3, box, cf));
+    return;
+  }
+  for (uint64_t i = i0; i < n && i < i0 + 4u; i++) {
+    const float q[3] = {p[3 * i], p[3 * i + 1], p[3 * i + 2]};
+    keys[i]          = hilbert_key<FLAGS>(q, box, cf);
+  }
 }
 
 __global__ __launch_bounds__(256) void codes_f64(const double* __restrict__ p, uint64_t n, const double* __restrict__ box,
@@ -400,7 +415,7 @@ hipError_t launch_hilbert_f32(const float* pts, uint64_t n, const float* box, ui
                               const CostFlags* cf)
 {
   if (n == 0) return hipSuccess;
-  const dim3 grid((uint32_t)((n + 255) / 256));
+  const dim3 grid((uint32_t)((n + 1023) / 1024));
   if (cf) hipLaunchKernelGGL(hilbert_f32<true>, grid, dim3(256), 0, s, pts, n, box, keys, *cf);
   else hipLaunchKernelGGL(hilbert_f32<false>, grid, dim3(256), 0, s, pts, n, box, keys, CostFlags{});
   return hipGetLastError();

@@ -25,6 +25,17 @@ __device__ __forceinline__ uint64_t spread3_21(uint64_t a)
   return b;
 }
 
+// the same interleave for values of at most 10 bits, in 32-bit arithmetic
+__device__ __forceinline__ uint32_t spread3_10(uint32_t a)
+{
+  uint32_t b = a & 0x3ffu;
+  b          = (b | b << 16) & 0x030000ffu;
+  b          = (b | b << 8) & 0x0300f00fu;
+  b          = (b | b << 4) & 0x030c30c3u;
+  b          = (b | b << 2) & 0x09249249u;
+  return b;
+}
+
 template <int B>
 __device__ __forceinline__ uint64_t hilbert_index(uint32_t x0, uint32_t x1, uint32_t x2)
 {
@@ -48,14 +59,18 @@ __device__ __forceinline__ uint64_t hilbert_index(uint32_t x0, uint32_t x1, uint
 #pragma unroll
   for (uint32_t q = 1u << (B - 1); q > 1u; q >>= 1)
     if (x[2] & q) t ^= q - 1u;
-  return (spread3_21(x[0] ^ t) << 2) | (spread3_21(x[1] ^ t) << 1) | spread3_21(x[2] ^ t);
+  if constexpr (B <= 10)
+    return (uint64_t)((spread3_10(x[0] ^ t) << 2) | (spread3_10(x[1] ^ t) << 1) | spread3_10(x[2] ^ t));
+  else
+    return (spread3_21(x[0] ^ t) << 2) | (spread3_21(x[1] ^ t) << 1) | spread3_21(x[2] ^ t);
 }
 
-// Grid cell of coordinate v in [lo, lo + ext) at 2^B cells per axis (clamped; NaN -> 0).
+// Grid cell of coordinate v in [lo, lo + ext) at 2^B cells per axis (clamped; NaN -> 0). The
+// cell size comes from the hardware reciprocal: the presort only needs a deterministic binning.
 template <int B>
 __device__ __forceinline__ uint32_t hilbert_bin(float v, float lo, float ext)
 {
-  const float f = ext > 0.0f ? (v - lo) * (float(1u << B) / ext) : 0.0f;
+  const float f = ext > 0.0f ? (v - lo) * (float(1u << B) * __builtin_amdgcn_rcpf(ext)) : 0.0f;
   return (uint32_t)fminf(fmaxf(f, 0.0f), float((1u << B) - 1u));
 }
 
