@@ -122,7 +122,7 @@ __global__ __launch_bounds__(256) void codes_f32(const float* __restrict__ p, ui
 
 // Query presort key: position on a 3-D Hilbert curve over a 1024^3 grid of the query box (30
 // bits). Consecutive cells of the curve share a face, so a run of 64 sorted queries never jumps
-// across the box the way a Morton run can at a high-level boundary (C2: 10% fewer triangle
+// across the box the way a Morton run can at a high-level boundary (16M queries on a 327,680-face displaced sphere: 10% fewer triangle
 // evaluations per wavefront).
 // With FLAGS, bits 31 / 30 carry the query's cost flags (see CostFlags).
 template <bool FLAGS>

@@ -200,7 +200,7 @@ __device__ __forceinline__ void scene64_block(const SceneDev& sc, const QueryArg
 // The list's length is only known on the device: a fixed grid of wavefronts strides over it,
 // kScene64Lanes entries per wavefront (the other lanes sit out). The list holds near-surface
 // queries from all over the scene, so a wavefront walks about one path per entry; with 32
-// entries the slowest wavefront, which ends the launch, ends earlier (C5: 0.98 -> 0.79 ms; 16
+// entries the slowest wavefront, which ends the launch, ends earlier (32M queries, 4,096 spheres + 1,024 boxes + 8 meshes: 0.98 -> 0.79 ms; 16
 // entries 0.81 ms).
 constexpr uint32_t kScene64Lanes = 32;
 template <int KM, int NB>

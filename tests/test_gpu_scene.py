@@ -90,7 +90,7 @@ def test_large_scene_matches_oracle(cuda):
     _, _, diag = u.bounds()
     o = oracle_scene(sc)
     q = np.concatenate([wl.uniform_box(30000, [0.0] * 3, [10.0] * 3, 6), near_sphere_points(sc, 10000, 7)])
-    check(u.value(q), o.value(q.astype(np.float64)), diag, "C5-shaped scene")
+    check(u.value(q), o.value(q.astype(np.float64)), diag, "spheres + boxes + meshes scene")
 
 
 def test_near_mesh_instances(cuda):
