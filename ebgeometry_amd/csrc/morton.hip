@@ -143,8 +143,8 @@ __device__ __forceinline__ uint32_t hilbert_key(const float (&q)[3], const float
   return key;
 }
 
-// Four points per thread: 48 contiguous bytes read as three 16-byte loads (when the points are
-// 16-byte aligned) and four keys stored at once.
+// Four points per thread: 48 contiguous bytes read as three 16-byte loads and four keys stored
+// at once (when both arrays are 16-byte aligned; otherwise point by point).
 template <bool FLAGS>
 __global__ __launch_bounds__(256) void hilbert_f32(const float* __restrict__ p, uint64_t n,
                                                    const float* __restrict__ box, uint32_t* __restrict__ keys,
@@ -152,7 +152,7 @@ __global__ __launch_bounds__(256) void hilbert_f32(const float* __restrict__ p, 
 {
   const uint64_t i0 = ((uint64_t)blockIdx.x * 256 + threadIdx.x) * 4u;
   if (i0 >= n) return;
-  if (i0 + 4u <= n && (reinterpret_cast<uintptr_t>(p) & 15u) == 0) {
+  if (i0 + 4u <= n && ((reinterpret_cast<uintptr_t>(p) | reinterpret_cast<uintptr_t>(keys)) & 15u) == 0) {
     const float4* v = reinterpret_cast<const float4*>(p + 3 * i0);
     const float4  a = v[0], b = v[1], c = v[2];
     const float   q0[3] = {a.x, a.y, a.z}, q1[3] = {a.w, b.x, b.y}, q2[3] = {b.z, b.w, c.x}, q3[3] = {c.y, c.z, c.w};

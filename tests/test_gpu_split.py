@@ -102,3 +102,24 @@ def test_split_on_tiny_meshes(cuda, golden, name):
     assert np.abs(d - ref).max() <= 1e-5 * diag
     nz = ref != 0
     assert np.array_equal(np.sign(d[nz]), np.sign(ref[nz]))
+
+
+def test_unaligned_points_give_the_same_results(cuda):
+    """A point array that starts 12 bytes into its allocation (a view from the second point) is
+    read point by point by the presort key kernel; results must equal those of a contiguous,
+    aligned copy of the same points, with and without the FP64 recheck."""
+    import torch
+    v, f = wl.displaced_icosphere(5, 21)
+    m = TriMeshSDF(v, f, branching=4, leaf_size=16)
+    n = 70_001  # above the split size, not a multiple of 4, 64 or 4096
+    q = np.concatenate([deep_queries(n, 8), wl.mixed_queries(n - n // 2 + 1, v, f, 9)])
+    base = torch.from_numpy(q).to(cuda)
+    view = base[1:]
+    assert view.data_ptr() % 16 != 0
+    copy = view.clone()
+    assert copy.data_ptr() % 16 == 0
+    for kw in ({}, {"fp64_recheck": False}):
+        d0 = m.query(view, **kw)
+        d1 = m.query(copy, **kw)
+        torch.cuda.synchronize()
+        assert torch.equal(d0, d1), kw
