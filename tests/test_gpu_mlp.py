@@ -62,12 +62,15 @@ def test_forward_matches_oracle(cuda, mesh):
     assert np.abs(pred - ref32).max() <= 5e-2 * scale
 
 
-def test_gradient_matches_oracle(cuda, mesh):
+@pytest.mark.parametrize("samples", [1000, 4100])
+def test_gradient_matches_oracle(cuda, mesh, samples):
+    """1000 samples: each weight gradient is one K loop per output tile; 4100 samples (K = 4128
+    rows): the weight gradients split K over the grid and add with atomics."""
     import torch
     m, v, f = mesh
     theta = small_theta(2)
     m.set_bound_params(theta)
-    q = wl.mixed_queries(1000, v, f, 6)
+    q = wl.mixed_queries(samples, v, f, 6)
     d_ref, _ = port.Mesh(v.astype(np.float64), f).sdf(q.astype(np.float64))
     qt = torch.from_numpy(q).to(cuda)
     tt = torch.from_numpy(np.abs(d_ref).astype(np.float32)).to(cuda)
