@@ -10,6 +10,20 @@ surface results (|d| < 1e-4 diag) are redone in FP64. The learned bound is train
 One step = one full query batch (all 16M points, inputs resident in HBM). `value` is the
 learned-bound throughput; the plain traversal's throughput is reported beside it.
 
+The same JSON line also carries, measured in the same run:
+  correctness  the GPU results (plain and learned) against the reference's CPU SDF on the
+               sample the CPU baseline times (|d - d_ref| / diag, signs), and the closest faces
+               against getClosestTriangle on a sub-sample (differences split into exact ties and
+               others); the run exits with status 3 if the 1e-5 diag bar or a sign fails;
+  c1           the 20,480-face unit icosphere with 1M queries uniform in [-1.5,1.5]^3 (seed 0):
+               the reference on all host cores over all 1M queries, the GPU beside it;
+  batched      the 256-mesh workload (seed 3) through one captured plan;
+  learned_c1   the learned bound on the C1 mesh (4M training pairs, 16M queries);
+  scene        the CSG union (4,096 spheres, 1,024 boxes, 8 meshes; 32M queries).
+`roofline` is priced on the bytes the packet traversal fetches (each record once per
+wavefront of 64 queries), against the HBM peak; the rocprofv3 summaries under profiles/ give
+the measured DRAM traffic and the counters of the resource that binds (instruction issue).
+
     python bench.py [--steps K] [--warmup W] [--gpus 1] [--skip-cpu] [--workload NAME]
 """
 from __future__ import annotations
@@ -46,6 +60,8 @@ def parse():
     ap.add_argument("--branching", type=int, default=4)
     ap.add_argument("--leaf", type=int, default=16)
     ap.add_argument("--no-scene", action="store_true", help="skip the CSG scene measurement")
+    ap.add_argument("--no-batched", action="store_true", help="skip the 256-mesh batch measurement")
+    ap.add_argument("--no-learned-c1", action="store_true", help="skip the learned bound on the C1 mesh")
     ap.add_argument("--plain-only", action="store_true",
                     help="profiling aid: time only the plain pipeline (no training, no learned bound)")
     ap.add_argument("--batched", action="store_true",
@@ -102,45 +118,195 @@ def copy_calibration(gib=2):
     print(json.dumps({"copy_bytes_per_launch": n, "launches": 3}), flush=True)
 
 
-def cpu_baseline(v, f, q, budget_s=12.0):
-    """Time the reference's own CPU SDF (compiled from its sources) on a bounded sample."""
-    nthreads = len(os.sched_getaffinity(0))
-    try:
-        from oracle import ref
-        variants = ref.variants()
-    except Exception:
-        variants = []
-    if variants:
-        var = variants[0]
-        info = ref.info(var)
+class CpuReference:
+    """The reference's own CPU SDF (compiled from its sources into oracle/_ref), or where that
+    library is absent the C restatement of its algorithm, built over (v, f) in FP64 from the same
+    float32 vertices. Queries split statically over all host cores."""
+
+    def __init__(self, v, f):
+        self.nthreads = len(os.sched_getaffinity(0))
+        try:
+            from oracle import ref
+            variants = ref.variants()
+        except Exception:
+            variants = []
         t0 = time.perf_counter()
-        mesh = ref.Mesh(v.astype(np.float64), f, max_leaf_groups=4, variant=var)
-        build_s = time.perf_counter() - t0
-        run = lambda p: mesh.sdf(p, nthreads=nthreads)  # noqa: E731
-        kind = "reference"
-        label = f"TriMeshSDF<double,uint32_t,K={info['K']},W={info['W']}> ({var} build, SAH, 4 leaf groups)"
-    else:
-        from oracle import port
-        t0 = time.perf_counter()
-        mesh = port.Mesh(v.astype(np.float64), f, K=4, W=4, max_leaf_groups=4)
-        build_s = time.perf_counter() - t0
-        run = lambda p: mesh.sdf(p, nthreads=nthreads)[0]  # noqa: E731
-        kind = "port"
-        label = "C restatement of the reference algorithm (K=4, W=4)"
+        if variants:
+            var = variants[0]
+            info = ref.info(var)
+            self.mesh = ref.Mesh(v.astype(np.float64), f, max_leaf_groups=4, variant=var)
+            self.kind = "reference"
+            self.label = f"TriMeshSDF<double,uint32_t,K={info['K']},W={info['W']}> ({var} build, SAH, 4 leaf groups)"
+        else:
+            from oracle import port
+            self.mesh = port.Mesh(v.astype(np.float64), f, K=4, W=4, max_leaf_groups=4)
+            self.kind = "port"
+            self.label = "C restatement of the reference algorithm (K=4, W=4)"
+        self.build_s = time.perf_counter() - t0
+
+    def sdf(self, p):
+        r = self.mesh.sdf(np.asarray(p, np.float64), nthreads=self.nthreads)
+        return r if self.kind == "reference" else r[0]
+
+    def closest(self, p):
+        """(d, face) -- getClosestTriangle (reference) or the restatement's closest face."""
+        p = np.asarray(p, np.float64)
+        if self.kind == "reference":
+            return self.mesh.closest(p, nthreads=self.nthreads)
+        return self.mesh.sdf(p, nthreads=self.nthreads)
+
+
+def cpu_baseline(cref, q, budget_s=12.0):
+    """Time the CPU reference on a bounded strided sample of the workload (about budget_s of
+    work on all host cores). Returns (baseline record, sample indices, reference distances)."""
     # calibrate on a strided slice, then run a sample sized for ~budget_s of CPU work
-    stride_q = lambda m: q[:: max(1, q.shape[0] // m)][:m].astype(np.float64)  # noqa: E731
-    p = stride_q(20_000)
+    stride = lambda m: np.arange(0, q.shape[0], max(1, q.shape[0] // m))[:m]  # noqa: E731
+    idx = stride(20_000)
     t0 = time.perf_counter()
-    run(p)
-    rate = p.shape[0] / (time.perf_counter() - t0)
+    cref.sdf(q[idx])
+    rate = idx.size / (time.perf_counter() - t0)
     m = int(min(q.shape[0], max(20_000, rate * budget_s)))
-    p = stride_q(m)
+    idx = stride(m)
     t0 = time.perf_counter()
-    run(p)
+    d_ref = cref.sdf(q[idx])
     dt = time.perf_counter() - t0
-    return {"value": p.shape[0] / dt, "unit": "queries/s", "cores": nthreads, "kind": kind,
-            "sample": f"{p.shape[0]} queries (every {max(1, q.shape[0] // p.shape[0])}th of the workload, same "
-                      f"float32 points promoted to double), {label}, build {build_s:.2f}s not timed"}
+    rec = {"value": idx.size / dt, "unit": "queries/s", "cores": cref.nthreads, "kind": cref.kind,
+           "sample": f"{idx.size} queries (every {max(1, q.shape[0] // idx.size)}th of the workload, same "
+                     f"float32 points promoted to double), {cref.label}, build {cref.build_s:.2f}s not timed"}
+    return rec, idx, d_ref
+
+
+def face_check(v, f, q, d_ref, tri_ref, tri, diag):
+    """Closest faces against the reference's: where they differ, the GPU face's own FP64
+    distance (the oracle's lane kernel) is compared with the minimum. Exact ties: within 1e-12
+    diag (faces meeting at the closest vertex or edge); the rest is reported."""
+    from oracle import port
+    bad = np.nonzero(tri != tri_ref)[0]
+    out = {"queries": int(q.shape[0]), "differing": int(bad.size), "exact_ties": 0, "other": 0,
+           "max_other_gap_over_diag": 0.0}
+    if bad.size == 0:
+        return out
+    om = port.Mesh(v.astype(np.float64), f)
+    fn, vn, en = om.normals()
+    vv, ff = v.astype(np.float64), f.astype(np.int64)
+    for i in bad:
+        t = int(tri[i])
+        a = abs(port.triangle_sdf(vv[ff[t]].ravel(), fn[t], vn[t], en[t], q[i].astype(np.float64)))
+        gap = a - abs(d_ref[i])
+        if gap <= 1e-12 * diag + 4e-16 * abs(d_ref[i]):
+            out["exact_ties"] += 1
+        else:
+            out["other"] += 1
+            out["max_other_gap_over_diag"] = max(out["max_other_gap_over_diag"], gap / diag)
+    return out
+
+
+def compare(d, d_ref, diag):
+    """|d - d_ref| / diag and sign agreement (where d_ref != 0) of a GPU result on a sample."""
+    err = np.abs(d.astype(np.float64) - d_ref)
+    nz = d_ref != 0
+    return {"max_abs_err_over_diag": float(err.max() / diag) if err.size else 0.0,
+            "sign_mismatches": int(np.count_nonzero(np.sign(d[nz]) != np.sign(d_ref[nz]))),
+            "within_1e-5_diag": bool(np.all(err <= 1e-5 * diag))}
+
+
+def packet_roofline(n, w, K, kernel_ms, kernel, traffic):
+    """Roofline of a packet-traversal launch, priced on the bytes it must fetch: per wavefront,
+    each interior node record (32 K bytes: K child boxes, references and counts) and each
+    triangle record it evaluates (128 B), fetched once for the wavefront's 64 queries, plus per
+    query its point (12 B) and its result (4 B). The counts come from the counting variant of
+    the same launch (work[4], work[5]). The per-query figure beside it (every query's own node
+    and triangle reads, SURVEY-style B_q) is what 64 separate traversals would read; a packet
+    shares those reads, so that figure is not an HBM rate and is reported for reference only."""
+    byts = 16 * n + w[4] * 32 * K + w[5] * 128
+    gbs = byts / (kernel_ms * 1e-3) / 1e9
+    lane = 16 * n + w[0] * 28 * K + w[1] * 36
+    return {"bound": "hbm", "achieved": round(gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": round(gbs / HBM_PEAK_GBS, 4), "traffic": traffic, "kernel": kernel,
+            "kernel_ms": round(kernel_ms, 3), "algorithmic_bytes_per_launch": int(byts),
+            "per_wavefront": {"interior_nodes": round(w[4] / (n / 64), 1), "triangles": round(w[5] / (n / 64), 1)},
+            "per_query": {"interior_nodes": round(w[0] / n, 2), "triangles": round(w[1] / n, 2),
+                          "fp64_rechecked": w[2], "fallbacks": w[3],
+                          "fallback_interior_nodes": round(w[6] / n, 3), "fallback_triangles": round(w[7] / n, 3)},
+            "per_query_bytes_if_unshared_GBs": round(lane / (kernel_ms * 1e-3) / 1e9, 1)}
+
+
+def c1_bench(a, dev, cref_cls):
+    """The 20,480-face unit icosphere (level 5) with 1,000,000 queries uniform in [-1.5,1.5]^3
+    (seed 0): the reference's CPU SDF over all of them on every host core, the GPU pipeline
+    beside it, and the GPU results against the reference's."""
+    import torch
+
+    from ebgeometry_amd import TriMeshSDF
+    from ebgeometry_amd import workloads as wl
+
+    v, f = wl.icosphere(5)
+    q = wl.uniform_box(1_000_000, [-1.5] * 3, [1.5] * 3, 0)
+    diag = float(np.linalg.norm(v.max(0).astype(np.float64) - v.min(0)))
+    mesh = TriMeshSDF(v, f, branching=a.branching, leaf_size=a.leaf)
+    qt = torch.from_numpy(q).to(dev)
+    out = torch.empty(q.shape[0], dtype=torch.float32, device=dev)
+    dt, _ = timed(lambda: mesh.query(qt, out=out), a.steps, a.warmup)
+    res = {"workload": "unit-icosphere-L5-1M", "faces": int(f.shape[0]), "queries": int(q.shape[0]),
+           "gpu": {"value": round(q.shape[0] / dt, 1), "unit": "queries/s", "ms_per_step": round(dt * 1e3, 3)}}
+    if a.skip_cpu:
+        return res
+    cref = cref_cls(v, f)
+    times = []
+    for _ in range(3):
+        t0 = time.perf_counter()
+        d_ref = cref.sdf(q)
+        times.append(time.perf_counter() - t0)
+    med = float(np.median(times))
+    res["cpu"] = {"value": round(q.shape[0] / med, 1), "unit": "queries/s", "cores": cref.nthreads,
+                  "kind": cref.kind, "sample": f"all {q.shape[0]} queries, median of 3 runs, {cref.label}",
+                  "build_seconds": round(cref.build_s, 3), "query_seconds": round(med, 3)}
+    res["correctness"] = compare(out.cpu().numpy(), d_ref, diag)
+    return res
+
+
+def learned_c1_bench(a, dev):
+    """The learned bound as C4 is written: the C1 mesh (unit icosphere, 20,480 faces), the network
+    trained on 4,000,000 BVH-labelled pairs drawn by the 16M-query law around this mesh (half
+    uniform in its 1.2x bbox, half near the surface, N(0, 0.02)), evaluated on 16,000,000 queries
+    drawn by the same law (another seed), with and without the bound."""
+    import torch
+
+    from ebgeometry_amd import TriMeshSDF
+    from ebgeometry_amd import workloads as wl
+
+    v, f = wl.icosphere(5)
+    q = wl.mixed_queries(a.queries, v, f, 2)
+    n = q.shape[0]
+    diag = float(np.linalg.norm(v.max(0).astype(np.float64) - v.min(0)))
+    mesh = TriMeshSDF(v, f, branching=a.branching, leaf_size=a.leaf)
+    t0 = time.perf_counter()
+    loss = mesh.train_bound(num_samples=a.train_samples, epochs=a.epochs, seed=3)
+    train_s = time.perf_counter() - t0
+    qt = torch.from_numpy(q).to(dev)
+    out = torch.empty(n, dtype=torch.float32, device=dev)
+    out_l = torch.empty(n, dtype=torch.float32, device=dev)
+    t_plain, _ = timed(lambda: mesh.query(qt, out=out), a.steps, a.warmup)
+    t_learn, _ = timed(lambda: mesh.query(qt, out=out_l, learned_bound=True), a.steps, a.warmup)
+    wk = {}
+    for name, kw in (("plain", {}), ("learned", {"learned_bound": True})):
+        w = torch.zeros(8, dtype=torch.int64, device=dev)
+        mesh.query(qt, work=w.data_ptr(), **kw)
+        torch.cuda.synchronize()
+        wk[name] = [int(x) for x in w.cpu().numpy()]
+    torch.cuda.synchronize()
+    return {"workload": "unit-icosphere-L5-learned-16M", "faces": int(f.shape[0]), "queries": n,
+            "train_samples": a.train_samples, "epochs": a.epochs, "train_seconds": round(train_s, 2),
+            "final_loss": round(loss, 5),
+            "learned": {"value": round(n / t_learn, 1), "ms_per_step": round(t_learn * 1e3, 3)},
+            "plain": {"value": round(n / t_plain, 1), "ms_per_step": round(t_plain * 1e3, 3)},
+            "speedup_learned_vs_plain": round(t_plain / t_learn, 3),
+            "max_abs_diff_vs_plain_over_diag": (out - out_l).abs().max().item() / diag,
+            "sign_agreement_vs_plain": bool(torch.equal(torch.sign(out), torch.sign(out_l))),
+            "per_query": {k: {"interior_nodes": round(w[0] / n, 2), "triangles": round(w[1] / n, 2),
+                              "fallbacks": w[3]} for k, w in wk.items()},
+            "per_wavefront": {k: {"interior_nodes": round(w[4] / (n / 64), 1), "triangles": round(w[5] / (n / 64), 1)}
+                              for k, w in wk.items()}}
 
 
 def scene_bench(a, dev):
@@ -178,14 +344,19 @@ def scene_bench(a, dev):
     u.value(q, out=out, work=w.data_ptr())
     torch.cuda.synchronize()
     w = [int(x) for x in w.cpu().numpy()]
-    byts = 16 * n + w[0] * 28 * 4 + w[6] * 16 + w[1] * 36
+    # bytes the packets fetch: per wavefront each node record (128 B, scene and K=4 mesh nodes),
+    # primitive record (128 B) and triangle record (128 B) once; per query point and result
+    byts = 16 * n + w[4] * 128 + w[7] * 128 + w[5] * 128
     gbs = byts / (kms * 1e-3) / 1e9
     log(f"scene {dt * 1e3:.2f} ms/step (traversal {kms:.2f} ms)")
     return {"workload": "csg-union-4096-spheres-1024-boxes-8-meshes-32M", "queries": n,
             "value": round(n / dt, 1), "unit": "queries/s", "ms_per_step": round(dt * 1e3, 3),
             "roofline": {"bound": "hbm", "achieved": round(gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                          "frac": round(gbs / HBM_PEAK_GBS, 4), "traffic": None, "kernel": "scene32_kernel<4,1,false>",
-                         "kernel_ms": round(kms, 3),
+                         "kernel_ms": round(kms, 3), "algorithmic_bytes_per_launch": int(byts),
+                         "per_wavefront": {"interior_nodes": round(w[4] / (n / 64), 1),
+                                           "primitives": round(w[7] / (n / 64), 1),
+                                           "triangles": round(w[5] / (n / 64), 1)},
                          "per_query": {"interior_nodes": round(w[0] / n, 2), "primitives": round(w[6] / n, 2),
                                        "triangles": round(w[1] / n, 2), "fp64_rechecked": w[2]}}}
 
@@ -221,8 +392,22 @@ def batched_bench(a):
     torch.cuda.synchronize()
     c = [int(x) for x in wk.cpu().numpy()]
     K = info["branching"]
-    byts = 16 * nq + c[0] * 28 * K + c[1] * 36
-    gbs = byts / (kms * 1e-3) / 1e9
+    # correctness of the captured plan's results: every 16th mesh, 200 strided queries each,
+    # against the oracle
+    plan.launch()
+    torch.cuda.synchronize()
+    worst, signs, checked = 0.0, 0, 0
+    from oracle import port
+    for i in range(0, len(w.verts), 16):
+        vv, ff, qq = w.verts[i], w.faces[i], w.queries[i]
+        sel = np.arange(0, qq.shape[0], max(1, qq.shape[0] // 200))[:200]
+        d_ref, _ = port.Mesh(vv.astype(np.float64), ff).sdf(qq[sel].astype(np.float64))
+        dg = plan.out[i].cpu().numpy()[sel]
+        dia = float(np.linalg.norm(vv.max(0).astype(np.float64) - vv.min(0)))
+        worst = max(worst, float(np.abs(dg - d_ref).max() / dia))
+        nz = d_ref != 0
+        signs += int(np.count_nonzero(np.sign(dg[nz]) != np.sign(d_ref[nz])))
+        checked += sel.size
     res = {"metric": "SDF queries/s on 1 MI355X, many-mesh batch (one captured plan)", "value": round(nq / dt, 1),
            "unit": "queries/s", "n_gpus": 1, "steps": a.steps, "warmup": a.warmup, "ms_per_step": round(dt * 1e3, 3),
            "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
@@ -230,14 +415,12 @@ def batched_bench(a):
                       "branching": K, "leaf_size": info["leaf_size"], "workgroup": plan.workgroup,
                       "tune_ms": info["tune_ms"]},
            "graph_replay_ms": round(replay_ms, 3),
-           "roofline": {"bound": "hbm", "achieved": round(gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                        "frac": round(gbs / HBM_PEAK_GBS, 4), "traffic": None,
-                        "kernel": f"batch32_kernel<{K},{info['stack_capacity'] // 64},false>",
-                        "algorithmic_bytes_per_launch": int(byts), "kernel_ms": round(kms, 3),
-                        "per_query": {"interior_nodes": round(c[0] / nq, 2), "triangles": round(c[1] / nq, 2)},
-                        "per_wavefront": {"interior_nodes": round(c[4] / (nq / 64), 1),
-                                          "triangles": round(c[5] / (nq / 64), 1)}}}
-    print(json.dumps(res), flush=True)
+           "roofline": packet_roofline(nq, c, K, kms, f"batch32_kernel<{K},{info['stack_capacity'] // 64},false>",
+                                       None),
+           "correctness": {"sample": f"{checked} queries (200 strided of every 16th mesh) against the oracle",
+                           "max_abs_err_over_diag": worst, "sign_mismatches": signs,
+                           "within_1e-5_diag": bool(worst <= 1e-5)}}
+    return res
 
 
 def main():
@@ -251,7 +434,7 @@ def main():
         copy_calibration()
         return
     if a.batched:
-        batched_bench(a)
+        print(json.dumps(batched_bench(a)), flush=True)
         return
 
     import torch
@@ -280,17 +463,18 @@ def main():
     K = info["branching"]
     kname = f"sdf32_kernel<{K},{info['stack_capacity'] // 64},false,{{mode}}>"
 
-    if a.plain_only:
-        # profiling aid: the plain pipeline alone (rocprofv3 --pmc cannot follow the training)
-        t_plain, trav_plain = timed(plain, a.steps, a.warmup, mesh.last_traversal_ms)
+    def work(**kw):
         w = torch.zeros(8, dtype=torch.int64, device=dev)
-        mesh.query(qt, out=out_l, work=w.data_ptr())
+        mesh.query(qt, out=out_l, work=w.data_ptr(), **kw)
         torch.cuda.synchronize()
-        w = [int(x) for x in w.cpu().numpy()]
-        byts = 16 * n + w[0] * 28 * K + w[1] * 36
-        print(json.dumps({"profile_run": "plain-only", "ms_per_step": round(t_plain * 1e3, 3),
-                          "kernel": kname.format(mode=0), "kernel_ms": round(trav_plain, 3),
-                          "algorithmic_bytes_per_launch": int(byts)}), flush=True)
+        return [int(x) for x in w.cpu().numpy()]
+
+    if a.plain_only:
+        # profiling aid: the plain pipeline alone (no training)
+        t_plain, trav_plain = timed(plain, a.steps, a.warmup, mesh.last_traversal_ms)
+        rp = packet_roofline(n, work(), K, trav_plain, kname.format(mode=0), None)
+        print(json.dumps({"profile_run": "plain-only", "ms_per_step": round(t_plain * 1e3, 3), "roofline": rp}),
+              flush=True)
         return
 
     t0 = time.perf_counter()
@@ -324,44 +508,47 @@ def main():
     torch.cuda.synchronize()
     dmax = (out - out_l).abs().max().item()
     sign_ok = bool(torch.equal(torch.sign(out), torch.sign(out_l)))
+    d_plain = out.cpu().numpy()
+    d_learn = out_l.cpu().numpy()
 
     # work counters (untimed counting variants) for the roofline's algorithmic bytes
-    def work(**kw):
-        w = torch.zeros(8, dtype=torch.int64, device=dev)
-        mesh.query(qt, out=out_l, work=w.data_ptr(), **kw)
-        torch.cuda.synchronize()
-        return [int(x) for x in w.cpu().numpy()]
-
     wp, wl_ = work(), work(learned_bound=True)
+    rl = packet_roofline(n, wl_, K, trav_learn, kname.format(mode=1), profiled_traffic("seeded"))
+    rp = packet_roofline(n, wp, K, trav_plain, kname.format(mode=0), profiled_traffic("plain"))
 
-    def roof(w, ms, kernel, traffic_key):
-        # main traversal launch only: per query 16 B (point in, distance out) + per interior node
-        # K child boxes and refs (24 K + 4 K B) + per real triangle tested 9 floats (36 B)
-        byts = 16 * n + w[0] * 28 * K + w[1] * 36
-        gbs = byts / (ms * 1e-3) / 1e9
-        return {"bound": "hbm", "achieved": round(gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": round(gbs / HBM_PEAK_GBS, 4), "traffic": profiled_traffic(traffic_key),
-                "kernel": kernel, "algorithmic_bytes_per_launch": int(byts), "kernel_ms": round(ms, 3),
-                "per_query": {"interior_nodes": round(w[0] / n, 2), "triangles": round(w[1] / n, 2),
-                              "fp64_rechecked": w[2], "fallbacks": w[3],
-                              "fallback_interior_nodes": round(w[6] / n, 3),
-                              "fallback_triangles": round(w[7] / n, 3)},
-                # what the wavefronts actually fetched (one scalar load serves 64 queries): the
-                # per-query algorithmic bytes above exceed it, so frac can pass 1
-                "per_wavefront": {"interior_nodes": round(w[4] / (n / 64), 1),
-                                  "triangles": round(w[5] / (n / 64), 1),
-                                  "fetched_bytes": int(16 * n + w[4] * 32 * K + w[5] * 128)}}
+    # closest faces of the whole workload (untimed), for the face check below
+    d_t, t_gpu = mesh.closest_triangle(qt)
+    torch.cuda.synchronize()
+    t_gpu = t_gpu.cpu().numpy().astype(np.uint32)
 
-    rl = roof(wl_, trav_learn, kname.format(mode=1), "seeded")
-    rp = roof(wp, trav_plain, kname.format(mode=0), "plain")
-
-    scene = None if a.no_scene else scene_bench(a, dev)
-
-    cpu = None
-    if not a.skip_cpu:
+    from_cpu = {}
+    cref = CpuReference(v, f)
+    if a.skip_cpu:
+        idx = np.arange(0, n, max(1, n // 50_000))[:50_000]
+        d_ref = cref.sdf(q[idx])
+        cpu = None
+    else:
         t0 = time.perf_counter()
-        cpu = cpu_baseline(v, f, q)
+        cpu, idx, d_ref = cpu_baseline(cref, q)
         log(f"cpu baseline {cpu['value']:.3g} q/s ({time.perf_counter() - t0:.1f}s)")
+    from_cpu["sample"] = f"{idx.size} strided queries of the workload against {cref.label}"
+    from_cpu["plain"] = compare(d_plain[idx], d_ref, diag)
+    from_cpu["learned"] = compare(d_learn[idx], d_ref, diag)
+    # faces against getClosestTriangle on a 100,000-query sub-sample
+    fidx = idx[:: max(1, idx.size // 100_000)][:100_000]
+    dref_f, tref_f = cref.closest(q[fidx])
+    from_cpu["faces"] = face_check(v, f, q[fidx], dref_f, tref_f, t_gpu[fidx], diag)
+    ok = from_cpu["plain"]["within_1e-5_diag"] and from_cpu["learned"]["within_1e-5_diag"] and \
+        from_cpu["plain"]["sign_mismatches"] == 0 and from_cpu["learned"]["sign_mismatches"] == 0
+    from_cpu["pass"] = bool(ok)
+    log(f"correctness: {from_cpu}")
+    del cref
+
+    c1 = c1_bench(a, dev, CpuReference)
+    log(f"c1: {c1}")
+    batched = None if a.no_batched else batched_bench(a)
+    learned_c1 = None if a.no_learned_c1 else learned_c1_bench(a, dev)
+    scene = None if a.no_scene else scene_bench(a, dev)
 
     value = n / t_learn
     res = {
@@ -381,12 +568,22 @@ def main():
                           "train_samples": a.train_samples, "epochs": a.epochs, "train_seconds": round(train_s, 2),
                           "final_loss": round(loss, 5), "max_abs_diff_vs_plain_over_diag": dmax / diag,
                           "sign_agreement_vs_plain": sign_ok},
+        "correctness": from_cpu,
         "roofline": rl,
         "roofline_plain": rp,
         "cpu_baseline": cpu,
+        "c1": c1,
+        "batched": batched,
+        "learned_c1": learned_c1,
         "scene": scene,
     }
     print(json.dumps(res), flush=True)
+    bad = not ok or (c1.get("correctness") and not (c1["correctness"]["within_1e-5_diag"] and
+                                                    c1["correctness"]["sign_mismatches"] == 0)) or \
+        (batched and not batched["correctness"]["within_1e-5_diag"])
+    if bad:
+        log("correctness bar failed")
+        sys.exit(3)
 
 
 if __name__ == "__main__":

@@ -69,8 +69,12 @@ SIGNATURES = [
     ("ebg_query_opts_default", None, [C.POINTER(QueryOpts)]),
     ("ebg_query_sdf", C.c_int, [_vp, _vp, _u64, _vp, _vp, C.POINTER(QueryOpts)]),
     ("ebg_query_sdf_host", C.c_int, [_vp, _vp, _u64, _vp, _vp, C.POINTER(QueryOpts)]),
+    ("ebg_query_sdf_noalloc", C.c_int, [_vp, _vp, _u64, _vp, _vp, C.POINTER(QueryOpts)]),
+    ("ebg_query_sdf_f64", C.c_int, [_vp, _vp, _u64, _vp, _vp, C.POINTER(QueryOpts)]),
     ("ebg_mesh_last_traversal_ms", C.c_double, [_vp]),
     ("ebg_mesh_last_split_packets", C.c_int64, [_vp]),
+    ("ebg_mesh_check", C.c_int, [_vp]),
+    ("ebg_scene_check", C.c_int, [_vp]),
     ("ebg_batch_plan_last_early_packets", C.c_int64, [_vp]),
     ("ebg_morton_codes_f32", C.c_int, [_vp, _u64, _vp, _vp]),
     ("ebg_morton_codes_f64", C.c_int, [_vp, _u64, _vp, _vp]),
@@ -97,6 +101,9 @@ SIGNATURES = [
     ("ebg_mlp_train", C.c_int, [_vp, C.POINTER(MlpOpts), C.POINTER(C.c_float)]),
     ("ebg_mlp_predict", C.c_int, [_vp, _vp, _u64, _vp, _vp]),
     ("ebg_mlp_param_count", _u64, []),
+    ("ebg_mlp_check_schedule", C.c_int, [_u64, _i32]),
+    ("ebg_mlp_save", C.c_int, [_vp, C.c_char_p]),
+    ("ebg_mlp_load", C.c_int, [_vp, C.c_char_p]),
     ("ebg_mlp_get_params", C.c_int, [_vp, _vp]),
     ("ebg_mlp_set_params", C.c_int, [_vp, _vp]),
     ("ebg_mlp_loss_grad", C.c_int, [_vp, _vp, _vp, _u64, _vp, _vp, _vp]),

@@ -42,6 +42,7 @@ SOURCES = [
     ("morton_build.hip", DEVICE),
     ("util.hip", DEVICE),
     ("mlp.hip", DEVICE),
+    ("noalloc.hip", DEVICE),
 ]
 HEADERS = ["ebgpu_internal.hpp", "kernels.hpp", "mesh_build.hpp", "mlp.hpp", "traverse_dev.hpp", "host_common.hpp"]
 

@@ -17,6 +17,7 @@
 #include <algorithm>
 #include <chrono>
 #include <cmath>
+#include <cstdio>
 #include <cstdlib>
 #include <cstring>
 #include <mutex>
@@ -168,6 +169,8 @@ static ebg_status upload_mesh(ebg_mesh* m, ebgpu::PackedMesh& pm)
   }
   d.diag          = float(pm.diag);
   d.num_tris      = uint32_t(m->nt);
+  EBG_HIP(upload(m->status, std::vector<uint32_t>(16, 0u)));
+  d.status = m->status.as<uint32_t>();
   m->device_bytes = m->nodes.bytes + m->tris.bytes + m->tripos.bytes + m->tsph.bytes + m->feat.bytes + m->vn32.bytes +
                     m->en32.bytes + m->vn64.bytes + m->en64.bytes + m->fn64.bytes;
   return EBG_OK;
@@ -210,6 +213,7 @@ ebg_status ebg_mesh_create(const float* xyz, uint64_t nv, const uint32_t* tri, u
   auto* m = new ebg_mesh;
   m->nv   = nv;
   m->nt   = nt;
+  m->hash = ebgpu::mesh_hash(xyz, nv, tri, nt);
   m->host_xyz.assign(xyz, xyz + 3 * nv);
   m->order = std::move(pm.order);
   const ebg_status st = upload_mesh(m, pm);
@@ -326,44 +330,41 @@ ebg_status ensure_cost_grid(ebg_mesh* m, hipStream_t s)
 } // namespace ebgpu
 using ebgpu::ensure_cost_grid;
 
-extern "C" {
-
-ebg_status ebg_query_sdf(const ebg_mesh* cm, const float* d_pts, uint64_t n, float* d_dist, uint32_t* d_tri,
-                         const ebg_query_opts* opts)
+namespace ebgpu {
+// The query pipeline over a workspace of at least n queries. With may_alloc false nothing is
+// allocated or created: split packets are used only if the mesh's cost grid already exists, the
+// bound network runs on the caller's stream, and the traversal is not timed.
+ebg_status query_pipeline(ebg_mesh* m, const float* d_pts, uint64_t n, float* d_dist, uint32_t* d_tri,
+                          const ebg_query_opts& o, const WsView& ws, bool may_alloc)
 {
-  if (!cm) return fail(EBG_INVALID_ARG, "mesh is null");
-  if (n == 0) return EBG_OK;
-  if (!d_pts || !d_dist) return fail(EBG_INVALID_ARG, "null point or output array");
-  if (n > 0xFFFFFFFFull) return fail(EBG_INVALID_ARG, "at most 2^32-1 queries per call");
-  ebg_mesh*      m = const_cast<ebg_mesh*>(cm);
-  ebg_query_opts o;
-  ebg_query_opts_default(&o);
-  if (opts) o = *opts;
-  if (o.learned_bound && !m->mlp.trained) return fail(EBG_NOT_READY, "learned bound requested but no network trained");
   hipStream_t s = static_cast<hipStream_t>(o.stream);
-
-  std::lock_guard<std::recursive_mutex> lock(m->mu);
-  Workspace&                  ws = m->ws;
-  EBG_HIP(ws.ensure(n));
-  if (!m->ev0) {
+  if (ws.cap < n) return fail(EBG_INVALID_ARG, "query workspace smaller than the batch");
+  if (may_alloc && !m->ev0) {
     EBG_HIP(hipEventCreate(&m->ev0));
     EBG_HIP(hipEventCreate(&m->ev1));
   }
+  const bool timed = m->ev0 && may_alloc;
   // counting launches run unsplit: the counters describe the plain traversal's own work
-  const bool heavy_first = o.presort && n >= kHeavyMinQ && !o.work;
+  const bool heavy_first = o.presort && n >= kHeavyMinQ && !o.work && (may_alloc || m->grid_ready);
   m->last_split           = heavy_first;
-  m->last_split_cap       = heavy_first ? ebgpu::Workspace::heavy_cap(n) : 0;
+  m->last_split_cap       = heavy_first ? Workspace::heavy_cap(n) : 0;
   if (heavy_first) {
     const ebg_status gs = ensure_cost_grid(m, s);
     if (gs != EBG_OK) return gs;
   }
-  uint32_t* cnt = ws.counters(); // [0] recheck, [1] fallback, [2] split packets, [3] early packets
-  EBG_HIP(ebgpu::launch_fill_u32(cnt, 4, 0u, s));
+  uint32_t* cnt = ws.counters; // [0] recheck, [1] fallback, [2] split packets, [3] early packets
+  EBG_HIP(launch_fill_u32(cnt, 4, 0u, s));
 
   // The bound network needs no presort (it writes the radius by query index), so it runs on a
   // second stream of the handle, in query order, while the presort runs on the caller's stream.
   const float* seeds = o.seed_radius;
-  if (o.learned_bound) {
+  const bool   fork  = o.learned_bound && (may_alloc || m->aux);
+  if (o.learned_bound && !fork) {
+    // no second stream without creating one: the network runs first on the caller's stream
+    EBG_HIP(mlp_forward(m->mlp, d_pts, nullptr, n, true, o.bound_scale, float(o.bound_offset * m->diag), ws.seeds, s));
+    seeds = ws.seeds;
+  }
+  else if (o.learned_bound) {
     if (!m->aux) {
       EBG_HIP(hipStreamCreateWithFlags(&m->aux, hipStreamNonBlocking));
       EBG_HIP(hipEventCreateWithFlags(&m->ev_fork, hipEventDisableTiming));
@@ -371,48 +372,48 @@ ebg_status ebg_query_sdf(const ebg_mesh* cm, const float* d_pts, uint64_t n, flo
     }
     EBG_HIP(hipEventRecord(m->ev_fork, s));
     EBG_HIP(hipStreamWaitEvent(m->aux, m->ev_fork, 0));
-    EBG_HIP(ebgpu::mlp_forward(m->mlp, d_pts, nullptr, n, true, o.bound_scale, float(o.bound_offset * m->diag),
-                               ws.seeds.as<float>(), m->aux));
+    EBG_HIP(mlp_forward(m->mlp, d_pts, nullptr, n, true, o.bound_scale, float(o.bound_offset * m->diag), ws.seeds,
+                        m->aux));
     EBG_HIP(hipEventRecord(m->ev_join, m->aux));
-    seeds = ws.seeds.as<float>();
+    seeds = ws.seeds;
   }
 
   // the presort keys carry each query's cost flags (cost-grid value against the heavy and early
   // thresholds) to the packet marking
-  ebgpu::CostFlags cf{};
+  CostFlags cf{};
   if (heavy_first) {
     cf.grid = m->cost_grid.as<uint32_t>();
     for (int c = 0; c < 3; c++) {
       cf.lo[c]    = m->grid_lo[c];
       cf.scale[c] = m->grid_scale[c];
     }
-    cf.g     = ebgpu::kCostGrid;
+    cf.g     = kCostGrid;
     cf.heavy = uint32_t(std::max(1.0, heavy_factor(n) * m->grid_mean));
     cf.early = uint32_t(std::max(1.0, kEarlyFactor * m->grid_mean));
   }
   const uint32_t* idx = nullptr;
   if (o.presort && n > 1) {
-    EBG_HIP(ws.presort(d_pts, n, s, heavy_first ? &cf : nullptr));
-    idx = ws.perm.as<uint32_t>();
+    EBG_HIP(presort(ws, d_pts, n, s, heavy_first ? &cf : nullptr));
+    idx = ws.perm;
   }
-  ebgpu::HeavyMarkArgs h{};
+  HeavyMarkArgs h{};
   if (heavy_first) {
-    h.skeys       = ws.codes_sorted.as<uint32_t>();
+    h.skeys       = ws.codes_sorted;
     h.n           = n;
-    h.cap         = ebgpu::Workspace::heavy_cap(n);
-    h.list        = ws.heavy_list();
+    h.cap         = Workspace::heavy_cap(n);
+    h.list        = ws.heavy_list;
     h.count       = cnt + 2;
-    h.flag        = ws.heavy_flag();
-    h.keys        = ws.heavy_keys();
-    h.early_cap   = ebgpu::Workspace::early_cap(n);
-    h.early_list  = ws.early_list();
+    h.flag        = ws.heavy_flag;
+    h.keys        = ws.heavy_keys;
+    h.early_cap   = Workspace::early_cap(n);
+    h.early_list  = ws.early_list;
     h.early_count = cnt + 3;
-    EBG_HIP(ebgpu::launch_heavy_mark(h, s));
+    EBG_HIP(launch_heavy_mark(h, s));
   }
-  if (o.learned_bound) EBG_HIP(hipStreamWaitEvent(s, m->ev_join, 0));
+  if (fork) EBG_HIP(hipStreamWaitEvent(s, m->ev_join, 0));
 
   const bool recheck = o.fp64_recheck || o.fp64_all;
-  ebgpu::QueryArgs a{};
+  QueryArgs a{};
   a.pts            = d_pts;
   a.idx            = idx;
   a.n              = n;
@@ -420,9 +421,9 @@ ebg_status ebg_query_sdf(const ebg_mesh* cm, const float* d_pts, uint64_t n, flo
   a.out_d          = d_dist;
   a.out_tri        = d_tri;
   a.recheck_abs    = o.fp64_all ? INFINITY : (recheck ? float(o.recheck_eps * m->diag) : 0.0f);
-  a.recheck_list   = recheck ? ws.recheck_list() : nullptr;
+  a.recheck_list   = recheck ? ws.recheck_list : nullptr;
   a.recheck_count  = cnt;
-  a.recheck_mask   = recheck ? ws.rmask.as<uint64_t>() : nullptr;
+  a.recheck_mask   = recheck ? ws.rmask : nullptr;
   a.fallback_list  = nullptr;
   a.fallback_count = cnt + 1;
   a.work           = reinterpret_cast<unsigned long long*>(o.work);
@@ -431,7 +432,7 @@ ebg_status ebg_query_sdf(const ebg_mesh* cm, const float* d_pts, uint64_t n, flo
   if (heavy_first) {
     // split packets run unseeded: exact without a fallback
     a.heavy_flag   = h.flag;
-    a.split_roots  = m->split_roots.as<ebgpu::SplitRoot>();
+    a.split_roots  = m->split_roots.as<SplitRoot>();
     a.split_list   = h.list;
     a.split_count  = h.count;
     a.split_keys   = h.keys;
@@ -448,17 +449,19 @@ ebg_status ebg_query_sdf(const ebg_mesh* cm, const float* d_pts, uint64_t n, flo
 
   // events bracket the FP32 traversal (one launch for the bulk and the split items, then the
   // split packets' resolve)
-  EBG_HIP(hipEventRecord(m->ev0, s));
-  EBG_HIP(ebgpu::launch_sdf32(m->dev, m->K, m->banks, a, count, s));
-  if (heavy_first) EBG_HIP(ebgpu::launch_sdf32_resolve(m->dev, a, s));
-  EBG_HIP(hipEventRecord(m->ev1, s));
-  m->timed = true;
+  if (timed) EBG_HIP(hipEventRecord(m->ev0, s));
+  EBG_HIP(launch_sdf32(m->dev, m->K, m->banks, a, count, s));
+  if (heavy_first) EBG_HIP(launch_sdf32_resolve(m->dev, a, s));
+  if (timed) {
+    EBG_HIP(hipEventRecord(m->ev1, s));
+    m->timed = true;
+  }
 
   if (recheck) {
     // the list follows the presort order, so FP64 packets hold neighbouring queries
-    EBG_HIP(ebgpu::launch_recheck_compact(a.recheck_mask, n, idx, ws.recheck_list(), cnt, s));
-    ebgpu::QueryArgs r = a;
-    r.idx              = ws.recheck_list();
+    EBG_HIP(launch_recheck_compact(a.recheck_mask, n, idx, ws.recheck_list, cnt, s));
+    QueryArgs r = a;
+    r.idx       = ws.recheck_list;
     r.count_ptr        = cnt;
     r.seed_r           = nullptr;
     r.recheck_list     = nullptr;
@@ -469,9 +472,63 @@ ebg_status ebg_query_sdf(const ebg_mesh* cm, const float* d_pts, uint64_t n, flo
     // accurate to a few ulp of the coordinates
     r.slack_abs = float(1e-5 * m->diag);
     r.slack_rel = 1e-4f;
-    EBG_HIP(ebgpu::launch_sdf64(m->dev, m->K, m->banks, r, s));
+    EBG_HIP(launch_sdf64(m->dev, m->K, m->banks, r, s));
   }
-  if (count) EBG_HIP(ebgpu::launch_add_counts(a.work, recheck ? cnt : nullptr, seeds ? cnt + 1 : nullptr, s));
+  if (count) EBG_HIP(launch_add_counts(a.work, recheck ? cnt : nullptr, seeds ? cnt + 1 : nullptr, s));
+  return EBG_OK;
+}
+
+} // namespace ebgpu
+
+extern "C" {
+
+ebg_status ebg_query_sdf(const ebg_mesh* cm, const float* d_pts, uint64_t n, float* d_dist, uint32_t* d_tri,
+                         const ebg_query_opts* opts)
+{
+  if (!cm) return fail(EBG_INVALID_ARG, "mesh is null");
+  if (n == 0) return EBG_OK;
+  if (!d_pts || !d_dist) return fail(EBG_INVALID_ARG, "null point or output array");
+  if (n > 0xFFFFFFFFull) return fail(EBG_INVALID_ARG, "at most 2^32-1 queries per call");
+  ebg_mesh*      m = const_cast<ebg_mesh*>(cm);
+  ebg_query_opts o;
+  ebg_query_opts_default(&o);
+  if (opts) o = *opts;
+  if (o.learned_bound && !m->mlp.trained) return fail(EBG_NOT_READY, "learned bound requested but no network trained");
+  std::lock_guard<std::recursive_mutex> lock(m->mu);
+  EBG_HIP(m->ws.ensure(n));
+  return ebgpu::query_pipeline(m, d_pts, n, d_dist, d_tri, o, m->ws.view(), true);
+}
+
+ebg_status ebg_query_sdf_f64(const ebg_mesh* cm, const double* d_pts, uint64_t n, double* d_dist, uint32_t* d_tri,
+                             const ebg_query_opts* opts)
+{
+  if (!cm) return fail(EBG_INVALID_ARG, "mesh is null");
+  if (n == 0) return EBG_OK;
+  if (!d_pts || !d_dist) return fail(EBG_INVALID_ARG, "null point or output array");
+  if (n > 0xFFFFFFFFull) return fail(EBG_INVALID_ARG, "at most 2^32-1 queries per call");
+  ebg_mesh*      m = const_cast<ebg_mesh*>(cm);
+  ebg_query_opts o;
+  ebg_query_opts_default(&o);
+  if (opts) o = *opts;
+  hipStream_t                           s = static_cast<hipStream_t>(o.stream);
+  std::lock_guard<std::recursive_mutex> lock(m->mu);
+  EBG_HIP(m->pts32.reserve(12 * n));
+  EBG_HIP(m->dist32.reserve(4 * n));
+  EBG_HIP(ebgpu::launch_f64_to_f32(d_pts, m->pts32.as<float>(), 3 * n, s));
+  const ebg_status st = ebg_query_sdf(m, m->pts32.as<float>(), n, m->dist32.as<float>(), d_tri, &o);
+  if (st != EBG_OK) return st;
+  EBG_HIP(ebgpu::launch_f32_to_f64(m->dist32.as<float>(), d_dist, n, s));
+  return EBG_OK;
+}
+
+ebg_status ebg_mesh_check(const ebg_mesh* m)
+{
+  if (!m) return fail(EBG_INVALID_ARG, "mesh is null");
+  uint32_t w = 0;
+  EBG_HIP(hipDeviceSynchronize());
+  EBG_HIP(hipMemcpy(&w, m->status.p, 4, hipMemcpyDeviceToHost));
+  if (w & ebgpu::kStatusStackOverflow)
+    return fail(EBG_STACK_OVERFLOW, "a traversal of this mesh needed more stack than its build-time worst case");
   return EBG_OK;
 }
 
@@ -593,6 +650,14 @@ void ebg_mlp_opts_default(ebg_mlp_opts* o)
 
 uint64_t ebg_mlp_param_count(void) { return uint64_t(ebgpu::kMlpParams); }
 
+ebg_status ebg_mlp_check_schedule(uint64_t num_samples, int32_t batch)
+{
+  if (num_samples < 1 || batch < 32 || batch > 65536) return fail(EBG_INVALID_ARG, "num_samples/batch out of range");
+  std::string why;
+  if (!ebgpu::mlp_check_train_schedule(num_samples, batch, &why)) return fail(EBG_INVALID_ARG, why);
+  return EBG_OK;
+}
+
 // Input normalisation of the network: the box_scale-scaled mesh bbox maps to [-1, 1]^3.
 static void mlp_set_box(ebg_mesh* m, float box_scale)
 {
@@ -618,6 +683,10 @@ ebg_status ebg_mlp_train(ebg_mesh* m, const ebg_mlp_opts* opts, float* loss)
   if (o.epochs < 1 || o.batch < 32 || o.batch > 65536) return fail(EBG_INVALID_ARG, "epochs/batch out of range");
   if (!(o.lr > 0.0f) || !(o.box_scale >= 1.0f) || !(o.near_sigma >= 0.0f) || !(o.under_weight >= 1.0f))
     return fail(EBG_INVALID_ARG, "lr, box_scale, near_sigma or under_weight out of range");
+  {
+    std::string why;
+    if (!ebgpu::mlp_check_train_schedule(o.num_samples, o.batch, &why)) return fail(EBG_INVALID_ARG, why);
+  }
   hipStream_t s = static_cast<hipStream_t>(o.stream);
   std::lock_guard<std::recursive_mutex> lock(m->mu);
   ebgpu::Mlp& mlp = m->mlp;
@@ -653,7 +722,9 @@ ebg_status ebg_mlp_train(ebg_mesh* m, const ebg_mlp_opts* opts, float* loss)
     for (uint64_t b = 0; b < n; b += o.batch, k++) {
       const int   nb = int(std::min<uint64_t>(o.batch, n - b));
       const float lr = float(o.lr * (0.05 + 0.95 * 0.5 * (1.0 + std::cos(M_PI * double(k) / total))));
-      EBG_HIP(ebgpu::mlp_loss_grad(mlp, m->train, pts.as<float>() + 3 * b, dist.as<float>() + b, nb, s));
+      std::string      why;
+      const hipError_t le = ebgpu::mlp_loss_grad(mlp, m->train, pts.as<float>() + 3 * b, dist.as<float>() + b, nb, s, &why);
+      if (le != hipSuccess) return fail(le == hipErrorOutOfMemory ? EBG_OOM : EBG_HIP_ERROR, why.empty() ? hipGetErrorString(le) : why);
       EBG_HIP(ebgpu::mlp_adam(mlp, lr, s));
     }
     float sum = 0.0f;
@@ -696,6 +767,73 @@ ebg_status ebg_mlp_set_params(ebg_mesh* m, const float* params)
   return EBG_OK;
 }
 
+// Network file: magic, format version, the mesh hash it was trained for, the input box and
+// loss settings, then the kMlpParams parameters in the public order; all little-endian.
+namespace {
+constexpr char     kMlpMagic[8] = {'E', 'B', 'G', 'M', 'L', 'P', '\0', '\0'};
+constexpr uint32_t kMlpVersion  = 1;
+struct MlpFileHeader
+{
+  char     magic[8];
+  uint32_t version, params;
+  uint64_t mesh_hash;
+  float    lo[3], scale[3], diag, box_scale, under_weight, reserved;
+};
+static_assert(sizeof(MlpFileHeader) == 64, "MlpFileHeader layout");
+} // namespace
+
+ebg_status ebg_mlp_save(const ebg_mesh* m, const char* path)
+{
+  if (!m || !path) return fail(EBG_INVALID_ARG, "null argument");
+  if (!m->mlp.master || !m->mlp.trained) return fail(EBG_NOT_READY, "no trained network");
+  MlpFileHeader h{};
+  std::memcpy(h.magic, kMlpMagic, 8);
+  h.version   = kMlpVersion;
+  h.params    = uint32_t(ebgpu::kMlpParams);
+  h.mesh_hash = m->hash;
+  for (int c = 0; c < 3; c++) {
+    h.lo[c]    = m->mlp.lo[c];
+    h.scale[c] = m->mlp.scale[c];
+  }
+  h.diag         = m->mlp.diag;
+  h.box_scale    = m->mlp.box_scale;
+  h.under_weight = m->mlp.under_weight;
+  std::vector<float> p(ebgpu::kMlpParams);
+  EBG_HIP(ebgpu::mlp_get_params_host(m->mlp, p.data()));
+  FILE* f = std::fopen(path, "wb");
+  if (!f) return fail(EBG_INVALID_ARG, std::string("cannot open ") + path + " for writing");
+  const bool ok = std::fwrite(&h, sizeof(h), 1, f) == 1 && std::fwrite(p.data(), sizeof(float), p.size(), f) == p.size();
+  if (std::fclose(f) != 0 || !ok) return fail(EBG_INVALID_ARG, std::string("write to ") + path + " failed");
+  return EBG_OK;
+}
+
+ebg_status ebg_mlp_load(ebg_mesh* m, const char* path)
+{
+  if (!m || !path) return fail(EBG_INVALID_ARG, "null argument");
+  FILE* f = std::fopen(path, "rb");
+  if (!f) return fail(EBG_INVALID_ARG, std::string("cannot open ") + path);
+  MlpFileHeader      h{};
+  std::vector<float> p(ebgpu::kMlpParams);
+  const bool ok = std::fread(&h, sizeof(h), 1, f) == 1 && std::fread(p.data(), sizeof(float), p.size(), f) == p.size();
+  std::fclose(f);
+  if (!ok || std::memcmp(h.magic, kMlpMagic, 8) != 0 || h.version != kMlpVersion || h.params != uint32_t(ebgpu::kMlpParams))
+    return fail(EBG_INVALID_ARG, std::string(path) + " is not a network file of this version");
+  if (h.mesh_hash != m->hash) return fail(EBG_INVALID_ARG, "the network in " + std::string(path) + " was trained for another mesh");
+  for (float x : p)
+    if (!std::isfinite(x)) return fail(EBG_INVALID_ARG, "non-finite parameter in " + std::string(path));
+  std::lock_guard<std::recursive_mutex> lock(m->mu);
+  EBG_HIP(ebgpu::mlp_set_params_host(m->mlp, p.data()));
+  for (int c = 0; c < 3; c++) {
+    m->mlp.lo[c]    = h.lo[c];
+    m->mlp.scale[c] = h.scale[c];
+  }
+  m->mlp.diag         = h.diag;
+  m->mlp.box_scale    = h.box_scale;
+  m->mlp.under_weight = h.under_weight;
+  m->mlp.trained      = true;
+  return EBG_OK;
+}
+
 ebg_status ebg_mlp_loss_grad(ebg_mesh* m, const float* d_pts, const float* d_target, uint64_t n, float* d_grad,
                              float* d_loss, void* stream)
 {
@@ -706,7 +844,11 @@ ebg_status ebg_mlp_loss_grad(ebg_mesh* m, const float* d_pts, const float* d_tar
   std::lock_guard<std::recursive_mutex> lock(m->mu);
   EBG_HIP(m->train.ensure(int(n)));
   EBG_HIP(hipMemsetAsync(m->train.loss, 0, sizeof(float), s));
-  EBG_HIP(ebgpu::mlp_loss_grad(m->mlp, m->train, d_pts, d_target, int(n), s));
+  {
+    std::string      why;
+    const hipError_t le = ebgpu::mlp_loss_grad(m->mlp, m->train, d_pts, d_target, int(n), s, &why);
+    if (le != hipSuccess) return fail(le == hipErrorOutOfMemory ? EBG_OOM : EBG_HIP_ERROR, why.empty() ? hipGetErrorString(le) : why);
+  }
   EBG_HIP(ebgpu::mlp_grad_public(m->mlp, d_grad, s));
   if (d_loss) EBG_HIP(hipMemcpyAsync(d_loss, m->train.loss, sizeof(float), hipMemcpyDeviceToDevice, s));
   return EBG_OK;

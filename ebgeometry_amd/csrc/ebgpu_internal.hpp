@@ -90,8 +90,11 @@ struct MeshDev
   float           root_lo[3], root_hi[3];
   float           diag;
   uint32_t        num_tris;
+  uint32_t*       status; // device word: bit 0 set when a traversal found its stack too small
+  uint64_t        reserved;
 };
 static_assert(sizeof(MeshDev) % 16 == 0, "MeshDev is read with 16-byte scalar loads");
+constexpr uint32_t kStatusStackOverflow = 1u;
 
 // ---------------------------------------------------------------------------------------------
 // CSG union scenes: primitives in BVH leaf order
@@ -133,6 +136,7 @@ struct SceneDev
   const MeshDev*   meshes; // device array, indexed by PrimRec::mesh
   float            root_lo[3], root_hi[3];
   float            diag;
+  uint32_t*        status; // device word: kStatusStackOverflow when the scene walk's stack was too small
 };
 
 } // namespace ebgpu

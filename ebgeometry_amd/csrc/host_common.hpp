@@ -73,6 +73,28 @@ hipError_t upload(DevBuf& b, const std::vector<T>& v)
   return e;
 }
 
+// Device memory of one query call, carved from a workspace: the presort keys and permutation,
+// the radix sort's scratch, the bounding-box partials, the FP64 recheck list, the counters, the
+// seeded radii, the recheck lane masks and the split/early packet schedule, for up to `cap`
+// queries.
+struct WsView
+{
+  uint64_t            cap = 0;
+  uint32_t*           codes = nullptr;        // cap (32-bit presort keys)
+  uint32_t*           codes_sorted = nullptr; // cap
+  uint32_t*           perm = nullptr;         // cap
+  void*               sort_scratch = nullptr; // radix_sort_scratch_bytes(cap)
+  void*               box = nullptr;          // 6 doubles x (kBboxParts + 1)
+  uint32_t*           recheck_list = nullptr; // cap
+  uint32_t*           counters = nullptr;     // 4: recheck, fallback, split packets, early packets
+  float*              seeds = nullptr;        // cap
+  uint64_t*           rmask = nullptr;        // packets(cap)
+  uint8_t*            heavy_flag = nullptr;   // packets(cap)
+  uint32_t*           heavy_list = nullptr;   // heavy_cap(cap)
+  unsigned long long* heavy_keys = nullptr;   // 64 x heavy_cap(cap)
+  uint32_t*           early_list = nullptr;   // early_cap(cap)
+};
+
 // Query workspace, grown on demand.
 struct Workspace
 {
@@ -115,18 +137,74 @@ struct Workspace
   uint32_t* fallback_list() const { return lists.as<uint32_t>() + cap; }
   uint32_t* counters() const { return lists.as<uint32_t>() + 2 * cap; } // [0] recheck [1] fallback [2] split [3] early
 
-  // Hilbert presort of n device points: the permutation lands in perm (valid until the next
-  // call). With cost flags the sorted keys, flags in bits 30 and 31, land in codes_sorted.
-  hipError_t presort(const float* d_pts, uint64_t n, hipStream_t s, const CostFlags* cf = nullptr)
+  WsView view() const
   {
-    hipError_t e;
-    if ((e = launch_bbox_f32(d_pts, n, box.as<float>(), box.as<float>() + 8, s)) != hipSuccess) return e;
-    // 30-bit Hilbert keys on a 1024^3 grid, sorted as 32-bit keys in 4 radix passes
-    if ((e = launch_hilbert_f32(d_pts, n, box.as<float>(), codes.as<uint32_t>(), s, cf)) != hipSuccess) return e;
-    return radix_sort_pairs32(codes.as<uint32_t>(), nullptr, codes_sorted.as<uint32_t>(), perm.as<uint32_t>(), n, 30,
-                              sort_scratch.p, cf != nullptr, s);
+    WsView v;
+    v.cap          = cap;
+    v.codes        = codes.as<uint32_t>();
+    v.codes_sorted = codes_sorted.as<uint32_t>();
+    v.perm         = perm.as<uint32_t>();
+    v.sort_scratch = sort_scratch.p;
+    v.box          = box.p;
+    v.recheck_list = recheck_list();
+    v.counters     = counters();
+    v.seeds        = seeds.as<float>();
+    v.rmask        = rmask.as<uint64_t>();
+    v.heavy_flag   = heavy_flag();
+    v.heavy_list   = heavy_list();
+    v.heavy_keys   = heavy_keys();
+    v.early_list   = early_list();
+    return v;
+  }
+
+  // The same arrays carved from one flat block of flat_bytes(cap) bytes (256-byte aligned parts).
+  static std::size_t part(std::size_t b) { return (b + 255) & ~std::size_t(255); }
+  static std::size_t flat_bytes(uint64_t cap)
+  {
+    const uint64_t np = packets(cap);
+    return 3 * part(4 * cap) + part(radix_sort_scratch_bytes(cap)) + part(sizeof(double) * 6 * (kBboxParts + 1)) +
+           part(4 * cap) + part(16) + part(4 * cap) + part(8 * np) + part(np) + part(4 * (size_t)heavy_cap(cap)) +
+           part(512 * (size_t)heavy_cap(cap)) + part(4 * (size_t)early_cap(cap));
+  }
+  static WsView carve(uint8_t* base, uint64_t cap)
+  {
+    const uint64_t np = packets(cap);
+    WsView         v;
+    v.cap    = cap;
+    auto take = [&](std::size_t b) {
+      uint8_t* p = base;
+      base += part(b);
+      return p;
+    };
+    v.codes        = reinterpret_cast<uint32_t*>(take(4 * cap));
+    v.codes_sorted = reinterpret_cast<uint32_t*>(take(4 * cap));
+    v.perm         = reinterpret_cast<uint32_t*>(take(4 * cap));
+    v.sort_scratch = take(radix_sort_scratch_bytes(cap));
+    v.box          = take(sizeof(double) * 6 * (kBboxParts + 1));
+    v.recheck_list = reinterpret_cast<uint32_t*>(take(4 * cap));
+    v.counters     = reinterpret_cast<uint32_t*>(take(16));
+    v.seeds        = reinterpret_cast<float*>(take(4 * cap));
+    v.rmask        = reinterpret_cast<uint64_t*>(take(8 * np));
+    v.heavy_flag   = take(np);
+    v.heavy_list   = reinterpret_cast<uint32_t*>(take(4 * (size_t)heavy_cap(cap)));
+    v.heavy_keys   = reinterpret_cast<unsigned long long*>(take(512 * (size_t)heavy_cap(cap)));
+    v.early_list   = reinterpret_cast<uint32_t*>(take(4 * (size_t)early_cap(cap)));
+    return v;
   }
 };
+
+// Hilbert presort of n device points into a workspace: the permutation lands in w.perm (valid
+// until the next call). With cost flags the sorted keys, flags in bits 30 and 31, land in
+// w.codes_sorted.
+inline hipError_t presort(const WsView& w, const float* d_pts, uint64_t n, hipStream_t s, const CostFlags* cf = nullptr)
+{
+  hipError_t e;
+  float*     box = static_cast<float*>(w.box);
+  if ((e = launch_bbox_f32(d_pts, n, box, box + 8, s)) != hipSuccess) return e;
+  // 30-bit Hilbert keys on a 1024^3 grid, sorted as 32-bit keys in 4 radix passes
+  if ((e = launch_hilbert_f32(d_pts, n, box, w.codes, s, cf)) != hipSuccess) return e;
+  return radix_sort_pairs32(w.codes, nullptr, w.codes_sorted, w.perm, n, 30, w.sort_scratch, cf != nullptr, s);
+}
 
 } // namespace ebgpu
 
@@ -144,6 +222,9 @@ struct ebg_mesh
   // work of a packet of 64 queries spread over the cell
   ebgpu::DevBuf          cost_grid;
   ebgpu::DevBuf          split_roots; // SplitRoot frontier covering the tree
+  ebgpu::DevBuf          status;      // one device word, MeshDev::status
+  ebgpu::DevBuf          pts32, dist32; // FP64-point queries: the points rounded to FP32, FP32 results
+  uint64_t               hash = 0;      // FNV-1a of the vertex and triangle arrays given at creation
   uint32_t               num_split_roots = 0;
   bool                   grid_ready = false;
   float                  grid_lo[3]{}, grid_scale[3]{};
@@ -179,6 +260,12 @@ constexpr uint32_t kCostGrid = 32; // cells per axis of a mesh's packet cost gri
 // Build the mesh's packet cost grid once (a counting launch of 64 lattice queries per cell on
 // stream s, then a wait); defined with the query pipeline (api.cpp).
 ebg_status ensure_cost_grid(ebg_mesh* m, hipStream_t s);
+// The mesh query pipeline (presort, [bound network], FP32 traversal with split packets, FP64
+// recheck) over a caller-chosen workspace of at least n queries. With may_alloc false nothing is
+// allocated or created: split packets only if the cost grid already exists, the bound network
+// on the caller's stream, no traversal timing. The caller holds m->mu.
+ebg_status query_pipeline(ebg_mesh* m, const float* d_pts, uint64_t n, float* d_dist, uint32_t* d_tri,
+                          const ebg_query_opts& o, const WsView& ws, bool may_alloc);
 inline void mesh_retain(ebg_mesh* m) { m->refs.fetch_add(1); }
 inline void mesh_release(ebg_mesh* m)
 {

@@ -158,6 +158,9 @@ hipError_t launch_batch64(const BatchArgs& a, int K, int banks, hipStream_t s);
 
 // util.hip
 hipError_t launch_fill_u32(uint32_t* p, uint64_t n, uint32_t v, hipStream_t s);
+// n doubles -> floats (round to nearest even) and n floats -> doubles
+hipError_t launch_f64_to_f32(const double* in, float* out, uint64_t n, hipStream_t s);
+hipError_t launch_f32_to_f64(const float* in, double* out, uint64_t n, hipStream_t s);
 hipError_t launch_add_counts(unsigned long long* work, const uint32_t* recheck, const uint32_t* fallback, hipStream_t s);
 
 // Morton / sort (morton.hip)

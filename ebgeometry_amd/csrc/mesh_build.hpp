@@ -76,4 +76,23 @@ struct WideBvh
 bool build_wide_bvh(const float* blo, const float* bhi, std::size_t n, int K, int leaf_size, WideBvh& out,
                     std::string& err);
 
+// 64-bit FNV-1a over the bytes of the vertex and triangle arrays (identifies the mesh a trained
+// network belongs to).
+inline uint64_t mesh_hash(const float* xyz, uint64_t nv, const uint32_t* tri, uint64_t nt)
+{
+  uint64_t h   = 0xcbf29ce484222325ull;
+  auto     mix = [&h](const void* p, std::size_t bytes) {
+    const unsigned char* c = static_cast<const unsigned char*>(p);
+    for (std::size_t i = 0; i < bytes; i++) {
+      h ^= c[i];
+      h *= 0x100000001b3ull;
+    }
+  };
+  mix(&nv, sizeof(nv));
+  mix(&nt, sizeof(nt));
+  mix(xyz, sizeof(float) * 3 * nv);
+  mix(tri, sizeof(uint32_t) * 3 * nt);
+  return h;
+}
+
 } // namespace ebgpu

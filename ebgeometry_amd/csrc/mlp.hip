@@ -15,6 +15,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <string>
 #include <vector>
 
 namespace ebgpu {
@@ -314,11 +315,45 @@ __global__ __launch_bounds__(256) void zero_rows_kernel(float* __restrict__ c, i
 // over blockIdx.z in chunks of kSplitK: a 128 x 128 gradient is only 64 output tiles.
 constexpr int kSplitK = 512;
 
-hipError_t gemm(const GemmArgs& a0, hipStream_t s)
+// Element counts of the arrays a GEMM may touch, for the host-side check before its launch.
+struct GemmExtent
+{
+  int64_t a, b, c, mask, bias;
+};
+
+// The launch shape and every index gemm_kernel forms, checked on the host: M, N multiples of 16
+// (one 16 x 16 tile per workgroup, no edge guards), K and the split chunk multiples of 32 (one
+// MFMA per 32), strides >= 0, so the largest index of each operand is reached at its last row
+// and column and must lie inside the array.
+bool gemm_in_bounds(const GemmArgs& a, const GemmExtent& x, std::string* why)
+{
+  auto bad = [&](const char* w) {
+    if (why) *why = w;
+    return false;
+  };
+  if (a.M <= 0 || a.N <= 0 || a.K <= 0 || a.M % 16 || a.N % 16 || a.K % 32) return bad("gemm: M, N % 16 or K % 32");
+  if (a.kchunk <= 0 || a.kchunk % 32 || a.kchunk > a.K) return bad("gemm: split chunk not a multiple of 32");
+  if (a.as0 < 0 || a.as1 < 0 || a.bs0 < 0 || a.bs1 < 0 || a.ldc < a.N || (a.mask && a.ms0 < a.N))
+    return bad("gemm: negative stride or overlapping rows");
+  if (!a.A || !a.B || (!a.Cf && !a.Cb)) return bad("gemm: null operand");
+  const int64_t M = a.M, N = a.N, K = a.K;
+  if ((M - 1) * a.as0 + (K - 1) * a.as1 >= x.a) return bad("gemm: A read past its end");
+  if ((K - 1) * a.bs0 + (N - 1) * a.bs1 >= x.b) return bad("gemm: B read past its end");
+  if ((M - 1) * a.ldc + N - 1 >= x.c) return bad("gemm: C written past its end");
+  if (a.mask && (M - 1) * a.ms0 + N - 1 >= x.mask) return bad("gemm: mask read past its end");
+  if (a.bias && N > x.bias) return bad("gemm: bias read past its end");
+  return true;
+}
+
+// With check_only nothing is enqueued (and the pointers may be placeholders): only the host
+// check runs.
+hipError_t gemm(const GemmArgs& a0, const GemmExtent& x, hipStream_t s, bool check_only, std::string* why)
 {
   GemmArgs   a     = a0;
   const bool plain = a.Cf && !a.Cb && !a.bias && !a.mask && !a.relu;
   a.kchunk         = (plain && a.K >= 4 * kSplitK) ? kSplitK : a.K;
+  if (!gemm_in_bounds(a, x, why)) return hipErrorInvalidValue;
+  if (check_only) return hipSuccess;
   const int splits = (a.K + a.kchunk - 1) / a.kchunk;
   if (splits > 1)
     hipLaunchKernelGGL(zero_rows_kernel, dim3((a.M * a.N + 255) / 256), dim3(256), 0, s, a.Cf, a.M, a.N, a.ldc);
@@ -341,33 +376,42 @@ __global__ __launch_bounds__(256) void features_kernel(const float* __restrict__
 // an asymmetric squared error in that log space, e = y - log(|d|/diag + c), loss = w e^2 with
 // w = under_weight where e < 0 (a too-small cull radius forces the exact fallback) and 1
 // otherwise. The prediction is (exp(y) - c) * diag.
+// One wavefront per 64 rows; a row's 128 BF16 activations are one coalesced 256-byte read (two
+// per lane), reduced across the wavefront. Rows in [n, rows) get a zero gradient.
 __global__ __launch_bounds__(256) void out_fwd_bwd_kernel(const uint16_t* __restrict__ h4, const float* __restrict__ master,
                                                           const float* __restrict__ target, int n, int rows,
                                                           float inv_diag, float inv_n, float under_weight,
                                                           float* __restrict__ gout, float* __restrict__ loss_sum,
                                                           uint16_t* __restrict__ d4)
 {
-  const int i = blockIdx.x * 256 + threadIdx.x;
-  if (i >= rows) return;
-  const float* w5 = master + kOffW5;
-  float        gi = 0.0f, li = 0.0f;
-  if (i < n) {
-    float y = master[kOffB5];
-    for (int k = 0; k < kMlpHidden; k++) y += bf2f(h4[(size_t)i * kMlpHidden + k]) * w5[k];
-    const float lt = logf(fabsf(target[i]) * inv_diag + kMlpLogC);
-    const float e  = y - lt;
-    const float w  = e < 0.0f ? under_weight : 1.0f;
-    li             = w * e * e;
-    gi             = 2.0f * w * e * inv_n;
+  const int   lane = threadIdx.x & 63;
+  const int   row0 = (blockIdx.x * 4 + (threadIdx.x >> 6)) * 64;
+  const float wa = master[kOffW5 + 2 * lane], wb = master[kOffW5 + 2 * lane + 1];
+  const float b5 = master[kOffB5];
+  float       lsum = 0.0f;
+  for (int r = 0; r < 64; r++) {
+    const int i = row0 + r; // wave-uniform
+    if (i >= rows) break;
+    const uint32_t hh = *reinterpret_cast<const uint32_t*>(h4 + (size_t)i * kMlpHidden + 2 * lane);
+    float          y  = bf2f((uint16_t)(hh & 0xFFFFu)) * wa + bf2f((uint16_t)(hh >> 16)) * wb;
+    for (int o = 32; o > 0; o >>= 1) y += __shfl_xor(y, o, 64);
+    y += b5;
+    float gi = 0.0f;
+    if (i < n) {
+      const float lt = logf(fabsf(target[i]) * inv_diag + kMlpLogC);
+      const float e  = y - lt;
+      const float w  = e < 0.0f ? under_weight : 1.0f;
+      lsum += w * e * e;
+      gi = 2.0f * w * e * inv_n;
+    }
+    if (lane == 0) gout[i] = gi;
+    const uint16_t h0 = (uint16_t)(hh & 0xFFFFu), h1 = (uint16_t)(hh >> 16);
+    const uint16_t d0 = f2bf((h0 == 0 || (h0 & 0x8000u)) ? 0.0f : gi * wa);
+    const uint16_t d1 = f2bf((h1 == 0 || (h1 & 0x8000u)) ? 0.0f : gi * wb);
+    *reinterpret_cast<uint32_t*>(d4 + (size_t)i * kMlpHidden + 2 * lane) = (uint32_t)d0 | ((uint32_t)d1 << 16);
   }
-  gout[i] = gi;
-  for (int k = 0; k < kMlpHidden; k++) {
-    const uint16_t h = h4[(size_t)i * kMlpHidden + k];
-    d4[(size_t)i * kMlpHidden + k] = f2bf((h == 0 || (h & 0x8000u)) ? 0.0f : gi * w5[k]);
-  }
-  // wave-reduce the loss then one atomic per wave
-  for (int o = 32; o > 0; o >>= 1) li += __shfl_xor(li, o, 64);
-  if ((threadIdx.x & 63) == 0) atomicAdd(loss_sum, li);
+  // every lane holds the same row sums: one atomic per wavefront
+  if (lane == 0 && lsum != 0.0f) atomicAdd(loss_sum, lsum);
 }
 
 // dw5[k] = sum_i h4[i,k] g[i] (block k < 128), db5 = sum_i g[i] (block 128)
@@ -688,19 +732,31 @@ hipError_t mlp_forward(const Mlp& m, const float* pts, const uint32_t* idx, uint
   return hipGetLastError();
 }
 
-hipError_t mlp_loss_grad(Mlp& m, MlpTrainBuffers& tb, const float* pts, const float* target, int n, hipStream_t s)
+// Launch sequence of one training step over n rows (n <= capacity rows of the buffers). Every
+// GEMM's shape and extents are checked on the host first; the other kernels index rows < rows
+// <= capacity, 128 or 64 columns, guarded by their grid. With check_only nothing is enqueued and
+// the buffer pointers may be placeholders.
+static hipError_t loss_grad_launches(Mlp& m, MlpTrainBuffers& tb, int capacity, const float* pts, const float* target,
+                                     int n, hipStream_t s, bool check_only, std::string* why)
 {
-  hipError_t e = tb.ensure(n);
-  if (e != hipSuccess) return e;
-  const int    rows = (n + 31) / 32 * 32;
-  const MlpDev d    = dev_of(m);
-  hipLaunchKernelGGL(features_kernel, dim3((rows + 255) / 256), dim3(256), 0, s, pts, n, rows, d, tb.x0);
+  if (n < 1 || n > capacity || capacity % 32) {
+    if (why) *why = "training batch larger than its buffers";
+    return hipErrorInvalidValue;
+  }
+  hipError_t    e;
+  const int     rows = (n + 31) / 32 * 32;
+  const int64_t cap  = capacity;
+  const MlpDev  d    = dev_of(m);
+  if (!check_only)
+    hipLaunchKernelGGL(features_kernel, dim3((rows + 255) / 256), dim3(256), 0, s, pts, n, rows, d, tb.x0);
   // forward
   const uint16_t* in    = tb.x0;
   int             in_ld = kMlpInPad;
   const int       wb[4] = {kBfW1, kBfW2, kBfW3, kBfW4};
   const int       bo[4] = {kOffB1, kOffB2, kOffB3, kOffB4};
   const int       wo[4] = {kOffW1, kOffW2, kOffW3, kOffW4};
+  const int64_t   wn[4] = {kMlpInPad * kMlpHidden, kMlpHidden * kMlpHidden, kMlpHidden * kMlpHidden,
+                           kMlpHidden * kMlpHidden};
   for (int L = 0; L < 4; L++) {
     GemmArgs a{};
     a.A    = in;
@@ -716,14 +772,16 @@ hipError_t mlp_loss_grad(Mlp& m, MlpTrainBuffers& tb, const float* pts, const fl
     a.relu = 1;
     a.Cb   = tb.h[L];
     a.ldc  = kMlpHidden;
-    if ((e = gemm(a, s)) != hipSuccess) return e;
+    if ((e = gemm(a, {cap * in_ld, wn[L], cap * kMlpHidden, 0, kMlpHidden}, s, check_only, why)) != hipSuccess) return e;
     in    = tb.h[L];
     in_ld = kMlpHidden;
   }
   // loss (accumulated into tb.loss[0]; the caller clears it) + output layer backward
-  hipLaunchKernelGGL(out_fwd_bwd_kernel, dim3((rows + 255) / 256), dim3(256), 0, s, tb.h[3], m.master, target, n,
-                     rows, 1.0f / m.diag, 1.0f / (float)n, m.under_weight, tb.g, tb.loss, tb.dl[3]);
-  hipLaunchKernelGGL(out_wgrad_kernel, dim3(kMlpHidden + 1), dim3(256), 0, s, tb.h[3], tb.g, rows, m.grad);
+  if (!check_only) {
+    hipLaunchKernelGGL(out_fwd_bwd_kernel, dim3((rows + 255) / 256), dim3(256), 0, s, tb.h[3], m.master, target, n,
+                       rows, 1.0f / m.diag, 1.0f / (float)n, m.under_weight, tb.g, tb.loss, tb.dl[3]);
+    hipLaunchKernelGGL(out_wgrad_kernel, dim3(kMlpHidden + 1), dim3(256), 0, s, tb.h[3], tb.g, rows, m.grad);
+  }
   // hidden layers, last to first
   for (int L = 3; L >= 0; L--) {
     const uint16_t* hin    = L > 0 ? tb.h[L - 1] : tb.x0;
@@ -740,8 +798,11 @@ hipError_t mlp_loss_grad(Mlp& m, MlpTrainBuffers& tb, const float* pts, const fl
     w.K   = rows;
     w.Cf  = m.grad + wo[L];
     w.ldc = kMlpHidden;
-    if ((e = gemm(w, s)) != hipSuccess) return e;
-    hipLaunchKernelGGL(colsum_kernel, dim3(kMlpHidden), dim3(256), 0, s, tb.dl[L], rows, kMlpHidden, m.grad + bo[L]);
+    if ((e = gemm(w, {cap * hin_ld, cap * kMlpHidden, kMaster - wo[L], 0, 0}, s, check_only, why)) != hipSuccess)
+      return e;
+    if (!check_only)
+      hipLaunchKernelGGL(colsum_kernel, dim3(kMlpHidden), dim3(256), 0, s, tb.dl[L], rows, kMlpHidden,
+                         m.grad + bo[L]);
     if (L > 0) {
       GemmArgs p{};
       p.A    = tb.dl[L];
@@ -757,10 +818,46 @@ hipError_t mlp_loss_grad(Mlp& m, MlpTrainBuffers& tb, const float* pts, const fl
       p.ms0  = kMlpHidden;
       p.Cb   = tb.dl[L - 1];
       p.ldc  = kMlpHidden;
-      if ((e = gemm(p, s)) != hipSuccess) return e;
+      if ((e = gemm(p, {cap * kMlpHidden, wn[L], cap * kMlpHidden, cap * kMlpHidden, 0}, s, check_only, why)) !=
+          hipSuccess)
+        return e;
     }
   }
-  return hipGetLastError();
+  return check_only ? hipSuccess : hipGetLastError();
+}
+
+hipError_t mlp_loss_grad(Mlp& m, MlpTrainBuffers& tb, const float* pts, const float* target, int n, hipStream_t s,
+                         std::string* why)
+{
+  hipError_t e = tb.ensure(n);
+  if (e != hipSuccess) return e;
+  return loss_grad_launches(m, tb, tb.batch, pts, target, n, s, false, why);
+}
+
+bool mlp_check_train_schedule(uint64_t num_samples, int batch, std::string* why)
+{
+  // the buffers ensure(batch) allocates, with placeholder addresses: only shapes and extents
+  // are examined
+  Mlp             m;
+  MlpTrainBuffers tb;
+  const int       cap  = (batch + 31) / 32 * 32;
+  uint16_t*       fake = reinterpret_cast<uint16_t*>(uintptr_t(256));
+  m.master             = reinterpret_cast<float*>(fake);
+  m.grad               = reinterpret_cast<float*>(fake);
+  m.wtrain             = fake;
+  tb.x0                = fake;
+  for (int i = 0; i < 4; i++) tb.h[i] = tb.dl[i] = fake;
+  std::vector<int> sizes{(int)std::min<uint64_t>(num_samples, (uint64_t)batch)};
+  if (num_samples > (uint64_t)batch && num_samples % (uint64_t)batch) sizes.push_back(int(num_samples % batch));
+  bool ok = true;
+  for (int n : sizes)
+    if (loss_grad_launches(m, tb, cap, nullptr, nullptr, n, nullptr, true, why) != hipSuccess) ok = false;
+  // the placeholders are not device memory: keep the destructors away from them
+  m.master = m.grad = nullptr;
+  m.wtrain          = nullptr;
+  tb.x0             = nullptr;
+  for (int i = 0; i < 4; i++) tb.h[i] = tb.dl[i] = nullptr;
+  return ok;
 }
 
 hipError_t mlp_adam(Mlp& m, float lr, hipStream_t s)

@@ -19,6 +19,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
+#include <string>
 
 namespace ebgpu {
 
@@ -102,8 +103,12 @@ struct MlpTrainBuffers
 
 // Loss and full parameter gradient (into m.grad, master layout) for n <= buffers.batch points
 // with targets |d| (physical units). The summed loss is added to tb.loss[0].
+// On failure *why (nullable) names the check that failed.
 hipError_t mlp_loss_grad(Mlp& m, MlpTrainBuffers& tb, const float* pts, const float* target, int n,
-                         hipStream_t s);
+                         hipStream_t s, std::string* why = nullptr);
+// Host-only: run the shape and extent checks of every launch of the training steps that
+// ebg_mlp_train enqueues for these sizes (full batches and the last, partial one). No GPU.
+bool mlp_check_train_schedule(uint64_t num_samples, int batch, std::string* why);
 // One Adam step on m (uses m.grad), then refresh the BF16 copies.
 hipError_t mlp_adam(Mlp& m, float lr, hipStream_t s);
 // Copy the gradient out in the public parameter order (device pointer, kMlpParams floats).

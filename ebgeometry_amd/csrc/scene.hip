@@ -72,6 +72,10 @@ __device__ __forceinline__ void scene_traverse(const SceneDev& sc, R px, R py, R
     if (e.cnt == 0u) {
       if (COUNT) { c.n_inner += need ? 1u : 0u; c.w_inner++; }
       push_children<4, NB, R, true>(sload_record(nodes + uni(e.ref)), px, py, pz, bound, st, top, lane, rep);
+      if (top > 64 * NB) { // beyond the build-time worst case: record it, keep what is held
+        if (lane == 0 && sc.status) atomicOr(sc.status, kStatusStackOverflow);
+        top = 64 * NB;
+      }
       continue;
     }
     const uint32_t first = uni(e.ref), cnt = uni(e.cnt);

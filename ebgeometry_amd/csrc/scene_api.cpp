@@ -21,7 +21,7 @@ using ebgpu::Workspace;
 struct ebg_scene
 {
   std::vector<ebg_mesh*> meshes; // retained
-  DevBuf                 nodes, prims, prims64, meshdev;
+  DevBuf                 nodes, prims, prims64, meshdev, status;
   ebgpu::SceneDev        dev{};
   int                    KM = 4, banks = 1, stack_need = 0;
   std::size_t            num_prims = 0, num_nodes = 0;
@@ -220,7 +220,8 @@ ebg_status ebg_scene_create(const ebg_prim* prims, uint64_t n, ebg_mesh* const* 
   for (uint64_t j = 0; j < num_meshes; j++) md[j] = meshes[j]->dev;
   hipError_t e;
   if ((e = upload(sc->nodes, wb.nodes)) != hipSuccess || (e = upload(sc->prims, rec_o)) != hipSuccess ||
-      (e = upload(sc->prims64, rec64_o)) != hipSuccess || (e = upload(sc->meshdev, md)) != hipSuccess) {
+      (e = upload(sc->prims64, rec64_o)) != hipSuccess || (e = upload(sc->meshdev, md)) != hipSuccess ||
+      (e = upload(sc->status, std::vector<uint32_t>(16, 0u))) != hipSuccess) {
     delete sc;
     return fail(e == hipErrorOutOfMemory ? EBG_OOM : EBG_HIP_ERROR, std::string("scene upload: ") + hipGetErrorString(e));
   }
@@ -236,6 +237,7 @@ ebg_status ebg_scene_create(const ebg_prim* prims, uint64_t n, ebg_mesh* const* 
   sc->dev.prims   = sc->prims.as<ebgpu::PrimRec>();
   sc->dev.prims64 = sc->prims64.as<ebgpu::PrimRec64>();
   sc->dev.meshes  = sc->meshdev.as<ebgpu::MeshDev>();
+  sc->dev.status  = sc->status.as<uint32_t>();
   *out            = sc;
   return EBG_OK;
 }
@@ -282,7 +284,7 @@ ebg_status ebg_scene_query(const ebg_scene* csc, const float* d_pts, uint64_t n,
   EBG_HIP(ebgpu::launch_fill_u32(cnt, 2, 0u, s));
   const uint32_t* idx = nullptr;
   if (o.presort && n > 1) {
-    EBG_HIP(ws.presort(d_pts, n, s));
+    EBG_HIP(ebgpu::presort(ws.view(), d_pts, n, s));
     idx = ws.perm.as<uint32_t>();
   }
   const bool       recheck = o.fp64_recheck || o.fp64_all;
@@ -334,6 +336,17 @@ ebg_status ebg_scene_query_host(const ebg_scene* sc, const float* pts, uint64_t 
   if (st != EBG_OK) return st;
   EBG_HIP(hipMemcpyAsync(value, d.p, 4 * n, hipMemcpyDeviceToHost, s));
   EBG_HIP(hipStreamSynchronize(s));
+  return EBG_OK;
+}
+
+ebg_status ebg_scene_check(const ebg_scene* sc)
+{
+  if (!sc) return fail(EBG_INVALID_ARG, "scene is null");
+  uint32_t w = 0;
+  EBG_HIP(hipDeviceSynchronize());
+  EBG_HIP(hipMemcpy(&w, sc->status.p, 4, hipMemcpyDeviceToHost));
+  if (w & ebgpu::kStatusStackOverflow)
+    return fail(EBG_STACK_OVERFLOW, "a walk of this scene's tree needed more stack than its build-time worst case");
   return EBG_OK;
 }
 

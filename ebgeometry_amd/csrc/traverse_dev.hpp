@@ -467,6 +467,13 @@ __device__ __forceinline__ void traverse_from(const MeshDev& m, const Entry& sta
     if (e.cnt == 0u) {
       if (COUNT) { n_inner += need ? 1u : 0u; w_inner++; }
       push_children<K, NB, R, WMIN>(sload_record(nodes + uni(e.ref)), px, py, pz, bound, st, top, lane, rep);
+      // The capacity is the tree's exact worst case, computed at build time; should a walk ever
+      // exceed it, the pushes beyond it were dropped: record that in the mesh's status word
+      // (the host reports it) and go on with the entries that are held.
+      if (top > 64 * NB) {
+        if (lane == 0 && m.status) atomicOr(m.status, kStatusStackOverflow);
+        top = 64 * NB;
+      }
     }
     else {
       // leaf: a triangle is evaluated by the whole wavefront (keeping the minimum is exact for

@@ -22,7 +22,35 @@ __global__ void add_counts_kernel(unsigned long long* work, const uint32_t* a, c
   }
 }
 
+// FP64 points rounded to FP32 (round to nearest even, as the host's (float) cast); FP32 results
+// widened to FP64.
+__global__ void f64_to_f32_kernel(const double* __restrict__ in, float* __restrict__ out, uint64_t n)
+{
+  const uint64_t i = (uint64_t)blockIdx.x * 256 + threadIdx.x;
+  if (i < n) out[i] = __double2float_rn(in[i]);
+}
+
+__global__ void f32_to_f64_kernel(const float* __restrict__ in, double* __restrict__ out, uint64_t n)
+{
+  const uint64_t i = (uint64_t)blockIdx.x * 256 + threadIdx.x;
+  if (i < n) out[i] = (double)in[i];
+}
+
 } // namespace
+
+hipError_t launch_f64_to_f32(const double* in, float* out, uint64_t n, hipStream_t s)
+{
+  if (n == 0) return hipSuccess;
+  hipLaunchKernelGGL(f64_to_f32_kernel, dim3((uint32_t)((n + 255) / 256)), dim3(256), 0, s, in, out, n);
+  return hipGetLastError();
+}
+
+hipError_t launch_f32_to_f64(const float* in, double* out, uint64_t n, hipStream_t s)
+{
+  if (n == 0) return hipSuccess;
+  hipLaunchKernelGGL(f32_to_f64_kernel, dim3((uint32_t)((n + 255) / 256)), dim3(256), 0, s, in, out, n);
+  return hipGetLastError();
+}
 
 hipError_t launch_fill_u32(uint32_t* p, uint64_t n, uint32_t v, hipStream_t s)
 {

@@ -146,6 +146,28 @@ void ebg_query_opts_default(ebg_query_opts* opts);
  * inside the same traversal launch; results are those of the plain traversal. */
 ebg_status ebg_query_sdf(const ebg_mesh* mesh, const float* d_pts, uint64_t n, float* d_dist,
                          uint32_t* d_tri, const ebg_query_opts* opts);
+/* Allocation-free form of ebg_query_sdf. No counterpart in the reference: for callers that
+ * cannot allocate device memory (code under HIP graph capture, serving loops, harnesses that
+ * forbid hipMalloc). Same arguments and results as ebg_query_sdf; it calls no hipMalloc, hipFree
+ * or other allocating routine and creates no stream or event. Its workspace is one fixed device
+ * array of 40 MiB per GPU, held from the library's first use on that GPU until the process ends,
+ * which limits a call to n <= EBG_NOALLOC_MAX_QUERIES (larger n: EBG_INVALID_ARG). Every call on
+ * one GPU shares that array: calls that overlap on one GPU (other streams, other threads) race
+ * and must be serialised by the caller. Differences from ebg_query_sdf: split packets are used
+ * only once an ebg_query_sdf call of >= 2^14 presorted queries has built the mesh's cost grid;
+ * the learned bound's network runs on opts->stream before the traversal (no second stream); the
+ * traversal is not timed (ebg_mesh_last_traversal_ms keeps the previous value). */
+#define EBG_NOALLOC_MAX_QUERIES 1048576u
+ebg_status ebg_query_sdf_noalloc(const ebg_mesh* mesh, const float* d_pts, uint64_t n, float* d_dist,
+                                 uint32_t* d_tri, const ebg_query_opts* opts);
+/* FP64 points and distances (the reference's T = double interface, SignedDistanceFunction<double>,
+ * Source/EBGeometry_SignedDistanceFunction.hpp:54-62): the points are rounded to FP32 on the
+ * device (round to nearest even), queried as by ebg_query_sdf, and the distances widened to
+ * double. Results are those of the rounded points: within 1e-5 diag of the reference's FP64
+ * value at the rounded point. Device pointers, asynchronous on opts->stream; uses a buffer of
+ * 16 bytes per query held by the mesh. */
+ebg_status ebg_query_sdf_f64(const ebg_mesh* mesh, const double* d_pts, uint64_t n, double* d_dist,
+                             uint32_t* d_tri, const ebg_query_opts* opts);
 /* Host-pointer convenience form; synchronous. */
 ebg_status ebg_query_sdf_host(const ebg_mesh* mesh, const float* pts, uint64_t n, float* dist,
                               uint32_t* tri, const ebg_query_opts* opts);
@@ -155,6 +177,13 @@ ebg_status ebg_query_sdf_host(const ebg_mesh* mesh, const float* pts, uint64_t n
  * packets, and their resolve; the FP64 pass is not included). Milliseconds, or <0 if none.
  * Waits for that launch. */
 double ebg_mesh_last_traversal_ms(const ebg_mesh* mesh);
+/* Waits for the device, then reports whether any traversal of this mesh since its creation
+ * (alone, in a batch or instanced in a scene) needed more traversal stack than the build-time
+ * worst case the handle was sized for: EBG_STACK_OVERFLOW (those queries' results are not
+ * exact), else EBG_OK. The capacity is the tree's exact worst case, so this is a guard, not an
+ * expected outcome. No counterpart in the reference (its stack is a fixed 256 entries,
+ * Source/EBGeometry_BVHImplem.hpp:1138). */
+ebg_status ebg_mesh_check(const ebg_mesh* mesh);
 /* Number of split packets of the last query on `mesh` (0 if none were split, <0 on error).
  * Diagnostic with no counterpart in the reference; waits for the device. */
 int64_t ebg_mesh_last_split_packets(const ebg_mesh* mesh);
@@ -211,6 +240,8 @@ ebg_status ebg_scene_query(const ebg_scene* scene, const float* d_pts, uint64_t 
 ebg_status ebg_scene_query_host(const ebg_scene* scene, const float* pts, uint64_t n, float* value,
                                 const ebg_query_opts* opts);
 double     ebg_scene_last_traversal_ms(const ebg_scene* scene);
+/* As ebg_mesh_check, for the scene's own tree walk (its meshes report through ebg_mesh_check). */
+ebg_status ebg_scene_check(const ebg_scene* scene);
 
 /* ---------------------------------------------------------------------------------------- */
 /* Batched queries over many meshes -- many TriMeshSDF objects (MeshDistanceFunctions.hpp:    */
@@ -334,6 +365,17 @@ ebg_status ebg_mlp_predict(const ebg_mesh* mesh, const float* d_pts, uint64_t n,
 uint64_t   ebg_mlp_param_count(void);
 ebg_status ebg_mlp_get_params(const ebg_mesh* mesh, float* params);
 ebg_status ebg_mlp_set_params(ebg_mesh* mesh, const float* params);
+/* Save the mesh's trained network (parameters, input box, loss settings) to a file, tagged with a
+ * 64-bit hash of the vertex and triangle arrays the mesh was created from; ebg_mlp_load reads it
+ * back into a mesh created from the same arrays (another mesh: EBG_INVALID_ARG) and arms the
+ * learned bound, so a network is trained once per mesh. No counterpart in the reference. */
+ebg_status ebg_mlp_save(const ebg_mesh* mesh, const char* path);
+ebg_status ebg_mlp_load(ebg_mesh* mesh, const char* path);
+/* Host-only check (no GPU needed, nothing is allocated or launched): the launch shapes and the
+ * largest index of every buffer access of each training step ebg_mlp_train would enqueue for
+ * num_samples pairs in batches of `batch` (full batches and the last, partial one) against the
+ * buffers it allocates. EBG_INVALID_ARG names the failing check. ebg_mlp_train runs it first. */
+ebg_status ebg_mlp_check_schedule(uint64_t num_samples, int32_t batch);
 /* Training loss summed over n <= 65536 device points with device targets |d| (d_loss, one
  * float, nullable) and its gradient (d_grad, ebg_mlp_param_count floats, parameter order
  * above). The loss is the one the trainer minimises: the network output y estimates

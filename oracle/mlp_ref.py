@@ -92,3 +92,32 @@ def predict(theta, pts, lo, scale, diag, bf16_mode=False):
     for W, b in zip([W1, W2, W3, W4], [b1, b2, b3, b4]):
         h = r(np.maximum(h @ r(W) + b, 0.0))
     return (np.exp(h @ w5 + b5[0]) - LOSS_C) * diag
+
+
+def adam_train(theta, pts, target, lo, scale, diag, batch, epochs, lr=2e-3, bf16_mode=True, under_weight=LOSS_UNDER):
+    """The GPU trainer's schedule restated: Adam (beta 0.9 / 0.999, eps 1e-8, bias-corrected)
+    over consecutive batches (the last one partial), learning rate decayed by a cosine from lr
+    to 0.05 lr over all steps. Returns (theta, mean loss of each epoch)."""
+    theta = np.array(theta, np.float64)
+    m = np.zeros_like(theta)
+    v = np.zeros_like(theta)
+    n = pts.shape[0]
+    spe = (n + batch - 1) // batch
+    total = float(spe * epochs)
+    k = 0
+    losses = []
+    for _ in range(epochs):
+        acc = 0.0
+        for b in range(0, n, batch):
+            nb = min(batch, n - b)
+            L, g = loss_and_grad(theta, pts[b:b + nb], target[b:b + nb], lo, scale, diag, bf16_mode, under_weight)
+            acc += L * nb
+            step_lr = lr * (0.05 + 0.95 * 0.5 * (1.0 + np.cos(np.pi * k / total)))
+            k += 1
+            m = 0.9 * m + 0.1 * g
+            v = 0.999 * v + 0.001 * g * g
+            mh = m / (1.0 - 0.9 ** k)
+            vh = v / (1.0 - 0.999 ** k)
+            theta = theta - step_lr * mh / (np.sqrt(vh) + 1e-8)
+        losses.append(acc / n)
+    return theta, losses

@@ -86,3 +86,16 @@ def test_mesh_create_validates_without_gpu():
     v[0, 0] = np.nan
     st = L.ebg_mesh_create(v.ctypes.data, 3, np.array([[0, 1, 2]], np.uint32).ctypes.data, 1, None, C.byref(h))
     assert st == 1
+
+
+def test_training_schedule_checks_pass_without_a_gpu():
+    """Every launch of the training steps ebg_mlp_train enqueues (shapes, split-K chunks and the
+    largest index of each buffer access) is checked on the host for the schedules the tests and
+    the bench run: the learned-bound test's 400,000 pairs in batches of 2,048 (last batch 640),
+    the bench's 4M in 4,096, the gradient tests' single batches of 1,000 and 4,100 rows."""
+    from ebgeometry_amd import _lib
+    L = _lib.lib()
+    for n, b in [(400_000, 2048), (4_000_000, 4096), (1000, 1024), (4100, 4100), (65536, 65536), (100, 32),
+                 (8_000_000, 16384), (33, 33)]:
+        assert L.ebg_mlp_check_schedule(n, b) == 0, (n, b, L.ebg_last_error())
+    assert L.ebg_mlp_check_schedule(1000, 16) != 0  # batch below the kernels' 32-row granularity
