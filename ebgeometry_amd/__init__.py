@@ -7,12 +7,13 @@ ImplicitFunction / SignedDistanceFunction / TriMeshSDF interface. Compute runs i
 libebgpu.so (hand-written HIP kernels + C ABI, include/ebgpu.h); this package is the host
 mirror of the reference API over that ABI.
 """
-from .sdf import (Build, ImplicitFunction, SignedDistanceFunction, TriMeshSDF, morton_codes, morton_sort,
-                  pseudo_normals)
+from .sdf import (Build, ImplicitFunction, SignedDistanceFunction, TriMeshSDF, grid_points, morton_codes,
+                  morton_sort, pseudo_normals)
 from .batch import BatchPlan, MeshBatch
 from .csg import (AABB, BoxSDF, BVHUnion, BVHUnionIF, Rotate, RotateIF, Scale, ScaleIF, SphereSDF, Translate,
                   TranslateIF)
 
-__all__ = ["Build", "ImplicitFunction", "SignedDistanceFunction", "TriMeshSDF", "morton_codes", "morton_sort",
+__all__ = ["Build", "ImplicitFunction", "SignedDistanceFunction", "TriMeshSDF", "grid_points", "morton_codes",
+           "morton_sort",
            "pseudo_normals", "AABB", "BoxSDF", "BVHUnion", "BVHUnionIF", "Rotate", "RotateIF", "Scale", "ScaleIF",
            "SphereSDF", "Translate", "TranslateIF", "MeshBatch", "BatchPlan"]

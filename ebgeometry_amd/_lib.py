@@ -71,6 +71,8 @@ SIGNATURES = [
     ("ebg_query_sdf_host", C.c_int, [_vp, _vp, _u64, _vp, _vp, C.POINTER(QueryOpts)]),
     ("ebg_query_sdf_noalloc", C.c_int, [_vp, _vp, _u64, _vp, _vp, C.POINTER(QueryOpts)]),
     ("ebg_query_sdf_f64", C.c_int, [_vp, _vp, _u64, _vp, _vp, C.POINTER(QueryOpts)]),
+    ("ebg_query_grid", C.c_int, [_vp, _vp, _vp, _vp, _vp, C.POINTER(QueryOpts)]),
+    ("ebg_query_grid_host", C.c_int, [_vp, _vp, _vp, _vp, _vp, C.POINTER(QueryOpts)]),
     ("ebg_mesh_last_traversal_ms", C.c_double, [_vp]),
     ("ebg_mesh_last_split_packets", C.c_int64, [_vp]),
     ("ebg_mesh_check", C.c_int, [_vp]),
@@ -85,6 +87,8 @@ SIGNATURES = [
     ("ebg_scene_bounds", C.c_int, [_vp, _vp, _vp, _vp]),
     ("ebg_scene_query", C.c_int, [_vp, _vp, _u64, _vp, C.POINTER(QueryOpts)]),
     ("ebg_scene_query_host", C.c_int, [_vp, _vp, _u64, _vp, C.POINTER(QueryOpts)]),
+    ("ebg_scene_query_grid", C.c_int, [_vp, _vp, _vp, _vp, _vp, C.POINTER(QueryOpts)]),
+    ("ebg_scene_query_grid_host", C.c_int, [_vp, _vp, _vp, _vp, _vp, C.POINTER(QueryOpts)]),
     ("ebg_scene_last_traversal_ms", C.c_double, [_vp]),
     ("ebg_batch_create", C.c_int, [_vp, _vp, _vp, _vp, _u64, C.POINTER(BatchOpts), C.POINTER(_vp)]),
     ("ebg_batch_destroy", C.c_int, [_vp]),

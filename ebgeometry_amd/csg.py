@@ -258,6 +258,25 @@ class BVHUnionIF(ImplicitFunction):
         check(lib().ebg_scene_query_host(self._h, p.ctypes.data, p.shape[0], v.ctypes.data, C.byref(o)))
         return float(v[0]) if single else v
 
+    def query_grid(self, origin, spacing, dims, device=None, out=None, **kw):
+        """Union value at every lattice point origin + (i, j, k) * spacing, shape
+        (dims[2], dims[1], dims[0]) (x fastest). CUDA tensor with device, else numpy."""
+        from .sdf import _grid_args
+        o3, s3, n3 = _grid_args(origin, spacing, dims)
+        shape = (int(n3[2]), int(n3[1]), int(n3[0]))
+        if device is not None:
+            torch = _torch()
+            v = out if out is not None else torch.empty(shape, dtype=torch.float32, device=device)
+            if "stream" not in kw:
+                kw["stream"] = torch.cuda.current_stream(v.device).cuda_stream
+            o = self.query_opts(**kw)
+            check(lib().ebg_scene_query_grid(self._h, o3, s3, n3, v.data_ptr(), C.byref(o)))
+            return v
+        v = np.empty(shape, dtype=np.float32)
+        o = self.query_opts(**kw)
+        check(lib().ebg_scene_query_grid_host(self._h, o3, s3, n3, v.ctypes.data, C.byref(o)))
+        return v
+
     def last_traversal_ms(self) -> float:
         return float(lib().ebg_scene_last_traversal_ms(self._h))
 

@@ -169,6 +169,7 @@ static ebg_status upload_mesh(ebg_mesh* m, ebgpu::PackedMesh& pm)
   }
   d.diag          = float(pm.diag);
   d.num_tris      = uint32_t(m->nt);
+  d.hmax          = pm.hmax;
   EBG_HIP(upload(m->status, std::vector<uint32_t>(16, 0u)));
   d.status = m->status.as<uint32_t>();
   m->device_bytes = m->nodes.bytes + m->tris.bytes + m->tripos.bytes + m->tsph.bytes + m->feat.bytes + m->vn32.bytes +
@@ -344,8 +345,10 @@ ebg_status query_pipeline(ebg_mesh* m, const float* d_pts, uint64_t n, float* d_
     EBG_HIP(hipEventCreate(&m->ev1));
   }
   const bool timed = m->ev0 && may_alloc;
-  // counting launches run unsplit: the counters describe the plain traversal's own work
-  const bool heavy_first = o.presort && n >= kHeavyMinQ && !o.work && (may_alloc || m->grid_ready);
+  // counting launches run unsplit: the counters describe the plain traversal's own work; so do
+  // closest-face queries (each lane decides its face from the band of its own whole walk)
+  const bool face        = d_tri != nullptr;
+  const bool heavy_first = o.presort && n >= kHeavyMinQ && !o.work && !face && (may_alloc || m->grid_ready);
   m->last_split           = heavy_first;
   m->last_split_cap       = heavy_first ? Workspace::heavy_cap(n) : 0;
   if (heavy_first) {
@@ -412,7 +415,8 @@ ebg_status query_pipeline(ebg_mesh* m, const float* d_pts, uint64_t n, float* d_
   }
   if (fork) EBG_HIP(hipStreamWaitEvent(s, m->ev_join, 0));
 
-  const bool recheck = o.fp64_recheck || o.fp64_all;
+  // the FP64 pass also decides the closest face of the lanes whose FP32 band overflowed
+  const bool recheck = o.fp64_recheck || o.fp64_all || face;
   QueryArgs a{};
   a.pts            = d_pts;
   a.idx            = idx;
@@ -420,7 +424,7 @@ ebg_status query_pipeline(ebg_mesh* m, const float* d_pts, uint64_t n, float* d_
   a.seed_r         = seeds;
   a.out_d          = d_dist;
   a.out_tri        = d_tri;
-  a.recheck_abs    = o.fp64_all ? INFINITY : (recheck ? float(o.recheck_eps * m->diag) : 0.0f);
+  a.recheck_abs    = o.fp64_all ? INFINITY : (o.fp64_recheck ? float(o.recheck_eps * m->diag) : 0.0f);
   a.recheck_list   = recheck ? ws.recheck_list : nullptr;
   a.recheck_count  = cnt;
   a.recheck_mask   = recheck ? ws.rmask : nullptr;
@@ -518,6 +522,74 @@ ebg_status ebg_query_sdf_f64(const ebg_mesh* cm, const double* d_pts, uint64_t n
   const ebg_status st = ebg_query_sdf(m, m->pts32.as<float>(), n, m->dist32.as<float>(), d_tri, &o);
   if (st != EBG_OK) return st;
   EBG_HIP(ebgpu::launch_f32_to_f64(m->dist32.as<float>(), d_dist, n, s));
+  return EBG_OK;
+}
+
+} // extern "C"
+
+namespace ebgpu {
+// Validate a lattice (origin, spacing, dims) and its point count.
+ebg_status grid_spec(const double* origin, const double* spacing, const uint64_t* dims, GridSpec& g, uint64_t& n)
+{
+  if (!origin || !spacing || !dims) return fail(EBG_INVALID_ARG, "null origin, spacing or dims");
+  n = 1;
+  for (int c = 0; c < 3; c++) {
+    if (!std::isfinite(origin[c]) || !std::isfinite(spacing[c]))
+      return fail(EBG_INVALID_ARG, "non-finite lattice origin or spacing");
+    if (dims[c] > 0xFFFFFFFFull) return fail(EBG_INVALID_ARG, "lattice dimension above 2^32-1");
+    g.origin[c]  = origin[c];
+    g.spacing[c] = spacing[c];
+    g.dims[c]    = dims[c];
+    n            = dims[c] == 0 ? 0 : n * dims[c];
+    if (n > 0xFFFFFFFFull) return fail(EBG_INVALID_ARG, "at most 2^32-1 lattice points per call");
+  }
+  return EBG_OK;
+}
+} // namespace ebgpu
+
+extern "C" {
+
+ebg_status ebg_query_grid(const ebg_mesh* cm, const double* origin, const double* spacing, const uint64_t* dims,
+                          float* d_dist, const ebg_query_opts* opts)
+{
+  if (!cm) return fail(EBG_INVALID_ARG, "mesh is null");
+  ebgpu::GridSpec g{};
+  uint64_t        n  = 0;
+  const ebg_status gs = ebgpu::grid_spec(origin, spacing, dims, g, n);
+  if (gs != EBG_OK) return gs;
+  if (n == 0) return EBG_OK;
+  if (!d_dist) return fail(EBG_INVALID_ARG, "null output array");
+  ebg_mesh*      m = const_cast<ebg_mesh*>(cm);
+  ebg_query_opts o;
+  ebg_query_opts_default(&o);
+  if (opts) o = *opts;
+  hipStream_t                           s = static_cast<hipStream_t>(o.stream);
+  std::lock_guard<std::recursive_mutex> lock(m->mu);
+  EBG_HIP(m->pts32.reserve(12 * n));
+  EBG_HIP(ebgpu::launch_grid_points(g, m->pts32.as<float>(), s));
+  return ebg_query_sdf(m, m->pts32.as<float>(), n, d_dist, nullptr, &o);
+}
+
+ebg_status ebg_query_grid_host(const ebg_mesh* m, const double* origin, const double* spacing, const uint64_t* dims,
+                               float* dist, const ebg_query_opts* opts)
+{
+  if (!m) return fail(EBG_INVALID_ARG, "mesh is null");
+  ebgpu::GridSpec g{};
+  uint64_t        n  = 0;
+  const ebg_status gs = ebgpu::grid_spec(origin, spacing, dims, g, n);
+  if (gs != EBG_OK) return gs;
+  if (n == 0) return EBG_OK;
+  if (!dist) return fail(EBG_INVALID_ARG, "null output array");
+  ebg_query_opts o;
+  ebg_query_opts_default(&o);
+  if (opts) o = *opts;
+  hipStream_t s = static_cast<hipStream_t>(o.stream);
+  DevBuf      d;
+  EBG_HIP(d.reserve(4 * n));
+  const ebg_status st = ebg_query_grid(m, origin, spacing, dims, d.as<float>(), &o);
+  if (st != EBG_OK) return st;
+  EBG_HIP(hipMemcpyAsync(dist, d.p, 4 * n, hipMemcpyDeviceToHost, s));
+  EBG_HIP(hipStreamSynchronize(s));
   return EBG_OK;
 }
 

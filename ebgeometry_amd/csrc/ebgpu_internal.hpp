@@ -91,7 +91,8 @@ struct MeshDev
   float           diag;
   uint32_t        num_tris;
   uint32_t*       status; // device word: bit 0 set when a traversal found its stack too small
-  uint64_t        reserved;
+  float           hmax;   // longest triangle edge (rounded up): scale of the FP32 distance error
+  uint32_t        pad;
 };
 static_assert(sizeof(MeshDev) % 16 == 0, "MeshDev is read with 16-byte scalar loads");
 constexpr uint32_t kStatusStackOverflow = 1u;

@@ -264,6 +264,8 @@ ebg_status ensure_cost_grid(ebg_mesh* m, hipStream_t s);
 // recheck) over a caller-chosen workspace of at least n queries. With may_alloc false nothing is
 // allocated or created: split packets only if the cost grid already exists, the bound network
 // on the caller's stream, no traversal timing. The caller holds m->mu.
+// Validate a lattice (origin, spacing, dims; at most 2^32-1 points) into g; n = its point count.
+ebg_status grid_spec(const double* origin, const double* spacing, const uint64_t* dims, GridSpec& g, uint64_t& n);
 ebg_status query_pipeline(ebg_mesh* m, const float* d_pts, uint64_t n, float* d_dist, uint32_t* d_tri,
                           const ebg_query_opts& o, const WsView& ws, bool may_alloc);
 inline void mesh_retain(ebg_mesh* m) { m->refs.fetch_add(1); }

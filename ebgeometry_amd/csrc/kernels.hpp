@@ -160,6 +160,14 @@ hipError_t launch_batch64(const BatchArgs& a, int K, int banks, hipStream_t s);
 hipError_t launch_fill_u32(uint32_t* p, uint64_t n, uint32_t v, hipStream_t s);
 // n doubles -> floats (round to nearest even) and n floats -> doubles
 hipError_t launch_f64_to_f32(const double* in, float* out, uint64_t n, hipStream_t s);
+
+// Lattice of dims[0] x dims[1] x dims[2] points origin + (i, j, k) * spacing, x fastest.
+struct GridSpec
+{
+  double   origin[3], spacing[3];
+  uint64_t dims[3];
+};
+hipError_t launch_grid_points(const GridSpec& g, float* out, hipStream_t s);
 hipError_t launch_f32_to_f64(const float* in, double* out, uint64_t n, hipStream_t s);
 hipError_t launch_add_counts(unsigned long long* work, const uint32_t* recheck, const uint32_t* fallback, hipStream_t s);
 

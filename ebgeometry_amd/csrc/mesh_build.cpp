@@ -123,6 +123,7 @@ void pack_triangles(const float* xyz, const uint32_t* tri, const std::vector<uin
   out.feat.assign(8 * nt, 0u);
   out.fn64.assign(4 * nt, 0.0);
   out.tsph.assign(8 * (nt + kSphPad), 0.0f);
+  double hmax = 0.0;
   for (std::size_t i = 0; i < nt; i++) {
     const uint32_t f  = order[i];
     const D3       x0 = ld3f(xyz + 3 * tri[3 * f]), x1 = ld3f(xyz + 3 * tri[3 * f + 1]),
@@ -166,7 +167,9 @@ void pack_triangles(const float* xyz, const uint32_t* tri, const std::vector<uin
     }
     out.feat[8 * i + 6] = f;
     for (int c = 0; c < 3; c++) out.fn64[4 * i + c] = pn.face[3 * f + c];
+    hmax = std::max(hmax, std::max(length(e[0]), std::max(length(e[1]), length(e[2]))));
   }
+  out.hmax = float(hmax * (1.0 + 1e-6));
   out.vn32.assign(4 * nv, 0.0f);
   out.vn64.assign(4 * nv, 0.0);
   for (std::size_t v = 0; v < nv; v++)

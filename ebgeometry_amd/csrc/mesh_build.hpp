@@ -31,6 +31,7 @@ struct PackedMesh
   int                   stack_need = 0;
   float                 lo[3], hi[3];
   double                diag = 0;
+  float                 hmax = 0; // longest triangle edge, rounded up
   std::vector<uint8_t>  nodes;  // Node<K>[num_nodes]
   std::vector<TriRec>   tris;   // leaf order
   std::vector<float>    tripos; // 12 per triangle, leaf order

@@ -28,6 +28,8 @@ struct ebg_scene
   double                 lo[3]{}, hi[3]{}, diag = 0;
   std::mutex             mu;
   Workspace              ws;
+  std::mutex             grid_mu; // held around the lattice buffer (ebg_scene_query_grid)
+  DevBuf                 gridpts;
   hipEvent_t             ev0 = nullptr, ev1 = nullptr;
   bool                   timed = false;
 
@@ -315,6 +317,50 @@ ebg_status ebg_scene_query(const ebg_scene* csc, const float* d_pts, uint64_t n,
     EBG_HIP(ebgpu::launch_scene64(sc->dev, sc->KM, sc->banks, r, s));
   }
   if (count) EBG_HIP(ebgpu::launch_add_counts(a.work, recheck ? cnt : nullptr, nullptr, s));
+  return EBG_OK;
+}
+
+ebg_status ebg_scene_query_grid(const ebg_scene* csc, const double* origin, const double* spacing,
+                                const uint64_t* dims, float* d_value, const ebg_query_opts* opts)
+{
+  if (!csc) return fail(EBG_INVALID_ARG, "scene is null");
+  ebgpu::GridSpec g{};
+  uint64_t        n  = 0;
+  const ebg_status gs = ebgpu::grid_spec(origin, spacing, dims, g, n);
+  if (gs != EBG_OK) return gs;
+  if (n == 0) return EBG_OK;
+  if (!d_value) return fail(EBG_INVALID_ARG, "null output array");
+  ebg_scene*     sc = const_cast<ebg_scene*>(csc);
+  ebg_query_opts o;
+  ebg_query_opts_default(&o);
+  if (opts) o = *opts;
+  hipStream_t                 s = static_cast<hipStream_t>(o.stream);
+  std::lock_guard<std::mutex> lock(sc->grid_mu);
+  EBG_HIP(sc->gridpts.reserve(12 * n));
+  EBG_HIP(ebgpu::launch_grid_points(g, sc->gridpts.as<float>(), s));
+  return ebg_scene_query(sc, sc->gridpts.as<float>(), n, d_value, &o);
+}
+
+ebg_status ebg_scene_query_grid_host(const ebg_scene* sc, const double* origin, const double* spacing,
+                                     const uint64_t* dims, float* value, const ebg_query_opts* opts)
+{
+  if (!sc) return fail(EBG_INVALID_ARG, "scene is null");
+  ebgpu::GridSpec g{};
+  uint64_t        n  = 0;
+  const ebg_status gs = ebgpu::grid_spec(origin, spacing, dims, g, n);
+  if (gs != EBG_OK) return gs;
+  if (n == 0) return EBG_OK;
+  if (!value) return fail(EBG_INVALID_ARG, "null output array");
+  ebg_query_opts o;
+  ebg_query_opts_default(&o);
+  if (opts) o = *opts;
+  hipStream_t s = static_cast<hipStream_t>(o.stream);
+  DevBuf      d;
+  EBG_HIP(d.reserve(4 * n));
+  const ebg_status st = ebg_scene_query_grid(sc, origin, spacing, dims, d.as<float>(), &o);
+  if (st != EBG_OK) return st;
+  EBG_HIP(hipMemcpyAsync(value, d.p, 4 * n, hipMemcpyDeviceToHost, s));
+  EBG_HIP(hipStreamSynchronize(s));
   return EBG_OK;
 }
 

@@ -47,8 +47,38 @@ enum Mode : int
   kSeeded = 1
 };
 
+// Closest face of a lane whose FP32 walk kept more than one triangle inside the band of its
+// minimum (FaceCands): the kept triangles are evaluated with the FP64 lane formulas and the
+// first strict FP64 minimum (in FP32 order) wins, as TriMeshSDF::getClosestTriangle keeps the
+// first strict minimum (Source/EBGeometry_MeshDistanceFunctionsImplem.hpp:511-549). Returns the
+// FP64 signed distance of that face.
+__device__ __forceinline__ float decide_face64(const MeshDev& m, const FaceCands& fc, float lim, float px, float py,
+                                               float pz, uint32_t& btri)
+{
+  double   bd = INFINITY;
+  uint32_t bt = fc.t[0], bf = 0;
+#pragma unroll
+  for (int i = 0; i < kFaceCands; i++) {
+    if (fc.d[i] <= lim && fc.t[i] != kInvalid) {
+      const uint32_t t = fc.t[i];
+      double         d2;
+      uint32_t       f;
+      tri_eval64(m.tripos + 12 * (size_t)t, m.fn64 + 4 * (size_t)t, px, py, pz, d2, f);
+      if (d2 < bd) {
+        bd = d2;
+        bt = t;
+        bf = f;
+      }
+    }
+  }
+  btri = bt;
+  return (float)resolve64(m, bt, bf, bd, px, py, pz);
+}
+
 // One packet: the 64 queries pk*64 .. pk*64+63 of the (presorted) query order, one per lane.
-template <int K, int NB, bool COUNT, int MODE>
+// FACE (closest faces requested): each lane decides its face exactly (decide_face64); a lane
+// whose band held more triangles than its list goes to the FP64 pass with the near-surface ones.
+template <int K, int NB, bool COUNT, int MODE, bool FACE>
 __device__ __forceinline__ void sdf32_packet(const MeshDev& m, const QueryArgs& a, uint64_t n, uint64_t pk)
 {
   const uint64_t tid    = pk * 64u + (uint64_t)__lane_id();
@@ -71,7 +101,10 @@ __device__ __forceinline__ void sdf32_packet(const MeshDev& m, const QueryArgs& 
   float    best = INFINITY;
   uint32_t btri = kInvalid, bfeat = 0, n_inner = 0, n_tris = 0, w_inner = 0, w_tris = 0;
   const float seed2 = bound;
-  traverse<K, NB, float, COUNT>(m, px, py, pz, bound, best, btri, bfeat, n_inner, n_tris, w_inner, w_tris);
+  FaceCands   fc;
+  if constexpr (FACE) fc.init(m.hmax);
+  traverse<K, NB, float, COUNT, false, FACE>(m, px, py, pz, bound, best, btri, bfeat, n_inner, n_tris, w_inner, w_tris,
+                                             &fc);
 
   // Exact only if some triangle lies within the seeded radius: otherwise nodes between the
   // seeded radius and the true distance were pruned, and the query must be redone unseeded.
@@ -83,14 +116,17 @@ __device__ __forceinline__ void sdf32_packet(const MeshDev& m, const QueryArgs& 
     const bool     redo  = active && !found;
     const uint64_t rmask = ballot(redo);
     if (rmask) {
-      float    best2 = INFINITY;
-      uint32_t btri2 = kInvalid, bfeat2 = 0, f_inner = 0, f_tris = 0;
-      traverse<K, NB, float, COUNT>(m, px, py, pz, redo ? INFINITY : -1.0f, best2, btri2, bfeat2, f_inner, f_tris,
-                                    w_inner, w_tris);
+      float     best2 = INFINITY;
+      uint32_t  btri2 = kInvalid, bfeat2 = 0, f_inner = 0, f_tris = 0;
+      FaceCands fc2;
+      if constexpr (FACE) fc2.init(m.hmax);
+      traverse<K, NB, float, COUNT, false, FACE>(m, px, py, pz, redo ? INFINITY : -1.0f, best2, btri2, bfeat2, f_inner,
+                                                 f_tris, w_inner, w_tris, &fc2);
       if (redo) {
         best  = best2;
         btri  = btri2;
         bfeat = bfeat2;
+        if constexpr (FACE) fc = fc2;
       }
       found = active && btri != kInvalid;
       if (__lane_id() == 0) atomicAdd(a.fallback_count, (uint32_t)__popcll(rmask));
@@ -104,17 +140,27 @@ __device__ __forceinline__ void sdf32_packet(const MeshDev& m, const QueryArgs& 
       }
     }
   }
-  float d = 0.0f;
+  float d   = 0.0f;
+  bool  ovf = false; // FACE: the band held more triangles than the list
   if (found) {
     d = resolve32(m, btri, bfeat, best, px, py, pz);
+    if constexpr (FACE) {
+      // a second triangle inside the band: decide in FP64 among the kept ones, unless the
+      // query is a near-surface one (the FP64 pass decides those from scratch) or the band
+      // overflowed the list (then the FP64 pass decides it too)
+      const float lim = fc.band(best);
+      ovf             = fc.evict <= lim;
+      if (!ovf && fc.d[1] <= lim && !(fabsf(d) < a.recheck_abs)) d = decide_face64(m, fc, lim, px, py, pz, btri);
+    }
     a.out_d[qi] = d;
     if (a.out_tri) a.out_tri[qi] = m.tris[btri].face;
   }
+  const bool rchk = found && (fabsf(d) < a.recheck_abs || ovf);
   if (a.recheck_mask) {
-    const uint64_t rm = ballot(found && fabsf(d) < a.recheck_abs);
+    const uint64_t rm = ballot(rchk);
     if (__lane_id() == 0) a.recheck_mask[pk] = rm;
   }
-  else if (a.recheck_list) wave_append(found && fabsf(d) < a.recheck_abs, qi, a.recheck_list, a.recheck_count);
+  else if (a.recheck_list) wave_append(rchk, qi, a.recheck_list, a.recheck_count);
   if (COUNT && a.packet_cost && __lane_id() == 0) a.packet_cost[pk] = w_inner + w_tris;
   if (COUNT && a.work) {
     const uint64_t si = wave_sum_u64(active ? (uint64_t)n_inner : 0ull);
@@ -169,8 +215,8 @@ __device__ __forceinline__ void split_item(const MeshDev& m, const QueryArgs& a,
 // mapped XCD-aware so that each XCD works through one contiguous (Morton) range of queries.
 // The leading split_blocks workgroups (dispatched first) work through the split items instead,
 // beside the packets of the bulk; packets flagged in heavy_flag are skipped.
-template <int K, int NB, bool COUNT, int MODE>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(K == 4 ? 8 : 1))) void sdf32_kernel(const MeshDev m, const QueryArgs a)
+template <int K, int NB, bool COUNT, int MODE, bool FACE>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(K == 4 && !FACE ? 8 : 1))) void sdf32_kernel(const MeshDev m, const QueryArgs a)
 {
   const uint32_t wave = uni(threadIdx.x >> 6); // known uniform: list entries load as scalars
   uint32_t       b = blockIdx.x, nb = gridDim.x;
@@ -189,7 +235,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(K == 4 ? 8 
   if (a.early_blocks) {
     if (b < a.early_blocks) {
       const uint32_t j = b * 4u + wave;
-      if (j < min(*a.early_count, a.early_cap)) sdf32_packet<K, NB, COUNT, MODE>(m, a, a.n, a.early_list[j]);
+      if (j < min(*a.early_count, a.early_cap)) sdf32_packet<K, NB, COUNT, MODE, FACE>(m, a, a.n, a.early_list[j]);
       return;
     }
     b -= a.early_blocks;
@@ -200,7 +246,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(K == 4 ? 8 
   const uint64_t pk  = (uint64_t)blk * 4u + wave;
   if (pk * 64u >= n) return;                    // whole wavefront past the end
   if (a.heavy_flag && a.heavy_flag[pk]) return; // whole wavefront: a split or early packet
-  sdf32_packet<K, NB, COUNT, MODE>(m, a, n, pk);
+  sdf32_packet<K, NB, COUNT, MODE, FACE>(m, a, n, pk);
 }
 
 // One wavefront per listed packet: the winning triangle of each query, as the plain pass ends.
@@ -310,11 +356,19 @@ __global__ __launch_bounds__(256) void sdf64_kernel(const MeshDev m, const Query
     sdf64_block<K, NB>(m, a, n, lane < kSdf64Lanes ? w * kSdf64Lanes + lane : n);
 }
 
+template <int K, int NB, int MODE, bool FACE>
+void launch32f(const MeshDev& m, const QueryArgs& a, bool count, uint32_t blocks, hipStream_t s)
+{
+  if (count) hipLaunchKernelGGL((sdf32_kernel<K, NB, true, MODE, FACE>), dim3(blocks), dim3(256), 0, s, m, a);
+  else hipLaunchKernelGGL((sdf32_kernel<K, NB, false, MODE, FACE>), dim3(blocks), dim3(256), 0, s, m, a);
+}
+
+// closest faces requested: the exact-face variant (its split packets are off: see launch32)
 template <int K, int NB, int MODE>
 void launch32m(const MeshDev& m, const QueryArgs& a, bool count, uint32_t blocks, hipStream_t s)
 {
-  if (count) hipLaunchKernelGGL((sdf32_kernel<K, NB, true, MODE>), dim3(blocks), dim3(256), 0, s, m, a);
-  else hipLaunchKernelGGL((sdf32_kernel<K, NB, false, MODE>), dim3(blocks), dim3(256), 0, s, m, a);
+  if (a.out_tri) launch32f<K, NB, MODE, true>(m, a, count, blocks, s);
+  else launch32f<K, NB, MODE, false>(m, a, count, blocks, s);
 }
 
 template <int K, int NB>
@@ -324,6 +378,9 @@ hipError_t launch32(const MeshDev& m, const QueryArgs& a, bool count, hipStream_
   if (blocks == 0) return hipSuccess;
   if (a.count_ptr) return hipErrorInvalidValue;
   if (a.split_blocks && (!a.split_nroots || !a.split_count || !a.heavy_flag)) return hipErrorInvalidValue;
+  // exact closest faces need each lane's whole candidate band in one walk: no split packets, and
+  // the FP64 pass must follow (lanes whose band overflowed are listed for it)
+  if (a.out_tri && (a.split_blocks || (!a.recheck_mask && !a.recheck_list))) return hipErrorInvalidValue;
   if (a.early_blocks && (!a.early_list || !a.early_count || !a.heavy_flag)) return hipErrorInvalidValue;
   if (a.seed_r) {
     if (!a.fallback_count) return hipErrorInvalidValue;

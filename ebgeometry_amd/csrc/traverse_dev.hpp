@@ -386,6 +386,62 @@ __device__ __forceinline__ void wave_append(bool pred, uint32_t idx, uint32_t* l
 }
 
 // ---------------------------------------------------------------------------------------
+// closest-face candidates (exact closest face from an FP32 walk)
+// ---------------------------------------------------------------------------------------
+
+// The FP32 distance of a point to a triangle differs from the FP64 lane formula's by at most
+// E(d) = 2^-21 (d + 2 hmax) (d the distance, hmax the mesh's longest edge: every rounding acts on
+// a vector no longer than d + 2 hmax; measured maximum 3 * 2^-24 (d + 2 hmax)). So the face that
+// is closest in FP64 has an FP32 distance within 2 E of the FP32 minimum, and a lane that keeps
+// every triangle inside that band can decide its face exactly, in FP64, among them.
+constexpr int kFaceCands = 8;
+
+struct FaceCands
+{
+  float    d[kFaceCands]; // FP32 d^2 of the kept triangles, ascending (+inf: empty slot)
+  uint32_t t[kFaceCands];
+  float    lim;           // band of the current minimum: triangles with d^2 <= lim are kept
+  float    evict;         // smallest d^2 pushed out of a full list
+  float    habs;          // 2^-19 hmax: the band's absolute part
+
+  __device__ __forceinline__ void init(float hmax)
+  {
+#pragma unroll
+    for (int i = 0; i < kFaceCands; i++) {
+      d[i] = INFINITY;
+      t[i] = kInvalid;
+    }
+    lim   = INFINITY;
+    evict = INFINITY;
+    habs  = hmax * 0x1p-19f;
+  }
+
+  // band(d^2) = (d (1 + 2^-20) + 2^-19 hmax)^2, rounded up
+  __device__ __forceinline__ float band(float d2) const
+  {
+    const float r = fmaf(__builtin_amdgcn_sqrtf(d2), 1.0f + 0x1p-20f, habs);
+    return r * r * (1.0f + 0x1p-21f);
+  }
+
+  // Sorted insert (fixed positions only: the list stays in VGPRs); a value pushed out of the
+  // full list is remembered in `evict`.
+  __device__ __forceinline__ void insert(float x, uint32_t xt)
+  {
+#pragma unroll
+    for (int i = 0; i < kFaceCands; i++) {
+      const bool     sw = x < d[i];
+      const float    od = d[i];
+      const uint32_t ot = t[i];
+      d[i]              = sw ? x : od;
+      t[i]              = sw ? xt : ot;
+      x                 = sw ? od : x;
+      xt                = sw ? ot : xt;
+    }
+    evict = fminf(evict, x);
+  }
+};
+
+// ---------------------------------------------------------------------------------------
 // the packet traversal
 // ---------------------------------------------------------------------------------------
 
@@ -445,11 +501,15 @@ __device__ __forceinline__ int rep_lane(R bound)
 
 // R = float: FP32 pass on TriRec records. R = double: FP64 recheck on raw positions.
 // The walk starts at `start` (the root, or a subtree root of a split packet).
-template <int K, int NB, class R, bool COUNT, bool WMIN = (sizeof(R) == 8)>
+// FACE (FP32 only): the cull bound is the band of the minimum (FaceCands::band) rather than the
+// minimum itself, the plane culls are off (their FP32 error is one of the coordinates, not of
+// the distance), and every triangle inside the band is kept in fc.
+template <int K, int NB, class R, bool COUNT, bool WMIN = (sizeof(R) == 8), bool FACE = false>
 __device__ __forceinline__ void traverse_from(const MeshDev& m, const Entry& start, R px, R py, R pz, R bound, R& best,
                                               uint32_t& btri, uint32_t& bfeat, uint32_t& n_inner, uint32_t& n_tris,
-                                              uint32_t& w_inner, uint32_t& w_tris)
+                                              uint32_t& w_inner, uint32_t& w_tris, FaceCands* fc = nullptr)
 {
+  static_assert(!FACE || sizeof(R) == 4, "closest-face candidates: FP32 walk only");
   const Node<K>* __restrict__ nodes = static_cast<const Node<K>*>(m.nodes);
   LaneStack<NB>               st;
   st.init();
@@ -498,7 +558,7 @@ __device__ __forceinline__ void traverse_from(const MeshDev& m, const Entry& sta
             const R    lim = rb + R(__uint_as_float(sp.w[8 * i + 3]));
             const bool in_s = sx * sx + sy * sy + sz * sz <= lim * lim;
             bool       in_p = true;
-            if constexpr (sizeof(R) == 4) {
+            if constexpr (sizeof(R) == 4 && !FACE) {
               // the FP32 plane (rounded normal and offset) may misjudge by a few ulps of the
               // coordinates; the FP64 pass, whose seed only just contains the minimum, skips it
               const float pd = fmaf(__uint_as_float(sp.w[8 * i + 4]), px,
@@ -524,6 +584,18 @@ __device__ __forceinline__ void traverse_from(const MeshDev& m, const Entry& sta
             if (ballot(fd * fd <= bound) == 0) continue;
             if (COUNT) w_tris++;
             tri_eval32<false>(rec, px, py, pz, d2, f);
+            if constexpr (FACE) {
+              if (d2 <= fc->lim) {
+                fc->insert(d2, t);
+                if (d2 < best) {
+                  best    = d2;
+                  btri    = t;
+                  fc->lim = fc->band(d2);
+                  bound   = fc->lim < bound ? fc->lim : bound;
+                }
+              }
+              continue;
+            }
           }
           else {
             if (COUNT) w_tris++;
@@ -554,13 +626,14 @@ __device__ __forceinline__ void traverse_from(const MeshDev& m, const Entry& sta
   }
 }
 
-template <int K, int NB, class R, bool COUNT, bool WMIN = (sizeof(R) == 8)>
+template <int K, int NB, class R, bool COUNT, bool WMIN = (sizeof(R) == 8), bool FACE = false>
 __device__ __forceinline__ void traverse(const MeshDev& m, R px, R py, R pz, R bound, R& best, uint32_t& btri,
-                                         uint32_t& bfeat, uint32_t& n_inner, uint32_t& n_tris, uint32_t& w_inner, uint32_t& w_tris)
+                                         uint32_t& bfeat, uint32_t& n_inner, uint32_t& n_tris, uint32_t& w_inner, uint32_t& w_tris,
+                                         FaceCands* fc = nullptr)
 {
   const Entry root{0u, 0u, m.root_lo[0], m.root_lo[1], m.root_lo[2], m.root_hi[0], m.root_hi[1], m.root_hi[2]};
-  traverse_from<K, NB, R, COUNT, WMIN>(m, root, px, py, pz, bound, best, btri, bfeat, n_inner, n_tris, w_inner,
-                                       w_tris);
+  traverse_from<K, NB, R, COUNT, WMIN, FACE>(m, root, px, py, pz, bound, best, btri, bfeat, n_inner, n_tris, w_inner,
+                                             w_tris, fc);
 }
 
 } // namespace dev

@@ -36,7 +36,30 @@ __global__ void f32_to_f64_kernel(const float* __restrict__ in, double* __restri
   if (i < n) out[i] = (double)in[i];
 }
 
+// Lattice points of a box of cells or nodes (x fastest, then y, then z: the layout of an AMReX
+// or Chombo FAB): point (i, j, k) = origin + (i, j, k) * spacing, each coordinate a separately
+// rounded FP64 product and sum, then rounded to FP32 (round to nearest even).
+__global__ void grid_points_kernel(GridSpec g, float* __restrict__ out)
+{
+  const uint64_t n = g.dims[0] * g.dims[1] * g.dims[2];
+  const uint64_t p = (uint64_t)blockIdx.x * 256 + threadIdx.x;
+  if (p >= n) return;
+  const uint64_t i = p % g.dims[0], r = p / g.dims[0];
+  const uint64_t j = r % g.dims[1], k = r / g.dims[1];
+  out[3 * p]       = __double2float_rn(__dadd_rn(g.origin[0], __dmul_rn((double)i, g.spacing[0])));
+  out[3 * p + 1]   = __double2float_rn(__dadd_rn(g.origin[1], __dmul_rn((double)j, g.spacing[1])));
+  out[3 * p + 2]   = __double2float_rn(__dadd_rn(g.origin[2], __dmul_rn((double)k, g.spacing[2])));
+}
+
 } // namespace
+
+hipError_t launch_grid_points(const GridSpec& g, float* out, hipStream_t s)
+{
+  const uint64_t n = g.dims[0] * g.dims[1] * g.dims[2];
+  if (n == 0) return hipSuccess;
+  hipLaunchKernelGGL(grid_points_kernel, dim3((uint32_t)((n + 255) / 256)), dim3(256), 0, s, g, out);
+  return hipGetLastError();
+}
 
 hipError_t launch_f64_to_f32(const double* in, float* out, uint64_t n, hipStream_t s)
 {

@@ -34,6 +34,23 @@ def _torch():
     return torch
 
 
+def _grid_args(origin, spacing, dims):
+    """ctypes arrays (double[3], double[3], uint64[3]) of a lattice."""
+    o = (C.c_double * 3)(*[float(x) for x in origin])
+    s = (C.c_double * 3)(*[float(x) for x in spacing])
+    n = (C.c_uint64 * 3)(*[int(x) for x in dims])
+    return o, s, n
+
+
+def grid_points(origin, spacing, dims) -> np.ndarray:
+    """The lattice points of query_grid as an (n, 3) float32 array, x fastest: each coordinate
+    origin + index * spacing in FP64 (product and sum each rounded), then rounded to FP32."""
+    nx, ny, nz = (int(x) for x in dims)
+    k, j, i = np.meshgrid(np.arange(nz), np.arange(ny), np.arange(nx), indexing="ij")
+    idx = np.stack([i.ravel(), j.ravel(), k.ravel()], 1).astype(np.float64)
+    return (np.asarray(origin, np.float64) + idx * np.asarray(spacing, np.float64)).astype(np.float32)
+
+
 def _is_tensor(x) -> bool:
     try:
         torch = _torch()
@@ -154,6 +171,25 @@ class TriMeshSDF(SignedDistanceFunction):
         check(lib().ebg_query_sdf(self._h, p.data_ptr(), n, d.data_ptr(), t.data_ptr() if t is not None else None,
                                   C.byref(o)))
         return (d, t) if with_triangle else d
+
+    def query_grid(self, origin, spacing, dims, device=None, out=None, **kw):
+        """Signed distance at every lattice point origin + (i, j, k) * spacing, as an array of
+        shape (dims[2], dims[1], dims[0]) (x fastest: a FAB's layout). With device (a torch
+        device) the result is a CUDA tensor made on torch's current stream; else numpy."""
+        o3, s3, n3 = _grid_args(origin, spacing, dims)
+        shape = (int(n3[2]), int(n3[1]), int(n3[0]))
+        if device is not None:
+            torch = _torch()
+            d = out if out is not None else torch.empty(shape, dtype=torch.float32, device=device)
+            if "stream" not in kw:
+                kw["stream"] = torch.cuda.current_stream(d.device).cuda_stream
+            o = self.query_opts(**kw)
+            check(lib().ebg_query_grid(self._h, o3, s3, n3, d.data_ptr(), C.byref(o)))
+            return d
+        d = np.empty(shape, dtype=np.float32)
+        o = self.query_opts(**kw)
+        check(lib().ebg_query_grid_host(self._h, o3, s3, n3, d.ctypes.data, C.byref(o)))
+        return d
 
     def signed_distance(self, points, **kw):
         single = not _is_tensor(points) and np.asarray(points).shape == (3,)

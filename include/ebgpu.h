@@ -172,6 +172,22 @@ ebg_status ebg_query_sdf_f64(const ebg_mesh* mesh, const double* d_pts, uint64_t
 ebg_status ebg_query_sdf_host(const ebg_mesh* mesh, const float* pts, uint64_t n, float* dist,
                               uint32_t* tri, const ebg_query_opts* opts);
 
+/* Lattice queries for embedded-boundary generators: the signed distance at every point
+ * origin + (i, j, k) * spacing, 0 <= i < dims[0], 0 <= j < dims[1], 0 <= k < dims[2], written
+ * x fastest (d_dist[(k * dims[1] + j) * dims[0] + i]: the layout of an AMReX or Chombo FAB over
+ * the box). Replaces the per-cell / per-node value() calls an EB generator makes through the
+ * reference (AMReXSDF::operator(), Integrations/AMReX/MeshSDF/main.cpp:23-69, called by
+ * EB2::GeometryShop for every node; ChomboSDF::value, Integrations/Chombo/MeshSDF/main.cpp:21-50):
+ * one call per box. The points are made on the device (each coordinate the FP64 product
+ * i * spacing plus origin, each rounded, then rounded to FP32), then queried as by
+ * ebg_query_sdf (presorted, FP64 recheck near the surface). At most 2^32-1 points per call;
+ * uses a buffer of 12 bytes per point held by the mesh. Device output, asynchronous on
+ * opts->stream; the _host form copies to a host array and synchronises. */
+ebg_status ebg_query_grid(const ebg_mesh* mesh, const double origin[3], const double spacing[3],
+                          const uint64_t dims[3], float* d_dist, const ebg_query_opts* opts);
+ebg_status ebg_query_grid_host(const ebg_mesh* mesh, const double origin[3], const double spacing[3],
+                               const uint64_t dims[3], float* dist, const ebg_query_opts* opts);
+
 /* Time the dominant traversal kernel of the last query on `mesh` (HIP events recorded on the
  * launch stream around the FP32 traversal -- the launch, plain or seeded, with its split
  * packets, and their resolve; the FP64 pass is not included). Milliseconds, or <0 if none.
@@ -239,6 +255,12 @@ ebg_status ebg_scene_query(const ebg_scene* scene, const float* d_pts, uint64_t 
                            const ebg_query_opts* opts);
 ebg_status ebg_scene_query_host(const ebg_scene* scene, const float* pts, uint64_t n, float* value,
                                 const ebg_query_opts* opts);
+/* Lattice form of ebg_scene_query (same lattice and layout as ebg_query_grid). */
+ebg_status ebg_scene_query_grid(const ebg_scene* scene, const double origin[3], const double spacing[3],
+                                const uint64_t dims[3], float* d_value, const ebg_query_opts* opts);
+ebg_status ebg_scene_query_grid_host(const ebg_scene* scene, const double origin[3],
+                                     const double spacing[3], const uint64_t dims[3], float* value,
+                                     const ebg_query_opts* opts);
 double     ebg_scene_last_traversal_ms(const ebg_scene* scene);
 /* As ebg_mesh_check, for the scene's own tree walk (its meshes report through ebg_mesh_check). */
 ebg_status ebg_scene_check(const ebg_scene* scene);
