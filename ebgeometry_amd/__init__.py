@@ -10,10 +10,23 @@ mirror of the reference API over that ABI.
 from .sdf import (Build, ImplicitFunction, SignedDistanceFunction, TriMeshSDF, grid_points, morton_codes,
                   morton_sort, pseudo_normals)
 from .batch import BatchPlan, MeshBatch
-from .csg import (AABB, BoxSDF, BVHUnion, BVHUnionIF, Rotate, RotateIF, Scale, ScaleIF, SphereSDF, Translate,
-                  TranslateIF)
+from .cloud import Hit, PointCloudBVH, PointCloudHashGrid
+from .csg import (AABB, Annular, AnnularIF, Blur, BlurIF, BoxSDF, BVHSmoothUnion, BVHSmoothUnionIF, BVHUnion,
+                  BVHUnionIF, CapsuleSDF, Complement, ComplementIF, ConeSDF, CylinderSDF, Difference, DifferenceIF,
+                  Elongate, ElongateIF, ExpMin, FiniteRepetition, FiniteRepetitionIF, InfiniteConeSDF,
+                  InfiniteCylinderSDF, Intersection, IntersectionIF, Mollify, MollifyIF, Offset, OffsetIF, PlaneSDF,
+                  Reflect, ReflectIF, Rotate, RotateIF, RoundedBoxSDF, RoundedCylinderSDF, Scale, ScaleIF,
+                  SmoothDifference, SmoothDifferenceIF, SmoothIntersection, SmoothIntersectionIF, SmoothMin,
+                  SmoothUnion, SmoothUnionIF, SphereSDF, TorusSDF, Translate, TranslateIF, Union, UnionIF)
 
 __all__ = ["Build", "ImplicitFunction", "SignedDistanceFunction", "TriMeshSDF", "grid_points", "morton_codes",
            "morton_sort",
            "pseudo_normals", "AABB", "BoxSDF", "BVHUnion", "BVHUnionIF", "Rotate", "RotateIF", "Scale", "ScaleIF",
-           "SphereSDF", "Translate", "TranslateIF", "MeshBatch", "BatchPlan"]
+           "SphereSDF", "Translate", "TranslateIF", "MeshBatch", "BatchPlan", "Annular", "AnnularIF", "Blur",
+           "BlurIF", "BVHSmoothUnion", "BVHSmoothUnionIF", "CapsuleSDF", "Complement", "ComplementIF", "ConeSDF",
+           "CylinderSDF", "Difference", "DifferenceIF", "Elongate", "ElongateIF", "ExpMin", "FiniteRepetition",
+           "FiniteRepetitionIF", "InfiniteConeSDF", "InfiniteCylinderSDF", "Intersection", "IntersectionIF",
+           "Mollify", "MollifyIF", "Offset", "OffsetIF", "PlaneSDF", "Reflect", "ReflectIF", "RoundedBoxSDF",
+           "RoundedCylinderSDF", "SmoothDifference", "SmoothDifferenceIF", "SmoothIntersection",
+           "SmoothIntersectionIF", "SmoothMin", "SmoothUnion", "SmoothUnionIF", "TorusSDF", "Union", "UnionIF",
+           "Hit", "PointCloudBVH", "PointCloudHashGrid"]

@@ -45,6 +45,14 @@ class Prim(C.Structure):
                 ("bv_hi", C.c_double * 3)]
 
 
+class CsgOp(C.Structure):
+    _fields_ = [("op", _i32), ("n", _i32), ("a", _i32), ("b", _i32), ("p", C.c_double * 12)]
+
+
+class CsgPrim(C.Structure):
+    _fields_ = [("start", C.c_uint32), ("count", C.c_uint32), ("bv_lo", C.c_double * 3), ("bv_hi", C.c_double * 3)]
+
+
 class BatchOpts(C.Structure):
     _fields_ = [("autotune", _i32), ("branching", _i32), ("leaf_size", _i32), ("threads", _i32)]
 
@@ -90,6 +98,21 @@ SIGNATURES = [
     ("ebg_scene_query_grid", C.c_int, [_vp, _vp, _vp, _vp, _vp, C.POINTER(QueryOpts)]),
     ("ebg_scene_query_grid_host", C.c_int, [_vp, _vp, _vp, _vp, _vp, C.POINTER(QueryOpts)]),
     ("ebg_scene_last_traversal_ms", C.c_double, [_vp]),
+    ("ebg_csg_create", C.c_int, [C.POINTER(CsgOp), _u64, _u64, C.POINTER(CsgPrim), _u64, _vp, _u64, C.c_double,
+                                 C.POINTER(_vp)]),
+    ("ebg_csg_destroy", C.c_int, [_vp]),
+    ("ebg_csg_query", C.c_int, [_vp, _vp, _u64, _vp, C.POINTER(QueryOpts)]),
+    ("ebg_csg_query_host", C.c_int, [_vp, _vp, _u64, _vp, C.POINTER(QueryOpts)]),
+    ("ebg_csg_query_grid", C.c_int, [_vp, _vp, _vp, _vp, _vp, C.POINTER(QueryOpts)]),
+    ("ebg_csg_query_grid_host", C.c_int, [_vp, _vp, _vp, _vp, _vp, C.POINTER(QueryOpts)]),
+    ("ebg_csg_check", C.c_int, [_vp]),
+    ("ebg_cloud_create", C.c_int, [_vp, _u64, C.POINTER(BuildOpts), C.POINTER(_vp)]),
+    ("ebg_cloud_destroy", C.c_int, [_vp]),
+    ("ebg_cloud_knn", C.c_int, [_vp, _vp, _u64, C.c_uint32, _vp, _vp, C.POINTER(QueryOpts)]),
+    ("ebg_cloud_knn_host", C.c_int, [_vp, _vp, _u64, C.c_uint32, _vp, _vp, C.POINTER(QueryOpts)]),
+    ("ebg_cloud_knn_self", C.c_int, [_vp, C.c_uint32, _vp, _vp, C.POINTER(QueryOpts)]),
+    ("ebg_cloud_knn_self_host", C.c_int, [_vp, C.c_uint32, _vp, _vp, C.POINTER(QueryOpts)]),
+    ("ebg_cloud_check", C.c_int, [_vp]),
     ("ebg_batch_create", C.c_int, [_vp, _vp, _vp, _vp, _u64, C.POINTER(BatchOpts), C.POINTER(_vp)]),
     ("ebg_batch_destroy", C.c_int, [_vp]),
     ("ebg_batch_get_info", C.c_int, [_vp, C.POINTER(BatchInfo)]),

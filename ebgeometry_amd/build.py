@@ -34,9 +34,13 @@ SOURCES = [
     ("mesh_build.cpp", HOST),
     ("api.cpp", HOST),
     ("scene_api.cpp", HOST),
+    ("csg_api.cpp", HOST),
+    ("cloud_api.cpp", HOST),
     ("batch_api.cpp", HOST),
     ("traverse.hip", DEVICE),
     ("scene.hip", DEVICE),
+    ("csg.hip", DEVICE),
+    ("cloud.hip", DEVICE),
     ("batch.hip", DEVICE),
     ("morton.hip", DEVICE),
     ("morton_build.hip", DEVICE),

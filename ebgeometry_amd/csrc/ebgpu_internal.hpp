@@ -140,4 +140,114 @@ struct SceneDev
   uint32_t*        status; // device word: kStatusStackOverflow when the scene walk's stack was too small
 };
 
+// ---------------------------------------------------------------------------------------------
+// CSG programs: an implicit-function expression evaluated per query by a small interpreter
+// ---------------------------------------------------------------------------------------------
+// Post-order instructions over a value stack and a point stack (the current point is the top).
+// Parameters p[] in the FP32 record (CsgInstr) and the FP64 record (CsgInstr64) are the same
+// numbers, rounded once.
+enum CsgOpcode : uint32_t
+{
+  // leaves: push f(current point)
+  kCsgSphere = 0,      // p0-2 centre, p3 radius
+  kCsgBox,             // p0-2 lo, p3-5 hi
+  kCsgPlane,           // p0-2 point, p3-5 unit normal
+  kCsgTorus,           // p0-2 centre, p3 major radius, p4 minor radius
+  kCsgCylinder,        // p0-2 centre, p3-5 unit axis, p6 length, p7 radius
+  kCsgInfCylinder,     // p0-2 centre, p3 radius; n = axis
+  kCsgCapsule,         // p0-2 first centre, p3-5 second centre, p6 radius
+  kCsgInfCone,         // p0-2 tip, p3 sin(half angle), p4 cos(half angle)
+  kCsgCone,            // p0-2 tip, p3 sin(half angle), p4 cos(half angle), p5 height
+  kCsgRoundedBox,      // p0-2 half extents, p3 rounding radius
+  kCsgRoundedCylinder, // p0 major radius, p1 minor radius, p2 half height
+  kCsgMesh,            // n = mesh index: the mesh's signed distance
+  // point transforms: save the current point, replace it by T(point)
+  kCsgTranslate,       // q = x - p0-2
+  kCsgRotate,          // n = axis, p0 cos, p1 sin (the point map of RotateIF::value)
+  kCsgScalePoint,      // q = x / p0
+  kCsgElongate,        // q = x - clamp(x, -e, e), e = p0-2
+  kCsgReflect,         // q = x * p0-2 (each +1 or -1)
+  kCsgRepeat,          // q_i = x_i - P_i round(clamp(x_i / P_i, -lo_i, hi_i)), P = p0-2, lo = p3-5, hi = p6-8
+  kCsgPopPoint,        // restore the saved point
+  // value ops on the top value
+  kCsgScaleValue,      // v * p0
+  kCsgNegate,          // -v
+  kCsgOffset,          // v - p0
+  kCsgAnnular,         // |v| - p0
+  // accumulators: pop v and fold it into the value(s) below
+  kCsgMinAcc,          // u = min(u, v)
+  kCsgMaxAcc,          // u = max(u, v)
+  kCsgAddAcc,          // u = u + v
+  kCsgTwoMinAcc,       // (a, b) = the two smallest of a, b, v (strict <, in order)
+  kCsgTwoMaxAcc,       // (a, b) = the two largest
+  kCsgSmoothMinEnd,    // (a, b) -> smooth min of a and b with length p0; n = 0 quadratic, 1 exponential
+  kCsgSmoothMaxEnd,    // (a, b) -> quadratic smooth max with length p0
+  kCsgDifference,      // (a, b) -> max(a, -b)
+  kCsgPushConst,       // push p0 (the +-inf starting values of the accumulators)
+  // unions accelerated by a BVH over primitive subprograms (a = BVH index): push the value
+  kCsgBvhUnion,
+  kCsgBvhSmoothUnion,  // p0 smoothing length; n = 0 quadratic, 1 exponential
+  kCsgNumOps
+};
+
+struct alignas(64) CsgInstr
+{
+  uint32_t op, n, a, b;
+  float    p[12];
+};
+static_assert(sizeof(CsgInstr) == 64, "CsgInstr layout");
+
+struct alignas(64) CsgInstr64
+{
+  double   p[12];
+  uint32_t op, n, a, b;
+  uint32_t pad[4];
+};
+static_assert(sizeof(CsgInstr64) == 128, "CsgInstr64 layout");
+
+// One primitive of a BVH union: the subprogram ops[start, start + count) (no BVH op inside),
+// leaving one value, and its bounding volume.
+struct CsgPrim
+{
+  uint32_t start, count;
+  float    bv[6];
+};
+static_assert(sizeof(CsgPrim) == 32, "CsgPrim layout");
+
+struct CsgBvhDev
+{
+  const void*    nodes; // Node<4>, leaves index prims
+  const CsgPrim* prims;
+  float          root_lo[3], root_hi[3];
+  uint32_t       pad[2];
+};
+static_assert(sizeof(CsgBvhDev) == 48, "CsgBvhDev layout");
+
+// ---------------------------------------------------------------------------------------------
+// Point clouds: K-wide tree over the points, leaves of at most 16 points
+// ---------------------------------------------------------------------------------------------
+struct CloudDev
+{
+  const void*     nodes;  // Node<K>; leaf ref = first point (leaf order), cnt = point count
+  const uint32_t* pts;    // 4 words per point in leaf order: x, y, z (float bits), cloud index;
+                          // padded by 16 points (chunked scalar loads)
+  float           root_lo[3], root_hi[3];
+  uint32_t*       status; // kStatusStackOverflow when a walk's stack was too small
+};
+constexpr uint32_t kKnnMax = 16; // largest k of one query
+
+constexpr int kCsgMaxValues = 16; // value stack depth
+constexpr int kCsgMaxPoints = 8;  // saved points
+
+struct CsgDev
+{
+  const CsgInstr*   ops;
+  const CsgInstr64* ops64;
+  const CsgBvhDev*  bvhs;
+  const MeshDev*    meshes;
+  uint32_t          main_count; // the program is ops[0, main_count)
+  uint32_t          pad;
+  uint32_t*         status;     // kStatusStackOverflow: a BVH walk's stack was too small
+};
+
 } // namespace ebgpu

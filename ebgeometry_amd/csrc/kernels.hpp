@@ -156,6 +156,27 @@ hipError_t launch_batch32(const BatchArgs& a, int K, int banks, int wg, bool cou
 // strides over groups of 64 entries.
 hipError_t launch_batch64(const BatchArgs& a, int K, int banks, hipStream_t s);
 
+// csg.hip: CSG programs. FP32 pass over a.n queries (a.idx: presort order), appending
+// |v| < a.recheck_abs to the recheck list; FP64 pass over that list (a.idx, *a.count_ptr).
+hipError_t launch_csg32(const CsgDev& g, int KM, int banks, const QueryArgs& a, hipStream_t s);
+hipError_t launch_csg64(const CsgDev& g, int KM, int banks, const QueryArgs& a, hipStream_t s);
+
+// cloud.hip: k nearest cloud points (k <= kKnnMax) of n query points (pts, in the order idx) or,
+// with self set, of the cloud's own n points in leaf order excluding each point itself. Row r of
+// out_idx / out_d2 (k entries, nearest first, missing entries kInvalid / +inf) belongs to query r
+// (self: to cloud index r).
+struct KnnArgs
+{
+  const float*    pts;
+  const uint32_t* idx;
+  uint64_t        n;
+  uint32_t        k;
+  uint32_t        self;
+  uint32_t*       out_idx;
+  float*          out_d2;
+};
+hipError_t launch_knn(const CloudDev& c, int K, int banks, const KnnArgs& a, hipStream_t s);
+
 // util.hip
 hipError_t launch_fill_u32(uint32_t* p, uint64_t n, uint32_t v, hipStream_t s);
 // n doubles -> floats (round to nearest even) and n floats -> doubles

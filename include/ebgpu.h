@@ -266,6 +266,139 @@ double     ebg_scene_last_traversal_ms(const ebg_scene* scene);
 ebg_status ebg_scene_check(const ebg_scene* scene);
 
 /* ---------------------------------------------------------------------------------------- */
+/* CSG programs -- the reference's implicit-function algebra on the GPU:                      */
+/*   analytic SDFs  Plane, Sphere, Box, Torus, Cylinder, InfiniteCylinder, Capsule,            */
+/*                  InfiniteCone, Cone, RoundedBox, RoundedCylinder                            */
+/*                  (Source/EBGeometry_AnalyticDistanceFunctions.hpp:41-1750), and meshes;     */
+/*   transforms     Translate, Rotate, Scale, Elongate, Reflect, Complement, Offset, Annular,   */
+/*                  Blur, Mollify (Source/EBGeometry_Transform.hpp:78-934);                    */
+/*   CSG            Union, SmoothUnion, Intersection, SmoothIntersection, Difference,          */
+/*                  SmoothDifference, FiniteRepetition, BVHUnion, BVHSmoothUnion               */
+/*                  (Source/EBGeometry_CSG.hpp:40-866, CSGImplem.hpp).                         */
+/* An expression is given as the post-order list of its instructions (a stack program): each  */
+/* leaf pushes its value at the current point; a point transform replaces the current point   */
+/* until the matching EBG_CSG_POP_POINT; value transforms change the top value; folds pop the  */
+/* top value into the one(s) below. The Python package (ebgeometry_amd.csg) compiles the      */
+/* reference's object names into this list.                                                   */
+/* ---------------------------------------------------------------------------------------- */
+
+typedef struct ebg_csg ebg_csg;
+
+enum ebg_csg_opcode
+{
+  /* leaves: push f(current point); p[] as listed */
+  EBG_CSG_SPHERE = 0,      /* p0-2 centre, p3 radius                                              */
+  EBG_CSG_BOX,             /* p0-2 low corner, p3-5 high corner                                   */
+  EBG_CSG_PLANE,           /* p0-2 point, p3-5 unit normal                                        */
+  EBG_CSG_TORUS,           /* p0-2 centre, p3 major radius, p4 minor radius                       */
+  EBG_CSG_CYLINDER,        /* p0-2 centre, p3-5 unit axis, p6 length, p7 radius                   */
+  EBG_CSG_INF_CYLINDER,    /* p0-2 centre, p3 radius; n = axis (0, 1, 2)                          */
+  EBG_CSG_CAPSULE,         /* p0-2, p3-5 the two sphere centres, p6 radius                        */
+  EBG_CSG_INF_CONE,        /* p0-2 tip, p3 sin and p4 cos of the half opening angle               */
+  EBG_CSG_CONE,            /* p0-2 tip, p3 sin, p4 cos of the half angle, p5 height               */
+  EBG_CSG_ROUNDED_BOX,     /* p0-2 half extents, p3 rounding radius                               */
+  EBG_CSG_ROUNDED_CYLINDER,/* p0 radius - curvature, p1 curvature, p2 height / 2 - curvature      */
+  EBG_CSG_MESH,            /* n = index into the program's meshes: their signed distance          */
+  /* point transforms: the current point x becomes q until EBG_CSG_POP_POINT */
+  EBG_CSG_TRANSLATE,       /* q = x - p0-2                                                        */
+  EBG_CSG_ROTATE,          /* n = axis, p0 = cos, p1 = sin of the angle (RotateIF's point map)    */
+  EBG_CSG_SCALE_POINT,     /* q = x / p0 (with EBG_CSG_SCALE_VALUE p0 after: ScaleIF)             */
+  EBG_CSG_ELONGATE,        /* q = x - clamp(x, -p0-2, p0-2)                                       */
+  EBG_CSG_REFLECT,         /* q = x * p0-2 (each +1 or -1)                                        */
+  EBG_CSG_REPEAT,          /* q_i = x_i - P_i round(clamp(x_i / P_i, -lo_i, hi_i)), P = p0-2,
+                              lo = p3-5, hi = p6-8 (FiniteRepetitionIF)                          */
+  EBG_CSG_POP_POINT,
+  /* value transforms of the top value */
+  EBG_CSG_SCALE_VALUE,     /* v * p0                                                              */
+  EBG_CSG_NEGATE,          /* -v (ComplementIF)                                                   */
+  EBG_CSG_OFFSET,          /* v - p0 (OffsetIF)                                                   */
+  EBG_CSG_ANNULAR,         /* |v| - p0 (AnnularIF)                                                */
+  /* folds: pop v into the value(s) below */
+  EBG_CSG_MIN_ACC,         /* u = min(u, v)                                                       */
+  EBG_CSG_MAX_ACC,         /* u = max(u, v)                                                       */
+  EBG_CSG_ADD_ACC,         /* u = u + v                                                           */
+  EBG_CSG_TWO_MIN_ACC,     /* (a, b) = two smallest of a, b, v (SmoothUnionIF's scan)             */
+  EBG_CSG_TWO_MAX_ACC,     /* (a, b) = two largest (SmoothIntersectionIF's scan)                  */
+  EBG_CSG_SMOOTH_MIN_END,  /* (a, b) -> smooth min, length p0; n = 0 SmoothMin, 1 ExpMin          */
+  EBG_CSG_SMOOTH_MAX_END,  /* (a, b) -> SmoothMax, length p0                                      */
+  EBG_CSG_DIFFERENCE,      /* (a, b) -> max(a, -b)                                                */
+  EBG_CSG_PUSH_CONST,      /* push p0 (may be +-inf: the starting values of the folds)            */
+  /* BVH unions: a = first primitive, b = primitive count; push min over the primitives        */
+  EBG_CSG_BVH_UNION,
+  EBG_CSG_BVH_SMOOTH_UNION /* smooth min of the two smallest, length p0; n = 0 or 1 as above      */
+};
+
+typedef struct ebg_csg_op
+{
+  int32_t op, n, a, b;
+  double  p[12];
+} ebg_csg_op;
+
+/* A primitive of a BVH union: the subprogram ops[start, start + count), which leaves one value
+ * and contains no BVH union, and its bounding volume (the AABB the reference's BVHUnionIF takes
+ * per primitive: the function's value at a point is at least the point's distance to it). */
+typedef struct ebg_csg_prim
+{
+  uint32_t start, count;
+  double   bv_lo[3], bv_hi[3];
+} ebg_csg_prim;
+
+/* The program is ops[0, main_count); ops beyond are subprograms of BVH-union primitives. All
+ * meshes must share their branching factor; they are retained by the program. length_scale sets
+ * the FP64 recheck threshold (recheck_eps * length_scale; 0 = 1). At most 16 stack values and 8
+ * saved points at any time (EBG_UNSUPPORTED beyond). */
+ebg_status ebg_csg_create(const ebg_csg_op* ops, uint64_t num_ops, uint64_t main_count,
+                          const ebg_csg_prim* prims, uint64_t num_prims, ebg_mesh* const* meshes,
+                          uint64_t num_meshes, double length_scale, ebg_csg** out);
+ebg_status ebg_csg_destroy(ebg_csg* program);
+/* Value at n points (device arrays, asynchronous on opts->stream): FP32 evaluation of every
+ * query, FP64 re-evaluation of those with |v| < recheck_eps * length_scale. */
+ebg_status ebg_csg_query(const ebg_csg* program, const float* d_pts, uint64_t n, float* d_value,
+                         const ebg_query_opts* opts);
+ebg_status ebg_csg_query_host(const ebg_csg* program, const float* pts, uint64_t n, float* value,
+                              const ebg_query_opts* opts);
+/* Lattice form (same lattice and layout as ebg_query_grid). */
+ebg_status ebg_csg_query_grid(const ebg_csg* program, const double origin[3], const double spacing[3],
+                              const uint64_t dims[3], float* d_value, const ebg_query_opts* opts);
+ebg_status ebg_csg_query_grid_host(const ebg_csg* program, const double origin[3],
+                                   const double spacing[3], const uint64_t dims[3], float* value,
+                                   const ebg_query_opts* opts);
+/* EBG_STACK_OVERFLOW if a BVH walk of this program ever exceeded its build-time stack bound. */
+ebg_status ebg_csg_check(const ebg_csg* program);
+
+/* ---------------------------------------------------------------------------------------- */
+/* Point clouds -- PointCloudBVH / PointCloudHashGrid queries (Source/EBGeometry_PointCloudBVH  */
+/* .hpp:115-161, PointCloudHashGrid.hpp:91-135): the k nearest cloud points of query points    */
+/* (closestPoint, closestPoints) and of the cloud's own points excluding themselves           */
+/* (nearestNeighbor(s), allNearestNeighbors), k = 1..16, by squared Euclidean distance in FP32. */
+/* Results are sorted nearest first; equal distances keep the order in which the walk met the */
+/* points (any order of exact ties is a valid answer). Missing entries (fewer than k points)   */
+/* are index 0xFFFFFFFF with squared distance +inf.                                            */
+/* ---------------------------------------------------------------------------------------- */
+
+typedef struct ebg_cloud ebg_cloud;
+
+/* n float32 xyz points (host array, copied); opts: branching 4 or 8, leaf_size 1..16 points
+ * (0 = 8); the builder must be EBG_BUILD_SAH. Cloud index i is the i-th input point. */
+ebg_status ebg_cloud_create(const float* xyz, uint64_t n, const ebg_build_opts* opts, ebg_cloud** out);
+ebg_status ebg_cloud_destroy(ebg_cloud* cloud);
+/* k nearest cloud points of n query points: d_index[q * k + j], d_dist2 (nullable) likewise.
+ * Device arrays, asynchronous on opts->stream (presorted along a Hilbert curve unless
+ * opts->presort is 0). */
+ebg_status ebg_cloud_knn(const ebg_cloud* cloud, const float* d_pts, uint64_t n, uint32_t k,
+                         uint32_t* d_index, float* d_dist2, const ebg_query_opts* opts);
+ebg_status ebg_cloud_knn_host(const ebg_cloud* cloud, const float* pts, uint64_t n, uint32_t k,
+                              uint32_t* index, float* dist2, const ebg_query_opts* opts);
+/* k nearest other points of every cloud point (row i: cloud point i; its own index excluded,
+ * coincident points with other indices included). */
+ebg_status ebg_cloud_knn_self(const ebg_cloud* cloud, uint32_t k, uint32_t* d_index, float* d_dist2,
+                              const ebg_query_opts* opts);
+ebg_status ebg_cloud_knn_self_host(const ebg_cloud* cloud, uint32_t k, uint32_t* index, float* dist2,
+                                   const ebg_query_opts* opts);
+/* EBG_STACK_OVERFLOW if a walk of this cloud ever exceeded its build-time stack bound. */
+ebg_status ebg_cloud_check(const ebg_cloud* cloud);
+
+/* ---------------------------------------------------------------------------------------- */
 /* Batched queries over many meshes -- many TriMeshSDF objects (MeshDistanceFunctions.hpp:    */
 /* 376-387) queried by one traversal launch. The reference has no counterpart: its callers    */
 /* loop over meshes and points. A batch owns its meshes (built from raw arrays, optionally    */

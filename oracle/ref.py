@@ -186,3 +186,169 @@ class Scene:
         v = np.zeros(p.shape[0])
         self.L.ebgref_scene_value(self._h, _p(p, _d), p.shape[0], _p(v, _d), nthreads or os.cpu_count() or 1)
         return v
+
+
+# ---------------------------------------------------------------------------------------------
+# expression trees of the package's objects, built with the reference's constructors
+# ---------------------------------------------------------------------------------------------
+
+def expr_nodes(f, mesh_ref: dict):
+    """Post-order (kind, n, params, bvs) constructor calls of the reference reproducing the package
+    expression f (ebgref_expr_value's node kinds). mesh_ref: id(TriMeshSDF) -> index into the
+    reference meshes passed to expr_value."""
+    from ebgeometry_amd import csg as g
+    from ebgeometry_amd.sdf import TriMeshSDF
+    kinds, ns, pars, bvs = [], [], [], []
+
+    def node(k, n=0, p=()):
+        kinds.append(k)
+        ns.append(n)
+        q = np.zeros(12)
+        q[:len(p)] = p
+        pars.append(q)
+
+    def walk(f):
+        if isinstance(f, g.SphereSDF):
+            node(0, 0, [*f.center, f.radius])
+        elif isinstance(f, g.BoxSDF):
+            node(1, 0, [*f.lo, *f.hi])
+        elif isinstance(f, g.PlaneSDF):
+            node(2, 0, [*f.point, *f.normal])
+        elif isinstance(f, g.TorusSDF):
+            node(3, 0, [*f.center, f.major, f.minor])
+        elif isinstance(f, g.CylinderSDF):
+            node(4, 0, [*f.center1, *f.center2, f.radius])
+        elif isinstance(f, g.InfiniteCylinderSDF):
+            node(5, f.axis, [*f.center, f.radius])
+        elif isinstance(f, g.CapsuleSDF):
+            node(6, 0, [*f.tip1, *f.tip2, f.radius])
+        elif isinstance(f, g.InfiniteConeSDF):
+            node(7, 0, [*f.tip, f.angle])
+        elif isinstance(f, g.ConeSDF):
+            node(8, 0, [*f.tip, f.height, f.angle])
+        elif isinstance(f, g.RoundedBoxSDF):
+            node(9, 0, [*f.dimensions, f.curvature])
+        elif isinstance(f, g.RoundedCylinderSDF):
+            node(10, 0, list(f.args))
+        elif isinstance(f, TriMeshSDF):
+            node(11, mesh_ref[id(f)])
+        elif isinstance(f, g.TranslateIF):
+            walk(f.f)
+            node(20, 0, f.shift)
+        elif isinstance(f, g.RotateIF):
+            walk(f.f)
+            node(21, f.axis, [f.angle])
+        elif isinstance(f, g.ScaleIF):
+            walk(f.f)
+            node(22, 0, [f.scale])
+        elif isinstance(f, g.ComplementIF):
+            walk(f.f)
+            node(23)
+        elif isinstance(f, g.OffsetIF):
+            walk(f.f)
+            node(24, 0, [f.offset])
+        elif isinstance(f, g.AnnularIF):
+            walk(f.f)
+            node(25, 0, [f.delta])
+        elif isinstance(f, g.BlurIF):
+            walk(f.f)
+            node(26, 0, [f.d, f.alpha])
+        elif isinstance(f, g.MollifyIF):
+            # the reference's Mollify factory: a sphere mollifier of radius |dist|
+            assert isinstance(f.mollifier, g.SphereSDF) and abs(f.mollifier.radius) == abs(f.max_value)
+            walk(f.f)
+            node(27, f.num_points, [f.max_value])
+        elif isinstance(f, g.ElongateIF):
+            walk(f.f)
+            node(28, 0, f.e)
+        elif isinstance(f, g.ReflectIF):
+            walk(f.f)
+            node(29, f.plane)
+        elif isinstance(f, g.FiniteRepetitionIF):
+            walk(f.f)
+            node(30, 0, [*f.period, *f.lo, *f.hi])
+        elif isinstance(f, g.UnionIF):
+            for x in f.fs:
+                walk(x)
+            node(40, len(f.fs))
+        elif isinstance(f, g.SmoothUnionIF):
+            for x in f.fs:
+                walk(x)
+            node(41, len(f.fs), [f.s, float(f.kind)])
+        elif isinstance(f, g.IntersectionIF):
+            for x in f.fs:
+                walk(x)
+            node(42, len(f.fs))
+        elif isinstance(f, g.SmoothIntersectionIF):
+            for x in f.fs:
+                walk(x)
+            node(43, len(f.fs), [f.s])
+        elif isinstance(f, g.DifferenceIF):
+            bs = list(f.b_args) if isinstance(f.b_args, (list, tuple)) else [f.b_args]
+            walk(f.a)
+            for x in bs:
+                walk(x)
+            node(44, 1 + len(bs)) if len(bs) > 1 else node(44, 2)
+        elif isinstance(f, g.SmoothDifferenceIF):
+            bs = list(f.b_args) if isinstance(f.b_args, (list, tuple)) else [f.b_args]
+            walk(f.a)
+            for x in bs:
+                walk(x)
+            node(45, 1 + len(bs), [f.s])
+        elif isinstance(f, g.BVHUnionIF):
+            for x, (lo, hi) in zip(f._prims, f._bounds()):
+                walk(x)
+                bvs.append(np.r_[np.asarray(lo, float), np.asarray(hi, float)])
+            node(46, len(f._prims))
+        elif isinstance(f, g.BVHSmoothUnionIF):
+            for x, (lo, hi) in zip(f.prims, f.bvs):
+                walk(x)
+                bvs.append(np.r_[np.asarray(lo, float), np.asarray(hi, float)])
+            node(47, len(f.prims), [f.s, float(f.kind)])
+        else:
+            raise TypeError(f"no reference counterpart for {type(f).__name__}")
+
+    walk(f)
+    return (np.array(kinds, np.int32), np.array(ns, np.int32), np.array(pars, np.float64).reshape(-1, 12),
+            np.array(bvs, np.float64).reshape(-1, 6))
+
+
+def expr_value(f, pts, meshes=(), mesh_ref=None, variant: str = "portable", nthreads=None):
+    """The reference's value() of the expression f at pts (n, 3) float64. meshes: reference Mesh
+    objects; mesh_ref: id(package TriMeshSDF) -> index into meshes."""
+    kinds, ns, pars, bvs = expr_nodes(f, mesh_ref or {})
+    L = lib(variant)
+    if not hasattr(L, "_expr_typed"):
+        L.ebgref_expr_value.restype = C.c_int
+        L.ebgref_expr_value.argtypes = [C.POINTER(C.c_int32), C.POINTER(C.c_int32), _d, C.c_int64, _d,
+                                        C.POINTER(C.c_void_p), _d, C.c_int64, _d, C.c_int]
+        L._expr_typed = True
+    p = _f64(pts, (-1, 3))
+    out = np.empty(p.shape[0], np.float64)
+    mh = (C.c_void_p * max(1, len(meshes)))(*[m.handle for m in meshes])
+    bv = bvs if bvs.size else np.zeros((1, 6))
+    rc = L.ebgref_expr_value(kinds.ctypes.data_as(C.POINTER(C.c_int32)), ns.ctypes.data_as(C.POINTER(C.c_int32)),
+                             _p(pars, _d), kinds.size, _p(bv, _d), mh, _p(p, _d), p.shape[0], _p(out, _d),
+                             nthreads or os.cpu_count() or 1)
+    if rc != 0:
+        raise ValueError(f"reference rejected expression node {-rc - 1}")
+    return out
+
+
+def cloud_knn(points, queries, k, self_query=False, variant: str = "portable", nthreads=None):
+    """The reference's PointCloudBVH<double>: closestPoints of every query (self_query False) or
+    allNearestNeighbors (True); (index int64 (n, k), d2 (n, k)), -1 / +inf where missing."""
+    L = lib(variant)
+    if not hasattr(L, "_cloud_typed"):
+        L.ebgref_cloud_knn.restype = C.c_int
+        L.ebgref_cloud_knn.argtypes = [_d, C.c_int64, _d, C.c_int64, C.c_int64, C.c_int, C.POINTER(C.c_int64), _d,
+                                       C.c_int]
+        L._cloud_typed = True
+    P = _f64(points, (-1, 3))
+    Q = P if self_query else _f64(queries, (-1, 3))
+    n = Q.shape[0]
+    idx = np.empty((n, k), np.int64)
+    d2 = np.empty((n, k), np.float64)
+    L.ebgref_cloud_knn(_p(P, _d), P.shape[0], _p(Q, _d), n, int(k), int(bool(self_query)),
+                       idx.ctypes.data_as(C.POINTER(C.c_int64)), _p(d2, _d), nthreads or os.cpu_count() or 1)
+    return idx, d2

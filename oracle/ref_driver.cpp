@@ -19,11 +19,17 @@
 //   * CSG           BVHUnionIF<double, ImplicitFunction<double>, AABBT<double>, K> over spheres,
 //                   boxes and Translate(Rotate(Rotate(Rotate(Scale(TriMeshSDF)))))) instances
 //                   (Source/EBGeometry_CSGImplem.hpp:466-501, TransformImplem.hpp:140-270).
+//   * point clouds  PointCloudBVH<double>: closestPoints (k nearest of external points) and
+//                   allNearestNeighbors (Source/EBGeometry_PointCloudBVHImplem.hpp:235-507).
+//   * expressions   any tree of the reference's analytic SDFs, transforms and CSG operators,
+//                   given in post order as constructor calls (ebgref_expr_value below), built
+//                   with the reference's own constructors and factories and evaluated by value().
 #include <EBGeometry.hpp>
 
 #include <algorithm>
 #include <cstdint>
 #include <cstring>
+#include <limits>
 #include <memory>
 #include <thread>
 #include <vector>
@@ -246,6 +252,105 @@ ebgref_scene_destroy(void* h)
   delete static_cast<RefScene*>(h);
 }
 
+// Expression nodes in post order: kind[i], n[i] (child count / axis / mesh index / samples) and
+// 12 parameters p[12 i ..]; BVH unions take their children's bounding volumes from bv (6 per
+// child: lo, hi), consumed in node order. Returns 0, or -(i + 1) for a malformed node i.
+int
+ebgref_expr_value(const int32_t* kind,
+                  const int32_t* nn,
+                  const double*  par,
+                  int64_t        num_nodes,
+                  const double*  bv,
+                  void* const*   meshes,
+                  const double*  pts,
+                  int64_t        n,
+                  double*        out,
+                  int            nthreads)
+{
+  using Ptr = std::shared_ptr<IF>;
+  std::vector<Ptr> st;
+  int64_t          nb = 0;
+  auto             v3 = [](const double* p) { return Vec3(p[0], p[1], p[2]); };
+  for (int64_t i = 0; i < num_nodes; i++) {
+    const double* p = par + 12 * i;
+    const int     m = nn[i];
+    auto          pop = [&st]() {
+      Ptr f = st.back();
+      st.pop_back();
+      return f;
+    };
+    auto popn = [&st](int c) {
+      std::vector<Ptr> v(st.end() - c, st.end());
+      st.resize(st.size() - c);
+      return v;
+    };
+    switch (kind[i]) {
+    case 0: st.push_back(std::make_shared<SphereSDF<double>>(v3(p), p[3])); break;
+    case 1: st.push_back(std::make_shared<BoxSDF<double>>(v3(p), v3(p + 3))); break;
+    case 2: st.push_back(std::make_shared<PlaneSDF<double>>(v3(p), v3(p + 3))); break;
+    case 3: st.push_back(std::make_shared<TorusSDF<double>>(v3(p), p[3], p[4])); break;
+    case 4: st.push_back(std::make_shared<CylinderSDF<double>>(v3(p), v3(p + 3), p[6])); break;
+    case 5: st.push_back(std::make_shared<InfiniteCylinderSDF<double>>(v3(p), p[3], size_t(m))); break;
+    case 6: st.push_back(std::make_shared<CapsuleSDF<double>>(v3(p), v3(p + 3), p[6])); break;
+    case 7: st.push_back(std::make_shared<InfiniteConeSDF<double>>(v3(p), p[3])); break;
+    case 8: st.push_back(std::make_shared<ConeSDF<double>>(v3(p), p[3], p[4])); break;
+    case 9: st.push_back(std::make_shared<RoundedBoxSDF<double>>(v3(p), p[3])); break;
+    case 10: st.push_back(std::make_shared<RoundedCylinderSDF<double>>(p[0], p[1], p[2])); break;
+    case 11: st.push_back(static_cast<RefMesh*>(meshes[m])->sdf); break;
+    case 20: st.push_back(Translate<double>(pop(), v3(p))); break;
+    case 21: st.push_back(Rotate<double>(pop(), p[0], size_t(m))); break;
+    case 22: st.push_back(Scale<double>(pop(), p[0])); break;
+    case 23: st.push_back(Complement<double>(pop())); break;
+    case 24: st.push_back(Offset<double>(pop(), p[0])); break;
+    case 25: st.push_back(Annular<double>(pop(), p[0])); break;
+    case 26: st.push_back(std::make_shared<BlurIF<double>>(pop(), p[0], p[1])); break;
+    case 27: st.push_back(Mollify<double>(pop(), p[0], size_t(m))); break;
+    case 28: st.push_back(Elongate<double>(pop(), v3(p))); break;
+    case 29: st.push_back(Reflect<double>(pop(), size_t(m))); break;
+    case 30: st.push_back(FiniteRepetition<double>(pop(), v3(p), v3(p + 3), v3(p + 6))); break;
+    case 40: st.push_back(Union<double>(popn(m))); break;
+    case 41:
+      if (p[1] != 0.0) st.push_back(std::make_shared<SmoothUnionIF<double>>(popn(m), p[0], ExpMin<double>));
+      else st.push_back(SmoothUnion<double>(popn(m), p[0]));
+      break;
+    case 42: st.push_back(Intersection<double>(popn(m))); break;
+    case 43: st.push_back(SmoothIntersection<double>(popn(m), p[0])); break;
+    case 44: {
+      std::vector<Ptr> v = popn(m);
+      if (m == 2) st.push_back(Difference<double>(v[0], v[1]));
+      else st.push_back(std::make_shared<DifferenceIF<double>>(v[0], std::vector<Ptr>(v.begin() + 1, v.end())));
+      break;
+    }
+    case 45: {
+      std::vector<Ptr> v = popn(m);
+      if (m == 2) st.push_back(SmoothDifference<double>(v[0], v[1], p[0]));
+      else
+        st.push_back(
+          std::make_shared<SmoothDifferenceIF<double>>(v[0], std::vector<Ptr>(v.begin() + 1, v.end()), p[0]));
+      break;
+    }
+    case 46:
+    case 47: {
+      std::vector<Ptr>  v = popn(m);
+      std::vector<AABB> b;
+      for (int c = 0; c < m; c++, nb++) b.emplace_back(v3(bv + 6 * nb), v3(bv + 6 * nb + 3));
+      if (kind[i] == 46) st.push_back(BVHUnion<double, IF, AABB, K>(v, b));
+      else if (p[1] != 0.0)
+        st.push_back(std::make_shared<BVHSmoothUnionIF<double, IF, AABB, K>>(v, b, p[0], ExpMin<double>));
+      else st.push_back(BVHSmoothUnion<double, IF, AABB, K>(v, b, p[0]));
+      break;
+    }
+    default: return int(-(i + 1));
+    }
+  }
+  if (st.size() != 1) return int(-(num_nodes + 1));
+  const IF& f = *st.back();
+  parallel_for(n, nthreads, [&](int64_t b, int64_t e) {
+    for (int64_t i = b; i < e; i++) out[i] = f.value(Vec3(pts[3 * i], pts[3 * i + 1], pts[3 * i + 2]));
+  });
+  return 0;
+}
+
 void
 ebgref_scene_value(void* h, const double* pts, int64_t n, double* v, int nthreads)
 {
@@ -253,6 +358,42 @@ ebgref_scene_value(void* h, const double* pts, int64_t n, double* v, int nthread
   parallel_for(n, nthreads, [&](int64_t b, int64_t e) {
     for (int64_t i = b; i < e; i++) v[i] = u.value(Vec3(pts[3 * i], pts[3 * i + 1], pts[3 * i + 2]));
   });
+}
+
+// k nearest cloud points of nq query points (self = 0) or of every cloud point excluding itself
+// (self = 1, allNearestNeighbors; queries ignored): index (cloud order) and squared distance,
+// missing entries index -1 / +inf.
+int
+ebgref_cloud_knn(const double* pts, int64_t n, const double* q, int64_t nq, int64_t k, int self, int64_t* idx,
+                 double* d2, int nthreads)
+{
+  std::vector<Vec3> pos(n);
+  for (int64_t i = 0; i < n; i++) pos[i] = Vec3(pts[3 * i], pts[3 * i + 1], pts[3 * i + 2]);
+  std::vector<std::size_t> meta(n);
+  for (int64_t i = 0; i < n; i++) meta[i] = size_t(i);
+  const PointCloudBVH<double> cloud(pos, meta);
+  using Hit = PointCloudBVH<double>::Hit;
+  if (self) {
+    const std::vector<Hit> r = cloud.allNearestNeighbors(size_t(k));
+    for (int64_t i = 0; i < n * k; i++) {
+      const bool ok = r[i].index < size_t(n);
+      idx[i]        = ok ? int64_t(r[i].index) : -1;
+      d2[i]         = ok ? r[i].distanceSquared : std::numeric_limits<double>::infinity();
+    }
+    return 0;
+  }
+  parallel_for(nq, nthreads, [&](int64_t b, int64_t e) {
+    std::vector<Hit> h(k);
+    for (int64_t i = b; i < e; i++) {
+      const size_t found = cloud.closestPoints(Vec3(q[3 * i], q[3 * i + 1], q[3 * i + 2]), size_t(k), h.data());
+      for (int64_t j = 0; j < k; j++) {
+        const bool ok         = size_t(j) < found;
+        idx[i * k + j]        = ok ? int64_t(h[j].index) : -1;
+        d2[i * k + j]         = ok ? h[j].distanceSquared : std::numeric_limits<double>::infinity();
+      }
+    }
+  });
+  return 0;
 }
 
 } // extern "C"
