@@ -7,8 +7,8 @@ ImplicitFunction / SignedDistanceFunction / TriMeshSDF interface. Compute runs i
 libebgpu.so (hand-written HIP kernels + C ABI, include/ebgpu.h); this package is the host
 mirror of the reference API over that ABI.
 """
-from .sdf import (Build, ImplicitFunction, SignedDistanceFunction, TriMeshSDF, grid_points, morton_codes,
-                  morton_sort, pseudo_normals)
+from .sdf import (Build, ImplicitFunction, MeshSDF, SignedDistanceFunction, TriMeshSDF, grid_points, morton_codes,
+                  morton_sort, polygons_to_triangles, pseudo_normals)
 from .batch import BatchPlan, MeshBatch
 from .cloud import Hit, PointCloudBVH, PointCloudHashGrid
 from .csg import (AABB, Annular, AnnularIF, Blur, BlurIF, BoxSDF, BVHSmoothUnion, BVHSmoothUnionIF, BVHUnion,
@@ -29,4 +29,4 @@ __all__ = ["Build", "ImplicitFunction", "SignedDistanceFunction", "TriMeshSDF", 
            "Mollify", "MollifyIF", "Offset", "OffsetIF", "PlaneSDF", "Reflect", "ReflectIF", "RoundedBoxSDF",
            "RoundedCylinderSDF", "SmoothDifference", "SmoothDifferenceIF", "SmoothIntersection",
            "SmoothIntersectionIF", "SmoothMin", "SmoothUnion", "SmoothUnionIF", "TorusSDF", "Union", "UnionIF",
-           "Hit", "PointCloudBVH", "PointCloudHashGrid"]
+           "Hit", "PointCloudBVH", "PointCloudHashGrid", "MeshSDF", "polygons_to_triangles"]

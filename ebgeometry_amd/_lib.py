@@ -71,6 +71,9 @@ class BatchItem(C.Structure):
 SIGNATURES = [
     ("ebg_last_error", C.c_char_p, []),
     ("ebg_mesh_create", C.c_int, [_vp, _u64, _vp, _u64, C.POINTER(BuildOpts), C.POINTER(_vp)]),
+    ("ebg_mesh_create_polygons", C.c_int, [_vp, _u64, _vp, _vp, _u64, C.POINTER(BuildOpts), C.POINTER(_vp)]),
+    ("ebg_mesh_triangle_faces", C.c_int, [_vp, _vp]),
+    ("ebg_polygons_to_triangles_host", C.c_int, [_vp, _u64, _vp, _vp, _u64, _vp, _vp, _vp, _vp, _vp]),
     ("ebg_mesh_destroy", C.c_int, [_vp]),
     ("ebg_mesh_get_info", C.c_int, [_vp, C.POINTER(MeshInfo)]),
     ("ebg_mesh_leaf_order", C.c_int, [_vp, _vp]),

@@ -44,11 +44,13 @@ SOURCES = [
     ("batch.hip", DEVICE),
     ("morton.hip", DEVICE),
     ("morton_build.hip", DEVICE),
+    ("gpu_build.hip", DEVICE),
     ("util.hip", DEVICE),
     ("mlp.hip", DEVICE),
     ("noalloc.hip", DEVICE),
 ]
-HEADERS = ["ebgpu_internal.hpp", "kernels.hpp", "mesh_build.hpp", "mlp.hpp", "traverse_dev.hpp", "host_common.hpp"]
+HEADERS = ["ebgpu_internal.hpp", "kernels.hpp", "mesh_build.hpp", "mlp.hpp", "traverse_dev.hpp", "host_common.hpp",
+           "gpu_build.hpp"]
 
 
 def _mtime(p):

@@ -10,6 +10,7 @@
 //   4. FP64 traversal for queries with |d| < recheck_eps * diag (sign is ill-conditioned in
 //      FP32 there), seeded with |d32| + slack
 #include "host_common.hpp"
+#include "gpu_build.hpp"
 #include "mesh_build.hpp"
 
 #include <hip/hip_runtime.h>
@@ -102,6 +103,9 @@ void ebg_query_opts_default(ebg_query_opts* o)
   o->bound_offset = 1e-3f;
 }
 
+static ebg_status finalize_mesh(ebg_mesh* m, const ebgpu::PackedMesh& pm);
+
+// Upload a mesh packed on the host, then finish the handle.
 static ebg_status upload_mesh(ebg_mesh* m, ebgpu::PackedMesh& pm)
 {
   EBG_HIP(upload(m->nodes, pm.nodes));
@@ -131,6 +135,33 @@ static ebg_status upload_mesh(ebg_mesh* m, ebgpu::PackedMesh& pm)
     if (nt) cdf[nt - 1] = 1.0f;
     EBG_HIP(upload(m->area_cdf, cdf));
   }
+  return finalize_mesh(m, pm);
+}
+
+// Adopt a mesh packed on the device (its nodes come from the host collapse), then finish the handle.
+static ebg_status adopt_gpu_mesh(ebg_mesh* m, ebgpu::GpuBuiltMesh& g)
+{
+  auto take = [](ebgpu::DevBuf& dst, ebgpu::DevBuf& src) {
+    std::swap(dst.p, src.p);
+    std::swap(dst.bytes, src.bytes);
+  };
+  EBG_HIP(upload(m->nodes, g.meta.nodes));
+  take(m->tris, g.tris);
+  take(m->tripos, g.tripos);
+  take(m->tsph, g.tsph);
+  take(m->feat, g.feat);
+  take(m->vn32, g.vn32);
+  take(m->en32, g.en32);
+  take(m->vn64, g.vn64);
+  take(m->en64, g.en64);
+  take(m->fn64, g.fn64);
+  take(m->area_cdf, g.area_cdf);
+  return finalize_mesh(m, g.meta);
+}
+
+// Split frontier, counts, stack banks, bounds and the device view of an uploaded mesh.
+static ebg_status finalize_mesh(ebg_mesh* m, const ebgpu::PackedMesh& pm)
+{
   {
     const std::vector<ebgpu::SplitRoot> roots =
       pm.K == 4 ? split_frontier<4>(pm, kSplitRoots) : split_frontier<8>(pm, kSplitRoots);
@@ -177,35 +208,45 @@ static ebg_status upload_mesh(ebg_mesh* m, ebgpu::PackedMesh& pm)
   return EBG_OK;
 }
 
-ebg_status ebg_mesh_create(const float* xyz, uint64_t nv, const uint32_t* tri, uint64_t nt,
-                           const ebg_build_opts* opts, ebg_mesh** out)
+// Build and upload a triangle mesh whose pseudo-normals are given (both creation entries).
+static ebg_status create_mesh(const float* xyz, uint64_t nv, const uint32_t* tri, uint64_t nt,
+                              const ebgpu::PseudoNormals* pn, const ebg_build_opts* opts,
+                              std::chrono::steady_clock::time_point t0, ebg_mesh** out)
 {
-  if (!out) return fail(EBG_INVALID_ARG, "out is null");
-  *out = nullptr;
-  if (!xyz || !tri || nv < 3 || nt < 1) return fail(EBG_INVALID_ARG, "empty mesh");
-  if (nt > 0x7FFFFFFFull / 3 || nv > 0xFFFFFFFFull) return fail(EBG_INVALID_ARG, "mesh too large");
-  for (uint64_t i = 0; i < 3 * nv; i++)
-    if (!std::isfinite(xyz[i])) return fail(EBG_INVALID_ARG, "non-finite vertex coordinate");
-  for (uint64_t i = 0; i < 3 * nt; i++)
-    if (tri[i] >= nv) return fail(EBG_INVALID_ARG, "triangle index out of range");
   ebg_build_opts o{};
   if (opts) o = *opts;
   const int K    = o.branching ? o.branching : 4;
   const int leaf = o.leaf_size ? o.leaf_size : 4;
   if (K != 4 && K != 8) return fail(EBG_INVALID_ARG, "branching must be 4 or 8");
   if (leaf < 1 || leaf > ebgpu::kMaxLeaf) return fail(EBG_INVALID_ARG, "leaf_size must be 1..16");
-
-  const auto t0 = std::chrono::steady_clock::now();
-  ebgpu::PseudoNormals pn;
-  std::string          err;
-  if (!ebgpu::compute_pseudo_normals(xyz, nv, tri, nt, pn, err)) return fail(EBG_INVALID_ARG, err);
   ebgpu::PackedMesh pm;
+  std::string       err;
+  if (o.builder == EBG_BUILD_SAH_GPU) {
+    ebgpu::GpuBuiltMesh g;
+    const ebg_status    bs = ebgpu::build_mesh_gpu_sah(xyz, nv, tri, nt, K, leaf, pn, g, err);
+    if (bs != EBG_OK) return fail(bs, err);
+    auto* m = new ebg_mesh;
+    m->nv   = nv;
+    m->nt   = nt;
+    m->hash = ebgpu::mesh_hash(xyz, nv, tri, nt);
+    m->host_xyz.assign(xyz, xyz + 3 * nv);
+    m->order = g.meta.order;
+    const ebg_status st = adopt_gpu_mesh(m, g);
+    if (st != EBG_OK) {
+      delete m;
+      return st;
+    }
+    m->build_seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    *out             = m;
+    return EBG_OK;
+  }
+  if (!pn) return fail(EBG_INVALID_ARG, "no pseudo-normals");
   if (o.builder == EBG_BUILD_MORTON) {
-    if (!ebgpu::build_morton_mesh_gpu(xyz, nv, tri, nt, K, o.leaf_size ? o.leaf_size : 1, pn, pm, err))
+    if (!ebgpu::build_morton_mesh_gpu(xyz, nv, tri, nt, K, o.leaf_size ? o.leaf_size : 1, *pn, pm, err))
       return fail(EBG_HIP_ERROR, err);
   }
   else if (o.builder == EBG_BUILD_SAH) {
-    if (!ebgpu::build_sah_mesh(xyz, nv, tri, nt, K, leaf, pn, pm, err)) return fail(EBG_INVALID_ARG, err);
+    if (!ebgpu::build_sah_mesh(xyz, nv, tri, nt, K, leaf, *pn, pm, err)) return fail(EBG_INVALID_ARG, err);
   }
   else {
     return fail(EBG_INVALID_ARG, "unknown builder");
@@ -224,6 +265,68 @@ ebg_status ebg_mesh_create(const float* xyz, uint64_t nv, const uint32_t* tri, u
   }
   m->build_seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
   *out             = m;
+  return EBG_OK;
+}
+
+static ebg_status check_vertices(const float* xyz, uint64_t nv)
+{
+  for (uint64_t i = 0; i < 3 * nv; i++)
+    if (!std::isfinite(xyz[i])) return fail(EBG_INVALID_ARG, "non-finite vertex coordinate");
+  return EBG_OK;
+}
+
+ebg_status ebg_mesh_create(const float* xyz, uint64_t nv, const uint32_t* tri, uint64_t nt,
+                           const ebg_build_opts* opts, ebg_mesh** out)
+{
+  if (!out) return fail(EBG_INVALID_ARG, "out is null");
+  *out = nullptr;
+  if (!xyz || !tri || nv < 3 || nt < 1) return fail(EBG_INVALID_ARG, "empty mesh");
+  if (nt > 0x7FFFFFFFull / 3 || nv > 0xFFFFFFFFull) return fail(EBG_INVALID_ARG, "mesh too large");
+  if (check_vertices(xyz, nv) != EBG_OK) return EBG_INVALID_ARG;
+  for (uint64_t i = 0; i < 3 * nt; i++)
+    if (tri[i] >= nv) return fail(EBG_INVALID_ARG, "triangle index out of range");
+  const auto t0 = std::chrono::steady_clock::now();
+  if (opts && opts->builder == EBG_BUILD_SAH_GPU) return create_mesh(xyz, nv, tri, nt, nullptr, opts, t0, out);
+  ebgpu::PseudoNormals pn;
+  std::string          err;
+  if (!ebgpu::compute_pseudo_normals(xyz, nv, tri, nt, pn, err)) return fail(EBG_INVALID_ARG, err);
+  return create_mesh(xyz, nv, tri, nt, &pn, opts, t0, out);
+}
+
+ebg_status ebg_mesh_create_polygons(const float* xyz, uint64_t nv, const uint32_t* face_offsets,
+                                    const uint32_t* face_vertices, uint64_t nf, const ebg_build_opts* opts,
+                                    ebg_mesh** out)
+{
+  if (!out) return fail(EBG_INVALID_ARG, "out is null");
+  *out = nullptr;
+  if (!xyz || !face_offsets || !face_vertices || nv < 3 || nf < 1) return fail(EBG_INVALID_ARG, "empty mesh");
+  if (nv > 0xFFFFFFFFull || nf > 0x7FFFFFFFull / 3 || face_offsets[nf] > 0x7FFFFFFFull / 3)
+    return fail(EBG_INVALID_ARG, "mesh too large");
+  if (check_vertices(xyz, nv) != EBG_OK) return EBG_INVALID_ARG;
+  const auto            t0 = std::chrono::steady_clock::now();
+  std::vector<uint32_t> tri, tri_face;
+  ebgpu::PseudoNormals  pn;
+  std::string           err;
+  if (!ebgpu::polygon_mesh(xyz, nv, face_offsets, face_vertices, nf, tri, tri_face, pn, err))
+    return fail(EBG_INVALID_ARG, err);
+  const uint64_t   nt = tri_face.size();
+  const ebg_status st = create_mesh(xyz, nv, tri.data(), nt, &pn, opts, t0, out);
+  if (st != EBG_OK) return st;
+  ebg_mesh* m = *out;
+  m->nf       = nf;
+  m->host_tri_face = tri_face;
+  if (upload(m->tri_face, tri_face) != hipSuccess) {
+    ebgpu::mesh_release(m);
+    *out = nullptr;
+    return fail(EBG_OOM, "face map upload");
+  }
+  return EBG_OK;
+}
+
+ebg_status ebg_mesh_triangle_faces(const ebg_mesh* m, uint32_t* tri_face)
+{
+  if (!m || !tri_face) return fail(EBG_INVALID_ARG, "null mesh or output");
+  for (uint64_t t = 0; t < m->nt; t++) tri_face[t] = m->host_tri_face.empty() ? uint32_t(t) : m->host_tri_face[t];
   return EBG_OK;
 }
 
@@ -247,6 +350,31 @@ ebg_status ebg_pseudo_normals_host(const float* xyz, uint64_t nv, const uint32_t
     for (int v = 0; v < 3; v++)
       for (int c = 0; c < 3; c++) {
         vn[9 * f + 3 * v + c] = pn.vert[3 * tri[3 * f + v] + c];
+        en[9 * f + 3 * v + c] = pn.edge[9 * f + 3 * v + c];
+      }
+  }
+  return EBG_OK;
+}
+
+ebg_status ebg_polygons_to_triangles_host(const float* xyz, uint64_t nv, const uint32_t* face_offsets,
+                                         const uint32_t* face_vertices, uint64_t nf, uint32_t* tri, uint32_t* tri_face,
+                                         double* fn, double* vn, double* en)
+{
+  if (!xyz || !face_offsets || !face_vertices || !tri || !tri_face || !fn || !vn || !en || nv < 3 || nf < 1)
+    return fail(EBG_INVALID_ARG, "bad arguments");
+  std::vector<uint32_t> t, tf;
+  ebgpu::PseudoNormals  pn;
+  std::string           err;
+  if (!ebgpu::polygon_mesh(xyz, nv, face_offsets, face_vertices, nf, t, tf, pn, err)) return fail(EBG_INVALID_ARG, err);
+  const uint64_t nt = tf.size();
+  if (nt != uint64_t(face_offsets[nf]) - 2 * nf) return fail(EBG_INVALID_ARG, "unexpected triangle count");
+  std::copy(t.begin(), t.end(), tri);
+  std::copy(tf.begin(), tf.end(), tri_face);
+  for (uint64_t f = 0; f < nt; f++) {
+    for (int c = 0; c < 3; c++) fn[3 * f + c] = pn.face[3 * f + c];
+    for (int v = 0; v < 3; v++)
+      for (int c = 0; c < 3; c++) {
+        vn[9 * f + 3 * v + c] = pn.vert[3 * t[3 * f + v] + c];
         en[9 * f + 3 * v + c] = pn.edge[9 * f + 3 * v + c];
       }
   }
@@ -478,6 +606,8 @@ ebg_status query_pipeline(ebg_mesh* m, const float* d_pts, uint64_t n, float* d_
     r.slack_rel = 1e-4f;
     EBG_HIP(launch_sdf64(m->dev, m->K, m->banks, r, s));
   }
+  // polygon meshes report the face each closest triangle was clipped from
+  if (d_tri && m->tri_face.p) EBG_HIP(launch_map_u32(d_tri, n, m->tri_face.as<uint32_t>(), uint32_t(m->nt), s));
   if (count) EBG_HIP(launch_add_counts(a.work, recheck ? cnt : nullptr, seeds ? cnt + 1 : nullptr, s));
   return EBG_OK;
 }

@@ -233,6 +233,11 @@ struct ebg_mesh
   int64_t                last_split_cap = 0;     // ... up to this many
   std::vector<float>     host_xyz; // vertex positions (scene instance bounds)
   std::vector<uint32_t>  order;    // original face index of each leaf-order triangle
+  // polygon meshes: the polygon each triangle was clipped from (host and device copies); the
+  // closest-face outputs are mapped through it
+  uint64_t               nf = 0;
+  std::vector<uint32_t>  host_tri_face;
+  ebgpu::DevBuf          tri_face;
   ebgpu::MeshDev         dev{};
   ebgpu::MlpTrainBuffers train;
   std::size_t            device_bytes = 0;
@@ -260,12 +265,12 @@ constexpr uint32_t kCostGrid = 32; // cells per axis of a mesh's packet cost gri
 // Build the mesh's packet cost grid once (a counting launch of 64 lattice queries per cell on
 // stream s, then a wait); defined with the query pipeline (api.cpp).
 ebg_status ensure_cost_grid(ebg_mesh* m, hipStream_t s);
+// Validate a lattice (origin, spacing, dims; at most 2^32-1 points) into g; n = its point count.
+ebg_status grid_spec(const double* origin, const double* spacing, const uint64_t* dims, GridSpec& g, uint64_t& n);
 // The mesh query pipeline (presort, [bound network], FP32 traversal with split packets, FP64
 // recheck) over a caller-chosen workspace of at least n queries. With may_alloc false nothing is
 // allocated or created: split packets only if the cost grid already exists, the bound network
 // on the caller's stream, no traversal timing. The caller holds m->mu.
-// Validate a lattice (origin, spacing, dims; at most 2^32-1 points) into g; n = its point count.
-ebg_status grid_spec(const double* origin, const double* spacing, const uint64_t* dims, GridSpec& g, uint64_t& n);
 ebg_status query_pipeline(ebg_mesh* m, const float* d_pts, uint64_t n, float* d_dist, uint32_t* d_tri,
                           const ebg_query_opts& o, const WsView& ws, bool may_alloc);
 inline void mesh_retain(ebg_mesh* m) { m->refs.fetch_add(1); }

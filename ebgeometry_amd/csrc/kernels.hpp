@@ -179,6 +179,8 @@ hipError_t launch_knn(const CloudDev& c, int K, int banks, const KnnArgs& a, hip
 
 // util.hip
 hipError_t launch_fill_u32(uint32_t* p, uint64_t n, uint32_t v, hipStream_t s);
+// v[i] <- map[v[i]] where v[i] < nmap (polygon meshes: triangle index -> face index)
+hipError_t launch_map_u32(uint32_t* v, uint64_t n, const uint32_t* map, uint32_t nmap, hipStream_t s);
 // n doubles -> floats (round to nearest even) and n floats -> doubles
 hipError_t launch_f64_to_f32(const double* in, float* out, uint64_t n, hipStream_t s);
 

@@ -113,6 +113,257 @@ bool compute_pseudo_normals(const float* xyz, std::size_t nv, const uint32_t* tr
   return true;
 }
 
+namespace {
+
+// Signed area of a 2D polygon given by coordinates (u, v) of its vertices (twice the area).
+double area2(const std::vector<double>& u, const std::vector<double>& v)
+{
+  double a = 0.0;
+  for (std::size_t i = 0, n = u.size(); i < n; i++) {
+    const std::size_t j = (i + 1) % n;
+    a += u[i] * v[j] - u[j] * v[i];
+  }
+  return a;
+}
+
+// Twice the signed area of the 2D triangle (a, b, c).
+inline double orient2(double au, double av, double bu, double bv, double cu, double cv)
+{
+  return (bu - au) * (cv - av) - (cu - au) * (bv - av);
+}
+
+// Ear clipping of one simple polygon given in 2D (vertex k at (u[k], v[k])): appends triangles as
+// triples of positions 0..n-1 in the polygon's own vertex order (each triangle keeps the polygon's
+// winding). An ear is a convex corner whose triangle holds no other vertex (points on its
+// boundary count as inside, so collinear chains are not cut through). Returns false when no ear
+// exists (a self-intersecting or degenerate polygon).
+bool ear_clip(const std::vector<double>& u, const std::vector<double>& v, std::vector<uint32_t>& out)
+{
+  const std::size_t n    = u.size();
+  // simple: no two edges that do not share a corner meet (touching counts)
+  for (std::size_t i = 0; i < n; i++)
+    for (std::size_t j = i + 1; j < n; j++) {
+      const std::size_t i1 = (i + 1) % n, j1 = (j + 1) % n;
+      if (j == i1 || i == j1) continue;
+      const double d1 = orient2(u[i], v[i], u[i1], v[i1], u[j], v[j]);
+      const double d2 = orient2(u[i], v[i], u[i1], v[i1], u[j1], v[j1]);
+      const double d3 = orient2(u[j], v[j], u[j1], v[j1], u[i], v[i]);
+      const double d4 = orient2(u[j], v[j], u[j1], v[j1], u[i1], v[i1]);
+      if (((d1 > 0) != (d2 > 0) || d1 == 0 || d2 == 0) && ((d3 > 0) != (d4 > 0) || d3 == 0 || d4 == 0)) {
+        // collinear pairs only meet when their extents overlap
+        if (d1 == 0 && d2 == 0) {
+          const bool ox = std::max(u[i], u[i1]) >= std::min(u[j], u[j1]) && std::max(u[j], u[j1]) >= std::min(u[i], u[i1]);
+          const bool oy = std::max(v[i], v[i1]) >= std::min(v[j], v[j1]) && std::max(v[j], v[j1]) >= std::min(v[i], v[i1]);
+          if (!(ox && oy)) continue;
+        }
+        return false;
+      }
+    }
+  const double      sgn  = area2(u, v) > 0.0 ? 1.0 : -1.0;
+  std::vector<uint32_t> ring(n);
+  for (std::size_t i = 0; i < n; i++) ring[i] = uint32_t(i);
+  while (ring.size() > 3) {
+    const std::size_t m     = ring.size();
+    bool              found = false;
+    for (std::size_t i = 0; i < m && !found; i++) {
+      const uint32_t a = ring[(i + m - 1) % m], b = ring[i], c = ring[(i + 1) % m];
+      if (sgn * orient2(u[a], v[a], u[b], v[b], u[c], v[c]) <= 0.0) continue; // reflex or flat corner
+      bool empty = true;
+      for (std::size_t j = 0; j < m && empty; j++) {
+        const uint32_t p = ring[j];
+        if (p == a || p == b || p == c) continue;
+        const double d0 = sgn * orient2(u[a], v[a], u[b], v[b], u[p], v[p]);
+        const double d1 = sgn * orient2(u[b], v[b], u[c], v[c], u[p], v[p]);
+        const double d2 = sgn * orient2(u[c], v[c], u[a], v[a], u[p], v[p]);
+        if (d0 >= 0.0 && d1 >= 0.0 && d2 >= 0.0) empty = false;
+      }
+      if (!empty) continue;
+      out.push_back(a);
+      out.push_back(b);
+      out.push_back(c);
+      ring.erase(ring.begin() + std::ptrdiff_t(i));
+      found = true;
+    }
+    if (!found) return false;
+  }
+  out.push_back(ring[0]);
+  out.push_back(ring[1]);
+  out.push_back(ring[2]);
+  return true;
+}
+
+} // namespace
+
+bool polygon_mesh(const float* xyz, std::size_t nv, const uint32_t* offsets, const uint32_t* idx, std::size_t nf,
+                  std::vector<uint32_t>& tri, std::vector<uint32_t>& tri_face, PseudoNormals& pn, std::string& err)
+{
+  if (offsets[0] != 0) {
+    err = "face offsets must start at 0";
+    return false;
+  }
+  for (std::size_t f = 0; f < nf; f++) {
+    if (offsets[f + 1] < offsets[f] + 3) {
+      err = "face " + std::to_string(f) + " has fewer than 3 vertices";
+      return false;
+    }
+    for (uint32_t k = offsets[f]; k < offsets[f + 1]; k++) {
+      if (idx[k] >= nv) {
+        err = "face " + std::to_string(f) + ": vertex index out of range";
+        return false;
+      }
+      for (uint32_t j = offsets[f]; j < k; j++)
+        if (idx[j] == idx[k]) {
+          err = "face " + std::to_string(f) + " repeats a vertex";
+          return false;
+        }
+    }
+  }
+  const std::size_t nh = offsets[nf]; // half-edges: one per face corner, k -> next corner
+  auto corner = [&](std::size_t f, std::size_t i) { // vertex of corner i (mod size) of face f
+    const std::size_t n = offsets[f + 1] - offsets[f];
+    return idx[offsets[f] + i % n];
+  };
+
+  // Face normal from the first corner triple that spans a plane, normalised
+  // (Source/EBGeometry_DCEL_FaceImplem.hpp:138-165).
+  std::vector<D3> fn(nf);
+  for (std::size_t f = 0; f < nf; f++) {
+    const std::size_t n = offsets[f + 1] - offsets[f];
+    D3                nrm{0, 0, 0};
+    for (std::size_t i = 0; i < n; i++) {
+      const D3 x0 = ld3f(xyz + 3 * corner(f, i)), x1 = ld3f(xyz + 3 * corner(f, i + 1)),
+               x2 = ld3f(xyz + 3 * corner(f, i + 2));
+      nrm = cross(x2 - x0, x2 - x1);
+      if (length(nrm) > 0.0) break;
+    }
+    const double l = length(nrm);
+    if (!(l > std::numeric_limits<double>::epsilon())) {
+      err = "degenerate face " + std::to_string(f);
+      return false;
+    }
+    fn[f] = div_by(nrm, l);
+  }
+
+  // The distance to a polygon is taken in its plane: refuse faces whose corners leave the plane
+  // through their centroid by more than 1e-5 of the face's extent.
+  for (std::size_t f = 0; f < nf; f++) {
+    const std::size_t n = offsets[f + 1] - offsets[f];
+    D3                c{0, 0, 0};
+    for (std::size_t i = 0; i < n; i++) c = c + ld3f(xyz + 3 * corner(f, i));
+    c             = (1.0 / double(n)) * c;
+    double ext = 0.0, dev = 0.0;
+    for (std::size_t i = 0; i < n; i++) {
+      const D3 d = ld3f(xyz + 3 * corner(f, i)) - c;
+      ext        = std::max(ext, length(d));
+      dev        = std::max(dev, std::abs(dot(d, fn[f])));
+    }
+    if (dev > 1e-5 * ext) {
+      err = "face " + std::to_string(f) + " is not planar";
+      return false;
+    }
+  }
+
+  // Half-edge pseudo-normals: normalise(n_face + n_pairface) (DCEL_EdgeImplem.hpp:130-165).
+  std::vector<std::size_t> face_of(nh);
+  for (std::size_t f = 0; f < nf; f++)
+    for (uint32_t k = offsets[f]; k < offsets[f + 1]; k++) face_of[k] = f;
+  std::vector<std::pair<uint64_t, uint32_t>> keys(nh);
+  for (std::size_t f = 0; f < nf; f++) {
+    const std::size_t n = offsets[f + 1] - offsets[f];
+    for (std::size_t i = 0; i < n; i++)
+      keys[offsets[f] + i] = {uint64_t(corner(f, i)) * uint64_t(nv) + corner(f, i + 1), uint32_t(offsets[f] + i)};
+  }
+  std::sort(keys.begin(), keys.end());
+  std::vector<D3> hn(nh);
+  for (std::size_t f = 0; f < nf; f++) {
+    const std::size_t n = offsets[f + 1] - offsets[f];
+    for (std::size_t i = 0; i < n; i++) {
+      const uint64_t q  = uint64_t(corner(f, i + 1)) * uint64_t(nv) + corner(f, i);
+      auto           it = std::lower_bound(keys.begin(), keys.end(), std::make_pair(q, uint32_t(0)));
+      D3             s  = fn[f];
+      if (it != keys.end() && it->first == q) s = s + fn[face_of[it->second]];
+      const double l = length(s);
+      hn[offsets[f] + i] = (l > std::numeric_limits<double>::epsilon()) ? div_by(s, l) : D3{0, 0, 0};
+    }
+  }
+
+  // Vertex pseudo-normals: sum over incident faces, in face order, of the corner angle times the
+  // face normal, normalised (DCEL_VertexImplem.hpp:190-300).
+  std::vector<D3>      vn(nv, D3{0, 0, 0});
+  std::vector<uint8_t> touched(nv, 0);
+  for (std::size_t f = 0; f < nf; f++) {
+    const std::size_t n = offsets[f + 1] - offsets[f];
+    for (std::size_t i = 0; i < n; i++) {
+      const uint32_t o  = corner(f, i);
+      const D3       x0 = ld3f(xyz + 3 * o);
+      D3             e1 = ld3f(xyz + 3 * corner(f, i + 1)) - x0;
+      D3             e2 = ld3f(xyz + 3 * corner(f, i + n - 1)) - x0;
+      e1              = div_by(e1, length(e1));
+      e2              = div_by(e2, length(e2));
+      const double al = std::acos(std::clamp(dot(e1, e2), -1.0, 1.0));
+      vn[o]           = vn[o] + al * fn[f];
+      touched[o]      = 1;
+    }
+  }
+
+  // Triangles: each face clipped into ears in the plane of its two non-dominant normal axes
+  // (the projection Source/EBGeometry_Polygon2DImplem.hpp:72-103 uses for its inside test).
+  tri.clear();
+  tri_face.clear();
+  std::vector<double>   u, w;
+  std::vector<uint32_t> ears;
+  std::vector<uint32_t> tri_corner; // per triangle corner: the face-corner index, or kInvalid
+  for (std::size_t f = 0; f < nf; f++) {
+    const std::size_t n = offsets[f + 1] - offsets[f];
+    int               drop = 0;
+    const double      a[3] = {std::abs(fn[f].x), std::abs(fn[f].y), std::abs(fn[f].z)};
+    for (int d = 1; d < 3; d++)
+      if (a[d] > a[drop]) drop = d;
+    const int ax = drop == 0 ? 1 : 0, ay = drop == 2 ? 1 : 2;
+    u.resize(n);
+    w.resize(n);
+    for (std::size_t i = 0; i < n; i++) {
+      u[i] = double(xyz[3 * corner(f, i) + ax]);
+      w[i] = double(xyz[3 * corner(f, i) + ay]);
+    }
+    ears.clear();
+    if (!ear_clip(u, w, ears)) {
+      err = "face " + std::to_string(f) + " is not a simple polygon";
+      return false;
+    }
+    for (std::size_t t = 0; t < ears.size(); t += 3) {
+      for (int k = 0; k < 3; k++) {
+        const uint32_t i = ears[t + k], j = ears[t + (k + 1) % 3];
+        tri.push_back(corner(f, i));
+        // the triangle edge i -> j is the face's half-edge i -> i+1 when j follows i
+        tri_corner.push_back((i + 1) % n == j ? uint32_t(offsets[f] + i) : kInvalid);
+      }
+      tri_face.push_back(uint32_t(f));
+    }
+  }
+
+  const std::size_t nt = tri_face.size();
+  pn.face.assign(3 * nt, 0.0);
+  pn.vert.assign(3 * nv, 0.0);
+  pn.edge.assign(9 * nt, 0.0);
+  for (std::size_t t = 0; t < nt; t++) {
+    const D3 n = fn[tri_face[t]];
+    pn.face[3 * t] = n.x, pn.face[3 * t + 1] = n.y, pn.face[3 * t + 2] = n.z;
+    for (int k = 0; k < 3; k++) {
+      // an edge inside the face (a diagonal of the clipping) has the face's own normal
+      const uint32_t h = tri_corner[3 * t + k];
+      const D3       e = h == kInvalid ? n : hn[h];
+      pn.edge[9 * t + 3 * k] = e.x, pn.edge[9 * t + 3 * k + 1] = e.y, pn.edge[9 * t + 3 * k + 2] = e.z;
+    }
+  }
+  for (std::size_t v = 0; v < nv; v++) {
+    if (!touched[v]) continue;
+    const D3 n = div_by(vn[v], length(vn[v]));
+    pn.vert[3 * v] = n.x, pn.vert[3 * v + 1] = n.y, pn.vert[3 * v + 2] = n.z;
+  }
+  return true;
+}
+
 void pack_triangles(const float* xyz, const uint32_t* tri, const std::vector<uint32_t>& order,
                     const PseudoNormals& pn, std::size_t nv, PackedMesh& out)
 {
@@ -193,12 +444,7 @@ void pack_triangles(const float* xyz, const uint32_t* tri, const std::vector<uin
 
 namespace {
 
-struct BNode
-{
-  float    lo[3], hi[3];
-  int32_t  left, right; // -1 for a leaf
-  uint32_t first, count;
-};
+using BNode = BinNode;
 
 inline float area(const float* lo, const float* hi)
 {
@@ -446,6 +692,34 @@ bool build_sah_mesh(const float* xyz, std::size_t nv, const uint32_t* tri, std::
   return true;
 }
 
+void collapse_binary(const std::vector<BinNode>& bn, int32_t root, int K, WideBvh& out)
+{
+  const BinNode& r = bn[root];
+  for (int c = 0; c < 3; c++) {
+    out.lo[c] = r.lo[c];
+    out.hi[c] = r.hi[c];
+  }
+  const double dx = double(r.hi[0]) - r.lo[0], dy = double(r.hi[1]) - r.lo[1], dz = double(r.hi[2]) - r.lo[2];
+  out.diag = std::sqrt(dx * dx + dy * dy + dz * dz);
+  PackedMesh tmp; // only to reuse compute_stack_need
+  tmp.K = K;
+  auto collapse = [&](auto tag) {
+    constexpr int KK = decltype(tag)::value;
+    Collapser<KK> col(bn);
+    if (r.left < 0) col.emit({root}); // single leaf: root with one leaf child
+    else col.emit({r.left, r.right});
+    out.num_nodes  = col.out.size();
+    out.num_leaves = col.leaves;
+    out.nodes.resize(col.out.size() * sizeof(Node<KK>));
+    std::memcpy(out.nodes.data(), col.out.data(), out.nodes.size());
+  };
+  if (K == 4) collapse(std::integral_constant<int, 4>{});
+  else collapse(std::integral_constant<int, 8>{});
+  tmp.nodes      = out.nodes;
+  tmp.num_nodes  = out.num_nodes;
+  out.stack_need = compute_stack_need(tmp);
+}
+
 bool build_wide_bvh(const float* blo, const float* bhi, std::size_t n, int K, int leaf_size, WideBvh& out,
                     std::string& err)
 {
@@ -470,32 +744,7 @@ bool build_wide_bvh(const float* blo, const float* bhi, std::size_t n, int K, in
   const int32_t root = sb.alloc();
   sb.build(root, 0, uint32_t(n), 0);
   sb.nodes.resize(std::size_t(sb.next.load()));
-
-  const BNode& r = sb.nodes[root];
-  for (int c = 0; c < 3; c++) {
-    out.lo[c] = r.lo[c];
-    out.hi[c] = r.hi[c];
-  }
-  const double dx = double(r.hi[0]) - r.lo[0], dy = double(r.hi[1]) - r.lo[1], dz = double(r.hi[2]) - r.lo[2];
-  out.diag = std::sqrt(dx * dx + dy * dy + dz * dz);
-
-  PackedMesh tmp; // only to reuse compute_stack_need
-  tmp.K = K;
-  auto collapse = [&](auto tag) {
-    constexpr int KK = decltype(tag)::value;
-    Collapser<KK> col(sb.nodes);
-    if (r.left < 0) col.emit({root}); // single leaf: root with one leaf child
-    else col.emit({r.left, r.right});
-    out.num_nodes  = col.out.size();
-    out.num_leaves = col.leaves;
-    out.nodes.resize(col.out.size() * sizeof(Node<KK>));
-    std::memcpy(out.nodes.data(), col.out.data(), out.nodes.size());
-  };
-  if (K == 4) collapse(std::integral_constant<int, 4>{});
-  else collapse(std::integral_constant<int, 8>{});
-  tmp.nodes      = out.nodes;
-  tmp.num_nodes  = out.num_nodes;
-  out.stack_need = compute_stack_need(tmp);
+  collapse_binary(sb.nodes, root, K, out);
   out.order      = std::move(sb.idx);
   return true;
 }

@@ -23,6 +23,18 @@ struct PseudoNormals
 bool compute_pseudo_normals(const float* xyz, std::size_t nv, const uint32_t* tri, std::size_t nt,
                             PseudoNormals& out, std::string& err);
 
+// A polygon mesh (the reference's DCEL MeshSDF input: planar simple polygons, faces given as
+// offsets[nf + 1] into idx) as triangles carrying the polygon mesh's own pseudo-normals: each
+// face is clipped into ears in the plane of its two non-dominant normal axes; every triangle has
+// its face's normal, the triangle edges on the face boundary the face half-edge's normal and the
+// edges inside the face the face normal, the vertices their angle-weighted normals over the
+// polygons. The distance to a face is the minimum over its triangles, and each closest feature
+// carries the normal the reference gives it, so the signed distance is the reference's. Faces
+// that are degenerate, repeat a vertex, leave their plane by more than 1e-5 of their extent or
+// cannot be clipped (self-intersecting) are refused.
+bool polygon_mesh(const float* xyz, std::size_t nv, const uint32_t* offsets, const uint32_t* idx, std::size_t nf,
+                  std::vector<uint32_t>& tri, std::vector<uint32_t>& tri_face, PseudoNormals& pn, std::string& err);
+
 // Packed, upload-ready mesh.
 struct PackedMesh
 {
@@ -76,6 +88,20 @@ struct WideBvh
 };
 bool build_wide_bvh(const float* blo, const float* bhi, std::size_t n, int K, int leaf_size, WideBvh& out,
                     std::string& err);
+
+// One node of a binary BVH (the SAH builders' output before it is collapsed to K-wide).
+struct BinNode
+{
+  float    lo[3], hi[3];
+  int32_t  left, right; // -1 for a leaf
+  uint32_t first, count; // range of the primitive order array
+};
+static_assert(sizeof(BinNode) == 40, "BinNode layout (also written by the GPU builder)");
+
+// Collapse a binary BVH into K-wide Node<K> records (each node takes the largest-area interior
+// child's children until it has K), DFS pre-order from the root; fills nodes, counts, root box,
+// diagonal and stack need of out (not its order).
+void collapse_binary(const std::vector<BinNode>& bn, int32_t root, int K, WideBvh& out);
 
 // 64-bit FNV-1a over the bytes of the vertex and triangle arrays (identifies the mesh a trained
 // network belongs to).

@@ -14,6 +14,16 @@ __global__ void fill_u32_kernel(uint32_t* p, uint64_t n, uint32_t v)
   if (i < n) p[i] = v;
 }
 
+// v[i] <- map[v[i]] for entries inside the map (kInvalid and other out-of-map values kept)
+__global__ void map_u32_kernel(uint32_t* v, uint64_t n, const uint32_t* map, uint32_t nmap)
+{
+  const uint64_t i = (uint64_t)blockIdx.x * 256 + threadIdx.x;
+  if (i < n) {
+    const uint32_t x = v[i];
+    if (x < nmap) v[i] = map[x];
+  }
+}
+
 __global__ void add_counts_kernel(unsigned long long* work, const uint32_t* a, const uint32_t* b)
 {
   if (threadIdx.x == 0) {
@@ -79,6 +89,13 @@ hipError_t launch_fill_u32(uint32_t* p, uint64_t n, uint32_t v, hipStream_t s)
 {
   if (n == 0) return hipSuccess;
   hipLaunchKernelGGL(fill_u32_kernel, dim3((uint32_t)((n + 255) / 256)), dim3(256), 0, s, p, n, v);
+  return hipGetLastError();
+}
+
+hipError_t launch_map_u32(uint32_t* v, uint64_t n, const uint32_t* map, uint32_t nmap, hipStream_t s)
+{
+  if (n == 0) return hipSuccess;
+  hipLaunchKernelGGL(map_u32_kernel, dim3((uint32_t)((n + 255) / 256)), dim3(256), 0, s, v, n, map, nmap);
   return hipGetLastError();
 }
 

@@ -27,6 +27,7 @@ class Build(IntEnum):
     """BVH build strategy (the reference's BVH::Build, Source/EBGeometry_BVH.hpp:47-55)."""
     SAH = 0
     Morton = 1
+    SAH_GPU = 2  # the SAH build on the device (same tree), normals and records too
 
 
 def _torch():
@@ -240,6 +241,74 @@ class TriMeshSDF(SignedDistanceFunction):
         if a.size != int(lib().ebg_mlp_param_count()):
             raise ValueError("wrong parameter count")
         check(lib().ebg_mlp_set_params(self._h, a.ctypes.data))
+
+
+def flatten_faces(faces):
+    """Polygon faces (a sequence of vertex-index sequences, or an (nf, k) array) as the C ABI's
+    (offsets uint32 (nf + 1), vertex indices uint32)."""
+    if isinstance(faces, np.ndarray) and faces.ndim == 2:
+        nf, k = faces.shape
+        return (np.arange(nf + 1, dtype=np.uint32) * k, np.ascontiguousarray(faces, dtype=np.uint32).reshape(-1))
+    sizes = np.array([len(f) for f in faces], dtype=np.int64)
+    off = np.zeros(len(sizes) + 1, dtype=np.uint32)
+    off[1:] = np.cumsum(sizes)
+    idx = np.ascontiguousarray(np.concatenate([np.asarray(f, dtype=np.uint32).reshape(-1) for f in faces]),
+                               dtype=np.uint32)
+    return off, idx
+
+
+class MeshSDF(TriMeshSDF):
+    """GPU signed distance to a mesh of planar polygon faces (drop-in for the reference's
+    MeshSDF<T, Meta, K>, Source/EBGeometry_MeshDistanceFunctions.hpp:149-281).
+
+    vertices: (nv, 3); faces: a sequence of vertex-index lists (each a planar, simple polygon,
+    outward oriented), or an (nf, k) array. The faces are clipped into triangles that carry the
+    polygon mesh's own pseudo-normals, so the signed distance is the reference's
+    FaceT::signedDistance minimum; every query of TriMeshSDF is available, and the closest-face
+    outputs (``closest_face``, ``query(..., with_triangle=True)``) hold polygon face indices.
+    """
+
+    def __init__(self, vertices, faces, build: Build = Build.SAH, branching: int = 4, leaf_size: int = 4):
+        v = np.ascontiguousarray(vertices, dtype=np.float32).reshape(-1, 3)
+        off, idx = flatten_faces(faces)
+        opts = BuildOpts(int(build), int(branching), int(leaf_size), 0)
+        h = C.c_void_p()
+        check(lib().ebg_mesh_create_polygons(v.ctypes.data, v.shape[0], off.ctypes.data, idx.ctypes.data,
+                                             len(off) - 1, C.byref(opts), C.byref(h)))
+        self._h = h
+        self.num_vertices = v.shape[0]
+        self.num_faces = len(off) - 1
+        self.num_triangles = int(off[-1]) - 2 * self.num_faces
+
+    def closest_face(self, points, **kw):
+        """(signed distance, index of the closest face): getClosestFaces(p, true)'s first entry
+        (MeshDistanceFunctionsImplem.hpp:297-354), whose other entries are the faces its walk
+        happened to improve on and are not a property of the mesh."""
+        return self.query(points, with_triangle=True, **kw)
+
+    def triangle_faces(self) -> np.ndarray:
+        """The face each clipped triangle belongs to."""
+        out = np.empty(self.num_triangles, dtype=np.uint32)
+        check(lib().ebg_mesh_triangle_faces(self._h, out.ctypes.data))
+        return out
+
+
+def polygons_to_triangles(vertices, faces):
+    """The host clipping of polygon faces into triangles: (tri (nt, 3), tri_face (nt,), face
+    normals (nt, 3), vertex normals (nt, 3, 3), edge normals (nt, 3, 3)), FP64 normals."""
+    v = np.ascontiguousarray(vertices, dtype=np.float32).reshape(-1, 3)
+    off, idx = flatten_faces(faces)
+    nf = len(off) - 1
+    nt = int(off[-1]) - 2 * nf
+    tri = np.zeros((nt, 3), dtype=np.uint32)
+    tf = np.zeros(nt, dtype=np.uint32)
+    fn = np.zeros((nt, 3))
+    vn = np.zeros((nt, 3, 3))
+    en = np.zeros((nt, 3, 3))
+    check(lib().ebg_polygons_to_triangles_host(v.ctypes.data, v.shape[0], off.ctypes.data, idx.ctypes.data, nf,
+                                               tri.ctypes.data, tf.ctypes.data, fn.ctypes.data, vn.ctypes.data,
+                                               en.ctypes.data))
+    return tri, tf, fn, vn, en
 
 
 def pseudo_normals(vertices, faces):

@@ -61,10 +61,14 @@ const char* ebg_last_error(void);
 enum ebg_builder
 {
   EBG_BUILD_SAH    = 0, /* binned SAH on the host (default; best traversal quality)            */
-  EBG_BUILD_MORTON = 1  /* Morton codes + radix sort + balanced K-ary tree on the GPU
+  EBG_BUILD_MORTON = 1, /* Morton codes + radix sort + balanced K-ary tree on the GPU
                            (the reference's Build::Morton, Source/EBGeometry_BVHImplem.hpp:207-312);
                            leaf_size 0 or 1 gives the reference's depth floor(log n / log K),
                            larger values fewer, fuller leaves (at most 16 triangles each)     */
+  EBG_BUILD_SAH_GPU = 2 /* the SAH build of EBG_BUILD_SAH (Build::SAH, Source/EBGeometry_BVH.hpp:
+                           521-641) on the GPU -- the same splits, hence the same tree -- with
+                           the pseudo-normals (MeshT::reconcile, DCEL_MeshImplem.hpp:304-309) and
+                           the packed triangle records also computed on the device            */
 };
 
 typedef struct ebg_build_opts
@@ -89,6 +93,20 @@ typedef struct ebg_mesh_info
  * oriented 2-manifold (like the reference's DCEL meshes). Host pointers; data is copied. */
 ebg_status ebg_mesh_create(const float* xyz, uint64_t num_vertices, const uint32_t* tri,
                            uint64_t num_triangles, const ebg_build_opts* opts, ebg_mesh** out);
+/* Polygon meshes: MeshSDF<T, Meta, K>(DCEL mesh, BVH::Build) over general polygon faces
+ * (Source/EBGeometry_MeshDistanceFunctions.hpp:149-281, built from a polygon soup by
+ * Soup::soupToDCEL, Source/EBGeometry_SoupImplem.hpp:148-226). Face f is the vertex list
+ * face_vertices[face_offsets[f] .. face_offsets[f+1]) (face_offsets has num_faces+1 entries,
+ * starting at 0); faces must be planar (corners within 1e-5 of the face's extent of its plane) and
+ * simple (not self-intersecting), else EBG_INVALID_ARG. The handle answers every mesh query: the
+ * signed distance is FaceT::signedDistance's minimum by magnitude (DCEL_FaceImplem.hpp:388-423),
+ * and closest-face outputs (d_tri) hold face indices. num_triangles in ebg_mesh_info counts the
+ * triangles the faces are clipped into. */
+ebg_status ebg_mesh_create_polygons(const float* xyz, uint64_t num_vertices, const uint32_t* face_offsets,
+                                    const uint32_t* face_vertices, uint64_t num_faces, const ebg_build_opts* opts,
+                                    ebg_mesh** out);
+/* The face each of the handle's num_triangles triangles belongs to (identity for triangle meshes). */
+ebg_status ebg_mesh_triangle_faces(const ebg_mesh* mesh, uint32_t* tri_face);
 ebg_status ebg_mesh_destroy(ebg_mesh* mesh);
 ebg_status ebg_mesh_get_info(const ebg_mesh* mesh, ebg_mesh_info* info);
 /* Original face index of each triangle in BVH leaf order (num_triangles uint32, host). For the
@@ -103,6 +121,16 @@ ebg_status ebg_mesh_leaf_order(const ebg_mesh* mesh, uint32_t* faces);
 ebg_status ebg_pseudo_normals_host(const float* xyz, uint64_t num_vertices, const uint32_t* tri,
                                    uint64_t num_triangles, double* face_n, double* vert_n,
                                    double* edge_n);
+
+/* Host-only (no GPU needed): the triangles ebg_mesh_create_polygons clips the faces into (each
+ * face of n corners gives n-2; tri: 3 per triangle, tri_face: its face) and their pseudo-normals in
+ * the layout of ebg_pseudo_normals_host: the face's normal, the polygon mesh's vertex normals, and
+ * per triangle edge the face half-edge's normal, or the face normal for an edge inside the face.
+ * Counterpart of the normals MeshT::reconcile gives a polygon mesh
+ * (Source/EBGeometry_DCEL_MeshImplem.hpp:304-309). */
+ebg_status ebg_polygons_to_triangles_host(const float* xyz, uint64_t num_vertices, const uint32_t* face_offsets,
+                                         const uint32_t* face_vertices, uint64_t num_faces, uint32_t* tri,
+                                         uint32_t* tri_face, double* face_n, double* vert_n, double* edge_n);
 
 /* ---------------------------------------------------------------------------------------- */
 /* Queries -- batched TriMeshSDF::signedDistance / getClosestTriangle                         */

@@ -6,6 +6,9 @@
  *                             54-62) backed by an ebg_mesh: drops in wherever the reference's
  *                             TriMeshSDF<T, Meta, K, W> (Source/EBGeometry_MeshDistanceFunctions.hpp:
  *                             376-387) is held as a std::shared_ptr<ImplicitFunction<T>>.
+ *   ebgpu::GpuMeshSDF<T>      the same over a mesh of planar polygon faces: the reference's
+ *                             MeshSDF<T, Meta, K> (Source/EBGeometry_MeshDistanceFunctions.hpp:
+ *                             149-281); closestTriangles gives face indices.
  *   ebgpu::GpuUnionIF<T>      an ImplicitFunction<T> backed by an ebg_scene (BVHUnionIF,
  *                             Source/EBGeometry_CSG.hpp:401-440).
  *
@@ -124,8 +127,42 @@ public:
     return m_mesh.get();
   }
 
+protected:
+  explicit GpuTriMeshSDF(ebg_mesh* m) { m_mesh.reset(m, [](ebg_mesh* p) { (void)ebg_mesh_destroy(p); }); }
+
 private:
   std::shared_ptr<ebg_mesh> m_mesh;
+};
+
+template <class T>
+class GpuMeshSDF : public GpuTriMeshSDF<T>
+{
+public:
+  using Vec3 = EBGeometry::Vec3T<T>;
+
+  // vertices and polygon faces (vertex-index lists, what Soup::soupToDCEL takes): planar, simple,
+  // outward oriented, closed
+  GpuMeshSDF(const std::vector<Vec3>& vertices, const std::vector<std::vector<std::size_t>>& faces,
+             int branching = 4, int leaf_size = 8)
+    : GpuTriMeshSDF<T>(create(vertices, faces, branching, leaf_size))
+  {}
+
+private:
+  static ebg_mesh*
+  create(const std::vector<Vec3>& vertices, const std::vector<std::vector<std::size_t>>& faces, int branching,
+         int leaf_size)
+  {
+    std::vector<float>    xyz = detail::to_f32<T>(vertices.data(), vertices.size());
+    std::vector<uint32_t> off{0u}, idx;
+    for (const auto& f : faces) {
+      for (std::size_t v : f) idx.push_back(uint32_t(v));
+      off.push_back(uint32_t(idx.size()));
+    }
+    ebg_build_opts o{EBG_BUILD_SAH, branching, leaf_size, 0};
+    ebg_mesh*      m = nullptr;
+    detail::check(ebg_mesh_create_polygons(xyz.data(), vertices.size(), off.data(), idx.data(), faces.size(), &o, &m));
+    return m;
+  }
 };
 
 template <class T>

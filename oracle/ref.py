@@ -352,3 +352,43 @@ def cloud_knn(points, queries, k, self_query=False, variant: str = "portable", n
     L.ebgref_cloud_knn(_p(P, _d), P.shape[0], _p(Q, _d), n, int(k), int(bool(self_query)),
                        idx.ctypes.data_as(C.POINTER(C.c_int64)), _p(d2, _d), nthreads or os.cpu_count() or 1)
     return idx, d2
+
+
+def flatten_faces(faces):
+    """Polygon faces (a list of vertex-index lists) as (offsets uint32 (nf + 1), indices uint32)."""
+    sizes = np.array([len(f) for f in faces], dtype=np.int64)
+    offsets = np.zeros(len(faces) + 1, dtype=np.uint32)
+    offsets[1:] = np.cumsum(sizes)
+    idx = np.ascontiguousarray(np.concatenate([np.asarray(f, dtype=np.uint32) for f in faces]), dtype=np.uint32)
+    return offsets, idx
+
+
+class PolyMesh:
+    """Reference MeshSDF<double, uint32_t, K>(DCEL mesh, SAH) over polygon faces via soupToDCEL."""
+
+    def __init__(self, verts, faces, variant: str = "portable"):
+        self.L = lib(variant)
+        if not hasattr(self.L, "_poly_typed"):
+            self.L.ebgref_polymesh_create.restype = C.c_void_p
+            self.L.ebgref_polymesh_create.argtypes = [_d, C.c_int64, _u32, _u32, C.c_int64]
+            self.L.ebgref_polymesh_destroy.argtypes = [C.c_void_p]
+            self.L.ebgref_polymesh_query.argtypes = [C.c_void_p, _d, C.c_int64, _d, _u32, _d, C.c_int]
+            self.L._poly_typed = True
+        v = _f64(verts, (-1, 3))
+        off, idx = flatten_faces(faces)
+        self._h = self.L.ebgref_polymesh_create(_p(v, _d), v.shape[0], _p(off, _u32), _p(idx, _u32), len(faces))
+
+    def __del__(self):
+        if getattr(self, "_h", None):
+            self.L.ebgref_polymesh_destroy(self._h)
+            self._h = None
+
+    def query(self, pts, nthreads: int | None = None):
+        """(signed distance, closest face index, its unsigned distance) per point."""
+        p = _f64(pts, (-1, 3))
+        n = p.shape[0]
+        d, dist = np.zeros(n), np.zeros(n)
+        face = np.zeros(n, dtype=np.uint32)
+        self.L.ebgref_polymesh_query(self._h, _p(p, _d), n, _p(d, _d), _p(face, _u32), _p(dist, _d),
+                                     nthreads or os.cpu_count() or 1)
+        return d, face, dist

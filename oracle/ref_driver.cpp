@@ -21,6 +21,10 @@
 //                   (Source/EBGeometry_CSGImplem.hpp:466-501, TransformImplem.hpp:140-270).
 //   * point clouds  PointCloudBVH<double>: closestPoints (k nearest of external points) and
 //                   allNearestNeighbors (Source/EBGeometry_PointCloudBVHImplem.hpp:235-507).
+//   * polygon mesh  MeshSDF<double, uint32_t, K>(mesh, BVH::Build::SAH) over a DCEL mesh of
+//                   general polygons built by Soup::soupToDCEL (Source/EBGeometry_
+//                   MeshDistanceFunctionsImplem.hpp:250-354): signedDistance and the first
+//                   entry of getClosestFaces(p, true) (face index in the metadata).
 //   * expressions   any tree of the reference's analytic SDFs, transforms and CSG operators,
 //                   given in post order as constructor calls (ebgref_expr_value below), built
 //                   with the reference's own constructors and factories and evaluated by value().
@@ -53,6 +57,14 @@ using Vec3   = Vec3T<double>;
 using AABB   = BoundingVolumes::AABBT<double>;
 using IF     = ImplicitFunction<double>;
 using RefUnion = BVHUnionIF<double, IF, AABB, K>;
+
+using PolySDF = MeshSDF<double, uint32_t, K>;
+
+struct RefPolyMesh
+{
+  std::shared_ptr<Mesh>    mesh;
+  std::shared_ptr<PolySDF> sdf;
+};
 
 struct RefMesh
 {
@@ -111,6 +123,46 @@ ebgref_mesh_create(const double* xyz, int64_t nv, const uint32_t* tri, int64_t n
   for (size_t i = 0; i < faces.size(); i++) faces[i]->setMetaData(uint32_t(i));
   m->sdf = std::make_shared<SDF>(m->mesh, BVH::Build::SAH, size_t(max_leaf_groups));
   return m;
+}
+
+void*
+ebgref_polymesh_create(const double* xyz, int64_t nv, const uint32_t* offsets, const uint32_t* idx, int64_t nf)
+{
+  std::vector<Vec3>                vertices(nv);
+  std::vector<std::vector<size_t>> facets(nf);
+  for (int64_t i = 0; i < nv; i++) vertices[i] = Vec3(xyz[3 * i], xyz[3 * i + 1], xyz[3 * i + 2]);
+  for (int64_t f = 0; f < nf; f++)
+    for (uint32_t k = offsets[f]; k < offsets[f + 1]; k++) facets[f].push_back(idx[k]);
+  auto* m = new RefPolyMesh;
+  m->mesh = std::make_shared<Mesh>();
+  Soup::soupToDCEL(*m->mesh, vertices, facets, "ebgref");
+  auto& faces = m->mesh->getFaces();
+  for (size_t i = 0; i < faces.size(); i++) faces[i]->setMetaData(uint32_t(i));
+  m->sdf = std::make_shared<PolySDF>(m->mesh, BVH::Build::SAH);
+  return m;
+}
+
+void
+ebgref_polymesh_destroy(void* h)
+{
+  delete static_cast<RefPolyMesh*>(h);
+}
+
+// Signed distance per point; face[i] = metadata of the first face of getClosestFaces(p, true),
+// dist[i] its unsigned distance.
+void
+ebgref_polymesh_query(void* h, const double* pts, int64_t n, double* d, uint32_t* face, double* dist, int nthreads)
+{
+  const PolySDF& sdf = *static_cast<RefPolyMesh*>(h)->sdf;
+  parallel_for(n, nthreads, [&](int64_t b, int64_t e) {
+    for (int64_t i = b; i < e; i++) {
+      const Vec3 p(pts[3 * i], pts[3 * i + 1], pts[3 * i + 2]);
+      d[i]         = sdf.signedDistance(p);
+      const auto c = sdf.getClosestFaces(p, true);
+      face[i]      = c.empty() ? 0xFFFFFFFFu : c.front().first->getMetaData();
+      dist[i]      = c.empty() ? std::numeric_limits<double>::infinity() : c.front().second;
+    }
+  });
 }
 
 void

@@ -40,9 +40,16 @@ int main()
       std::make_shared<ebgpu::GpuTriMeshSDF<float>>(std::vector<EBGeometry::Vec3T<float>>{
         EBGeometry::Vec3T<float>(0, 0, 0), EBGeometry::Vec3T<float>(1, 0, 0), EBGeometry::Vec3T<float>(0, 1, 0),
         EBGeometry::Vec3T<float>(0, 0, 1)}, t);
-    const double d = f->value(V(2, 2, 2));
-    std::printf("device %.6f %.6f\n", d, double(g->value(EBGeometry::Vec3T<float>(2, 2, 2))));
-    return std::isfinite(d) ? 0 : 4;
+    // the unit cube with quad faces, as MeshSDF<double, Meta, K> takes it
+    std::vector<V> cv;
+    for (int i = 0; i < 8; i++) cv.push_back(V(i >> 2 & 1, i >> 1 & 1, i & 1));
+    std::vector<std::vector<size_t>> quads = {{0, 1, 3, 2}, {4, 6, 7, 5}, {0, 4, 5, 1},
+                                              {2, 3, 7, 6}, {0, 2, 6, 4}, {1, 5, 7, 3}};
+    std::shared_ptr<EBGeometry::SignedDistanceFunction<double>> c =
+      std::make_shared<ebgpu::GpuMeshSDF<double>>(cv, quads);
+    const double d = f->value(V(2, 2, 2)), dc = c->signedDistance(V(0.5, 0.5, 0.25));
+    std::printf("device %.6f %.6f %.6f\n", d, double(g->value(EBGeometry::Vec3T<float>(2, 2, 2))), dc);
+    return std::isfinite(d) && std::abs(dc + 0.25) < 1e-6 ? 0 : 4;
   }
   catch (const std::runtime_error& e) {
     std::printf("no device: %s\n", e.what());
