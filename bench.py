@@ -67,6 +67,8 @@ def parse():
     ap.add_argument("--batched", action="store_true",
                     help="instead: the many-mesh workload (256 displaced meshes L3-L7 under affine maps, seed 3, "
                          "10k-1M queries each) through one captured batch plan")
+    ap.add_argument("--valu-calibration", action="store_true",
+                    help="run only the FP32 FMA kernel that sets the VALU issue ceiling (for rocprofv3 --pmc)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a 2 GiB device copy (to scale rocprofv3 FETCH_SIZE / WRITE_SIZE)")
     return ap.parse_args()
@@ -103,6 +105,23 @@ def profiled_traffic(name):
         if line.startswith(key):
             return int(float(line.split(":", 1)[1]))
     return None
+
+
+def valu_calibration(iters=2048, launches=4):
+    """A kernel of independent FP32 FMA chains (libebgcalib.so) that keeps every SIMD's VALU busy:
+    its VALU wavefront-instructions per SIMD per cycle are the issue ceiling the traversal
+    kernels' rocprofv3 VALU counts are set against."""
+    import ctypes as C
+    L = C.CDLL(os.path.join(ROOT, "ebgeometry_amd", "libebgcalib.so"))
+    L.ebgcal_valu_fma.argtypes = [C.c_int, C.c_int, C.POINTER(C.c_float), C.POINTER(C.c_double)]
+    ms = (C.c_float * launches)()
+    w = C.c_double()
+    if L.ebgcal_valu_fma(iters, launches, ms, C.byref(w)) != 0:
+        raise RuntimeError("VALU calibration kernel failed")
+    best = min(ms)
+    print(json.dumps({"valu_fma_wave_instructions_per_launch": w.value, "launch_ms": [round(x, 4) for x in ms],
+                      "per_simd_per_cycle_at_2400MHz": round(w.value / (best * 1e-3) / 1024 / 2.4e9, 4)}),
+          flush=True)
 
 
 def copy_calibration(gib=2):
@@ -432,6 +451,9 @@ def main():
 
     if a.copy_calibration:
         copy_calibration()
+        return
+    if a.valu_calibration:
+        valu_calibration()
         return
     if a.batched:
         print(json.dumps(batched_bench(a)), flush=True)

@@ -3,7 +3,9 @@
     python -m ebgeometry_amd.build            # incremental
     python -m ebgeometry_amd.build --force    # rebuild everything
 
-Objects go to ebgeometry_amd/_build/, the library to ebgeometry_amd/libebgpu.so.
+Objects go to ebgeometry_amd/_build/, the library to ebgeometry_amd/libebgpu.so; the VALU
+calibration kernel bench.py uses to price the traversal's instruction issue goes to
+ebgeometry_amd/libebgcalib.so.
 """
 from __future__ import annotations
 
@@ -19,6 +21,9 @@ ROOT = os.path.dirname(HERE)
 CSRC = os.path.join(HERE, "csrc")
 BUILD = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libebgpu.so")
+# measurement aid (bench.py --valu-calibration), not linked into the query library
+CALIB_SRC = os.path.join(CSRC, "calib", "valu_calib.hip")
+CALIB_LIB = os.path.join(HERE, "libebgcalib.so")
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = os.environ.get("EBGPU_ARCH", "gfx950")
@@ -86,6 +91,11 @@ def build(force: bool = False, verbose: bool = False) -> str:
         if r.returncode != 0:
             raise RuntimeError(f"link failed:\n$ {' '.join(cmd)}\n{r.stdout}\n{r.stderr}")
         shutil.move(tmp, LIB)
+    if force or _mtime(CALIB_LIB) < _mtime(CALIB_SRC):
+        cmd = [HIPCC] + COMMON + DEVICE + ["-shared", CALIB_SRC, "-o", CALIB_LIB]
+        r = subprocess.run(cmd, capture_output=True, text=True)
+        if r.returncode != 0:
+            raise RuntimeError(f"calibration build failed:\n$ {' '.join(cmd)}\n{r.stdout}\n{r.stderr}")
     if verbose:
         print("built", LIB)
     return LIB

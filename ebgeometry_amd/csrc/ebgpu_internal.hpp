@@ -64,7 +64,7 @@ struct alignas(64) TriRec
   float    e1[3];  // v2 - v1   (v32)
   float    e2[3];  // v0 - v2   (v13)
   float    n[3];   // unit face normal (x2-x0)x(x2-x1)/|.|
-  float    c0[3];  // e0 x n: sign of c0.(p-v0) tests the face region
+  float    c0[3];  // +-e0 x n, pointing out of the triangle: sign of c0.(p-v0) tests the face region
   float    c1[3];  // e1 x n
   float    c2[3];  // e2 x n
   float    inv[3]; // 1/|e_i|^2

@@ -736,9 +736,13 @@ __global__ __launch_bounds__(256) void pack_kernel(const float* __restrict__ xyz
   put(r.e1, e[1]);
   put(r.e2, e[2]);
   put(r.n, n);
-  put(r.c0, dcross(e[0], n));
-  put(r.c1, dcross(e[1], n));
-  put(r.c2, dcross(e[2], n));
+  // edge normals in the face plane, pointing out of the triangle
+  const Dv3    w  = dcross(e[0], e[1]);
+  const double sg = w.x * n.x + w.y * n.y + w.z * n.z >= 0.0 ? 1.0 : -1.0;
+  auto         sc = [sg](Dv3 a) { return Dv3{sg * a.x, sg * a.y, sg * a.z}; };
+  put(r.c0, sc(dcross(e[0], n)));
+  put(r.c1, sc(dcross(e[1], n)));
+  put(r.c2, sc(dcross(e[2], n)));
   for (int c = 0; c < 3; c++) r.inv[c] = float(1.0 / (e[c].x * e[c].x + e[c].y * e[c].y + e[c].z * e[c].z));
   r.face      = f;
   const Dv3 cen = {double(float((x0.x + x1.x + x2.x) / 3.0)), double(float((x0.y + x1.y + x2.y) / 3.0)),

@@ -394,9 +394,12 @@ void pack_triangles(const float* xyz, const uint32_t* tri, const std::vector<uin
     put(r.e1, e[1]);
     put(r.e2, e[2]);
     put(r.n, n);
-    put(r.c0, cross(e[0], n));
-    put(r.c1, cross(e[1], n));
-    put(r.c2, cross(e[2], n));
+    // edge normals in the face plane, pointing out of the triangle (the face normal of a polygon
+    // may point against a clipped triangle's winding)
+    const double sg = dot(cross(e[0], e[1]), n) >= 0.0 ? 1.0 : -1.0;
+    put(r.c0, sg * cross(e[0], n));
+    put(r.c1, sg * cross(e[1], n));
+    put(r.c2, sg * cross(e[2], n));
     for (int c = 0; c < 3; c++) r.inv[c] = float(1.0 / dot(e[c], e[c]));
     r.face = f;
     // bounding sphere about the (float-rounded) centroid; the radius is measured in double from

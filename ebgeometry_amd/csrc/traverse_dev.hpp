@@ -246,6 +246,39 @@ __device__ __forceinline__ void tri_eval32(const TriRec& r, float px, float py, 
   }
 }
 
+// tri_eval32<false> behind a wavefront-level cull: d^2 = fd^2 + (in-plane distance)^2, and the
+// in-plane distance is at least the distance past any edge line the projected point lies beyond
+// (c_i points out of the triangle, |c_i| = |e_i|), so lb = fd^2 + max_i max(c_i.p_i, 0)^2 / |e_i|^2
+// <= d^2. When no lane's lb reaches its bound the wavefront skips the rest of the evaluation
+// (returns false); otherwise d2 is tri_eval32's, operation for operation. The margin 2^-16
+// keeps the cull from acting on rounding-level differences of lb and d2.
+__device__ __forceinline__ bool tri_eval32_culled(const TriRec& r, float px, float py, float pz, float bound, float& d2)
+{
+  const float p1x = px - r.v0[0], p1y = py - r.v0[1], p1z = pz - r.v0[2];
+  const float p2x = p1x - r.e0[0], p2y = p1y - r.e0[1], p2z = p1z - r.e0[2];
+  const float p3x = p1x + r.e2[0], p3y = p1y + r.e2[1], p3z = p1z + r.e2[2];
+  const float s0  = r.c0[0] * p1x + r.c0[1] * p1y + r.c0[2] * p1z;
+  const float s1  = r.c1[0] * p2x + r.c1[1] * p2y + r.c1[2] * p2z;
+  const float s2  = r.c2[0] * p3x + r.c2[1] * p3y + r.c2[2] * p3z;
+  const float fd  = r.n[0] * p1x + r.n[1] * p1y + r.n[2] * p1z;
+  const float o0 = fmaxf(s0, 0.0f), o1 = fmaxf(s1, 0.0f), o2 = fmaxf(s2, 0.0f);
+  const float out = fmaxf(fmaxf(o0 * o0 * r.inv[0], o1 * o1 * r.inv[1]), o2 * o2 * r.inv[2]);
+  const float lb  = fmaf(fd, fd, out) * (1.0f - 0x1p-16f);
+  if (ballot(lb <= bound) == 0) return false;
+  const bool  inface = ((s0 > 0.0f) == (s1 > 0.0f)) && ((s1 > 0.0f) == (s2 > 0.0f));
+  const float t0  = __builtin_amdgcn_fmed3f((p1x * r.e0[0] + p1y * r.e0[1] + p1z * r.e0[2]) * r.inv[0], 0.0f, 1.0f);
+  const float t1  = __builtin_amdgcn_fmed3f((p2x * r.e1[0] + p2y * r.e1[1] + p2z * r.e1[2]) * r.inv[1], 0.0f, 1.0f);
+  const float t2  = __builtin_amdgcn_fmed3f((p3x * r.e2[0] + p3y * r.e2[1] + p3z * r.e2[2]) * r.inv[2], 0.0f, 1.0f);
+  const float y0x = p1x - t0 * r.e0[0], y0y = p1y - t0 * r.e0[1], y0z = p1z - t0 * r.e0[2];
+  const float y1x = p2x - t1 * r.e1[0], y1y = p2y - t1 * r.e1[1], y1z = p2z - t1 * r.e1[2];
+  const float y2x = p3x - t2 * r.e2[0], y2y = p3y - t2 * r.e2[1], y2z = p3z - t2 * r.e2[2];
+  const float q0  = y0x * y0x + y0y * y0y + y0z * y0z;
+  const float q1  = y1x * y1x + y1y * y1y + y1z * y1z;
+  const float q2  = y2x * y2x + y2y * y2y + y2z * y2z;
+  d2              = inface ? fd * fd : fminf(fminf(q0, q1), q2);
+  return true;
+}
+
 // Signed distance of the chosen feature (FP32): face -> n.(p-v0); vertex/edge ->
 // sqrt(d2) * sgn(pseudo-normal . (p - closest point)), sgn(x) = x > 0 ? 1 : -1.
 __device__ __forceinline__ float resolve32(const MeshDev& m, uint32_t t, uint32_t feat, float d2, float px, float py, float pz)
@@ -579,12 +612,16 @@ __device__ __forceinline__ void traverse_from(const MeshDev& m, const Entry& sta
           uint32_t f = 0;
           if constexpr (sizeof(R) == 4) {
             const TriRec rec = sload_record(m.tris + t);
-            // the distance to the triangle's plane is a lower bound on the distance to it
-            const float  fd  = rec.n[0] * (px - rec.v0[0]) + rec.n[1] * (py - rec.v0[1]) + rec.n[2] * (pz - rec.v0[2]);
-            if (ballot(fd * fd <= bound) == 0) continue;
-            if (COUNT) w_tris++;
-            tri_eval32<false>(rec, px, py, pz, d2, f);
-            if constexpr (FACE) {
+            if constexpr (!FACE) {
+              if (!tri_eval32_culled(rec, px, py, pz, bound, d2)) continue;
+              if (COUNT) w_tris++;
+            }
+            else {
+              // the distance to the triangle's plane is a lower bound on the distance to it
+              const float fd = rec.n[0] * (px - rec.v0[0]) + rec.n[1] * (py - rec.v0[1]) + rec.n[2] * (pz - rec.v0[2]);
+              if (ballot(fd * fd <= bound) == 0) continue;
+              if (COUNT) w_tris++;
+              tri_eval32<false>(rec, px, py, pz, d2, f);
               if (d2 <= fc->lim) {
                 fc->insert(d2, t);
                 if (d2 < best) {
