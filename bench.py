@@ -403,7 +403,7 @@ def batched_bench(a):
     dt, replay_ms = timed(plan.launch, a.steps, a.warmup, plan.last_traversal_ms)
     # the traversal kernel alone: the same items and workgroup through a plan without a graph
     # (events inside a graph are not timed), timed the same way
-    eager = b.plan(items, capture=False, workgroup=plan.workgroup)
+    eager = b.plan(items, capture=False, workgroup=plan.workgroup, persistent=plan.persistent)
     _, kms = timed(eager.launch, a.steps, a.warmup, eager.last_traversal_ms)
     del eager
     wk = torch.zeros(8, dtype=torch.int64, device="cuda:0")
@@ -432,10 +432,11 @@ def batched_bench(a):
            "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
            "config": {"workload": "batched-256-meshes", "meshes": 256, "faces": nt, "queries": nq, "seed": 3,
                       "branching": K, "leaf_size": info["leaf_size"], "workgroup": plan.workgroup,
+                      "launch": "persistent" if plan.persistent else "one wavefront per packet",
                       "tune_ms": info["tune_ms"]},
            "graph_replay_ms": round(replay_ms, 3),
-           "roofline": packet_roofline(nq, c, K, kms, f"batch32_kernel<{K},{info['stack_capacity'] // 64},false>",
-                                       None),
+           "roofline": packet_roofline(nq, c, K, kms, f"batch32_kernel<{K},{info['stack_capacity'] // 64},false,"
+                                       f"{str(plan.persistent).lower()}>", None),
            "correctness": {"sample": f"{checked} queries (200 strided of every 16th mesh) against the oracle",
                            "max_abs_err_over_diag": worst, "sign_mismatches": signs,
                            "within_1e-5_diag": bool(worst <= 1e-5)}}

@@ -85,8 +85,11 @@ class MeshBatch:
         check(lib().ebg_batch_query(self._h, arr, len(items), C.byref(o)))
         return (outs, tris) if with_triangle else outs
 
-    def plan(self, items, with_triangle: bool = False, workgroup: int = 0, capture: bool = True, **kw) -> "BatchPlan":
-        return BatchPlan(self, items, with_triangle, workgroup, capture, **kw)
+    def plan(self, items, with_triangle: bool = False, workgroup: int = 0, capture: bool = True,
+             persistent: bool = False, **kw) -> "BatchPlan":
+        """workgroup 0 times the launch shapes and keeps the fastest; else 64/128/256 lanes, with
+        persistent=True for the persistent grid whose wavefronts take packets from a queue."""
+        return BatchPlan(self, items, with_triangle, workgroup | (PERSISTENT if persistent else 0), capture, **kw)
 
     def mesh(self, i: int) -> TriMeshSDF:
         """Mesh i of the batch as a TriMeshSDF view (the batch keeps it alive)."""
@@ -99,6 +102,9 @@ class MeshBatch:
         m.num_vertices = self._v[i].shape[0]
         m.num_triangles = self._f[i].shape[0]
         return m
+
+
+PERSISTENT = 0x10000  # EBG_PLAN_PERSISTENT
 
 
 class BatchPlan:
@@ -130,7 +136,13 @@ class BatchPlan:
 
     @property
     def workgroup(self) -> int:
-        return int(lib().ebg_batch_plan_workgroup(self._h))
+        """Lanes per workgroup of the traversal launch."""
+        return int(lib().ebg_batch_plan_workgroup(self._h)) & 0xFFFF
+
+    @property
+    def persistent(self) -> bool:
+        """True when the traversal runs as the persistent grid."""
+        return bool(int(lib().ebg_batch_plan_workgroup(self._h)) & PERSISTENT)
 
     def launch(self, stream=None):
         torch = _torch()

@@ -19,6 +19,8 @@
 
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
+
 namespace ebgpu {
 
 namespace {
@@ -65,31 +67,10 @@ __global__ __launch_bounds__(256) void batch_keys_kernel(const BatchItemDev* __r
   keys[g] = KeyT(((uint64_t)lo << 24) | hilbert_index<8>(x[0], x[1], x[2]));
 }
 
+// One packet: the wavefront's 64 queries of one item against its mesh.
 template <int K, int NB, bool COUNT>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(K == 4 ? 8 : 1))) void batch32_kernel(const BatchArgs a)
+__device__ __forceinline__ void batch_packet(const BatchArgs& a, uint32_t p)
 {
-  const uint32_t wpb = blockDim.x >> 6;
-  // wave-uniform, and known to be: the packet and mesh records below are then scalar loads.
-  // Plain dispatch order (no XCD remap): all XCDs work on the same stretch of the LPT-ordered
-  // packet table, so the meshes in flight at any time are few and stay in the Infinity Cache;
-  // with a contiguous eighth of the table per XCD, eight stretches of meshes were in flight and
-  // the launch took 2.5x as long (163 against 66 ms on the 256-mesh workload).
-  uint32_t w = uni(blockIdx.x * wpb + (threadIdx.x >> 6));
-  uint32_t p;
-  if (a.early_blocks) {
-    // the early packets first (they run longest), then the table in order without them
-    const uint32_t ew = a.early_blocks * wpb;
-    if (w < ew) {
-      if (w >= min(*a.early_count, a.early_cap)) return; // whole wavefront
-      p = uni(a.early_list[w]);
-    }
-    else {
-      p = w - ew;
-      if (p >= a.num_packets || a.flag[p]) return; // whole wavefront
-    }
-  }
-  else p = w;
-  if (p >= a.num_packets) return; // whole wavefront (p is wave-uniform)
   const SWords<4>    pk = sload<4>(a.packets + 4 * (size_t)p);
   const uint32_t     item = pk.w[0], first = pk.w[1];
   const BatchItemDev it   = sload_record(a.items + item);
@@ -130,6 +111,59 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(K == 4 ? 8 
       atomicAdd(a.work + 4, (unsigned long long)w_inner);
       atomicAdd(a.work + 5, (unsigned long long)w_tris);
     }
+  }
+}
+
+// PERSIST = false: one wavefront per packet (the early list first, then the table without the
+// early packets). PERSIST = true: a grid that fits the device at once; each wavefront takes queue
+// entries in the same order until the queue is exhausted (no wavefront waits on another).
+template <int K, int NB, bool COUNT, bool PERSIST>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(K == 4 ? 8 : 1))) void batch32_kernel(const BatchArgs a)
+{
+  if constexpr (PERSIST) {
+    const uint32_t ne   = a.early_blocks ? min(*a.early_count, a.early_cap) : 0u;
+    const uint32_t skip = a.early_blocks ? a.early_cap : 0u; // queue entries before the table
+    for (;;) {
+      uint32_t w = 0;
+      if (__lane_id() == 0) w = atomicAdd(a.queue, 1u);
+      w = uni(rdlane(w, 0));
+      if (w >= a.queue_total) return; // whole wavefront
+      uint32_t p;
+      if (w < skip) {
+        if (w >= ne) continue;
+        p = uni(a.early_list[w]);
+      }
+      else {
+        p = w - skip;
+        if (p >= a.num_packets || (a.early_blocks && a.flag[p])) continue;
+      }
+      batch_packet<K, NB, COUNT>(a, p);
+    }
+  }
+  else {
+    const uint32_t wpb = blockDim.x >> 6;
+    // wave-uniform, and known to be: the packet and mesh records below are then scalar loads.
+    // Plain dispatch order (no XCD remap): all XCDs work on the same stretch of the LPT-ordered
+    // packet table, so the meshes in flight at any time are few and stay in the Infinity Cache;
+    // with a contiguous eighth of the table per XCD, eight stretches of meshes were in flight and
+    // the launch took 2.5x as long (163 against 66 ms on the 256-mesh workload).
+    uint32_t w = uni(blockIdx.x * wpb + (threadIdx.x >> 6));
+    uint32_t p;
+    if (a.early_blocks) {
+      // the early packets first (they run longest), then the table in order without them
+      const uint32_t ew = a.early_blocks * wpb;
+      if (w < ew) {
+        if (w >= min(*a.early_count, a.early_cap)) return; // whole wavefront
+        p = uni(a.early_list[w]);
+      }
+      else {
+        p = w - ew;
+        if (p >= a.num_packets || a.flag[p]) return; // whole wavefront
+      }
+    }
+    else p = w;
+    if (p >= a.num_packets) return; // whole wavefront (p is wave-uniform)
+    batch_packet<K, NB, COUNT>(a, p);
   }
 }
 
@@ -226,12 +260,31 @@ __global__ __launch_bounds__(256) void batch_mark_kernel(const BatchArgs a)
 template <int K, int NB>
 hipError_t launch32(const BatchArgs& a, int wg, bool count, hipStream_t s)
 {
+  if (a.persist_blocks) {
+    if (!a.queue) return hipErrorInvalidValue;
+    if (count) hipLaunchKernelGGL((batch32_kernel<K, NB, true, true>), dim3(a.persist_blocks), dim3(wg), 0, s, a);
+    else hipLaunchKernelGGL((batch32_kernel<K, NB, false, true>), dim3(a.persist_blocks), dim3(wg), 0, s, a);
+    return hipGetLastError();
+  }
   const uint32_t wpb    = (uint32_t)wg / 64u;
   const uint32_t blocks = (a.num_packets + wpb - 1) / wpb + a.early_blocks;
   if (blocks == 0) return hipSuccess;
-  if (count) hipLaunchKernelGGL((batch32_kernel<K, NB, true>), dim3(blocks), dim3(wg), 0, s, a);
-  else hipLaunchKernelGGL((batch32_kernel<K, NB, false>), dim3(blocks), dim3(wg), 0, s, a);
+  if (count) hipLaunchKernelGGL((batch32_kernel<K, NB, true, false>), dim3(blocks), dim3(wg), 0, s, a);
+  else hipLaunchKernelGGL((batch32_kernel<K, NB, false, false>), dim3(blocks), dim3(wg), 0, s, a);
   return hipGetLastError();
+}
+
+template <int K, int NB>
+hipError_t persist_blocks(int wg, bool count, uint32_t* blocks)
+{
+  int        nb = 0, dev = 0, cus = 0;
+  hipError_t e  = count ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, batch32_kernel<K, NB, true, true>, wg, 0)
+                        : hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, batch32_kernel<K, NB, false, true>, wg, 0);
+  if (e != hipSuccess) return e;
+  if ((e = hipGetDevice(&dev)) != hipSuccess) return e;
+  if ((e = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev)) != hipSuccess) return e;
+  *blocks = (uint32_t)(std::max(nb, 1) * std::max(cus, 1));
+  return hipSuccess;
 }
 
 template <int K, int NB>
@@ -280,6 +333,19 @@ hipError_t launch_batch32(const BatchArgs& a, int K, int banks, int wg, bool cou
   if (banks <= 1) return launch32<8, 1>(a, wg, count, s);
   if (banks == 2) return launch32<8, 2>(a, wg, count, s);
   return launch32<8, 4>(a, wg, count, s);
+}
+
+hipError_t batch32_persistent_blocks(int K, int banks, int wg, bool count, uint32_t* blocks)
+{
+  if (wg != 64 && wg != 128 && wg != 256) return hipErrorInvalidValue;
+  if (K == 4) {
+    if (banks <= 1) return persist_blocks<4, 1>(wg, count, blocks);
+    if (banks == 2) return persist_blocks<4, 2>(wg, count, blocks);
+    return persist_blocks<4, 4>(wg, count, blocks);
+  }
+  if (banks <= 1) return persist_blocks<8, 1>(wg, count, blocks);
+  if (banks == 2) return persist_blocks<8, 2>(wg, count, blocks);
+  return persist_blocks<8, 4>(wg, count, blocks);
 }
 
 hipError_t launch_batch64(const BatchArgs& a, int K, int banks, hipStream_t s)

@@ -42,6 +42,8 @@ struct ebg_batch_plan
   DevBuf          grids, flags, early; // early packets: per-mesh grid records, packet flags, list
   uint32_t        early_cap = 0;       // 0: no early packets
   int             wg = 256;
+  bool            persistent = false;  // one grid that fits the device, wavefronts take packets from a queue
+  uint32_t        persist_blocks = 0;
   bool            presort = true, fp64 = true;
   unsigned long long* work = nullptr;
   hipGraph_t      graph = nullptr;
@@ -186,7 +188,9 @@ ebg_status enqueue(ebg_batch_plan* p, hipStream_t s, bool events)
   a.recheck_list  = p->fp64 ? p->recheck.as<uint32_t>() : nullptr;
   a.recheck_count = p->counter.as<uint32_t>();
   a.work          = p->work;
-  if (p->fp64 || p->early_cap) EBG_HIP(ebgpu::launch_fill_u32(a.recheck_count, 2, 0u, s)); // recheck, early
+  // counters: [0] recheck list, [1] early list, [2] persistent queue (a fill kernel, so that a
+  // captured plan clears them on every replay)
+  if (p->fp64 || p->early_cap || p->persistent) EBG_HIP(ebgpu::launch_fill_u32(a.recheck_count, 3, 0u, s));
   if (p->presort && p->total > 1) {
     int item_bits = 1;
     while ((1ull << item_bits) < p->num_items) item_bits++;
@@ -208,6 +212,11 @@ ebg_status enqueue(ebg_batch_plan* p, hipStream_t s, bool events)
     a.early_cap    = p->early_cap;
     a.early_blocks = (p->early_cap + uint32_t(p->wg / 64) - 1) / uint32_t(p->wg / 64);
     EBG_HIP(ebgpu::launch_batch_mark(a, s));
+  }
+  if (p->persistent) {
+    a.queue          = p->counter.as<uint32_t>() + 2;
+    a.queue_total    = (a.early_blocks ? a.early_cap : 0u) + p->num_packets;
+    a.persist_blocks = p->persist_blocks;
   }
   if (events) EBG_HIP(hipEventRecord(p->ev0, s));
   EBG_HIP(ebgpu::launch_batch32(a, b->K, b->banks, p->wg, p->work != nullptr, s));
@@ -382,8 +391,10 @@ ebg_status ebg_batch_plan_create(ebg_batch* b, const ebg_batch_item* items, uint
   if (!b) return fail(EBG_INVALID_ARG, "batch is null");
   if (num_items && !items) return fail(EBG_INVALID_ARG, "items is null");
   if (num_items > 0xFFFFFFull) return fail(EBG_INVALID_ARG, "too many items");
+  const bool want_persistent = (workgroup & EBG_PLAN_PERSISTENT) != 0;
+  workgroup &= ~EBG_PLAN_PERSISTENT;
   if (workgroup != 0 && workgroup != 64 && workgroup != 128 && workgroup != 256)
-    return fail(EBG_INVALID_ARG, "workgroup must be 0, 64, 128 or 256");
+    return fail(EBG_INVALID_ARG, "workgroup must be 0, 64, 128 or 256 (optionally | EBG_PLAN_PERSISTENT)");
   ebg_query_opts o;
   ebg_query_opts_default(&o);
   if (opts) o = *opts;
@@ -440,6 +451,7 @@ ebg_status ebg_batch_plan_create(ebg_batch* b, const ebg_batch_item* items, uint
   p->fp64        = (o.fp64_recheck || o.fp64_all) && p->num_packets > 0;
   p->work        = reinterpret_cast<unsigned long long*>(o.work);
   p->wg          = workgroup ? workgroup : 256;
+  p->persistent  = want_persistent;
   auto bail      = [&](hipError_t e, const char* what) {
     delete p;
     return fail(e == hipErrorOutOfMemory ? EBG_OOM : EBG_HIP_ERROR, std::string(what) + ": " + hipGetErrorString(e));
@@ -503,12 +515,22 @@ ebg_status ebg_batch_plan_create(ebg_batch* b, const ebg_batch_item* items, uint
     }
   }
 
+  auto set_shape = [&](int wg, bool persistent) -> hipError_t {
+    p->wg         = wg;
+    p->persistent = persistent;
+    return persistent ? ebgpu::batch32_persistent_blocks(b->K, b->banks, wg, p->work != nullptr, &p->persist_blocks)
+                      : hipSuccess;
+  };
+  if ((e = set_shape(p->wg, p->persistent)) != hipSuccess) return bail(e, "occupancy");
   if (!workgroup && p->num_packets > 0) {
-    // pick the workgroup shape by timing the traversal on the items as they are
+    // pick the launch shape by timing the traversal on the items as they are: one wavefront per
+    // packet in workgroups of 256, 128 or 64 lanes, or the persistent grid (64 or 256 lanes)
     double best = 1e300;
     int    pick = 256;
-    for (int wg : {256, 128, 64}) {
-      p->wg = wg;
+    bool   pick_p = false;
+    const std::pair<int, bool> shapes[] = {{256, false}, {128, false}, {64, false}, {64, true}, {256, true}};
+    for (const auto& sh : shapes) {
+      if ((e = set_shape(sh.first, sh.second)) != hipSuccess) return bail(e, "occupancy");
       double tmin = 1e300;
       for (int rep = 0; rep < 3; rep++) {
         const ebg_status st = enqueue(p, s, true);
@@ -519,11 +541,12 @@ ebg_status ebg_batch_plan_create(ebg_batch* b, const ebg_batch_item* items, uint
         tmin = std::min(tmin, elapsed_ms(p->ev0, p->ev1));
       }
       if (tmin < best) {
-        best = tmin;
-        pick = wg;
+        best   = tmin;
+        pick   = sh.first;
+        pick_p = sh.second;
       }
     }
-    p->wg = pick;
+    if ((e = set_shape(pick, pick_p)) != hipSuccess) return bail(e, "occupancy");
   }
   if (capture) {
     hipStream_t cs = nullptr;
@@ -571,7 +594,10 @@ ebg_status ebg_batch_plan_destroy(ebg_batch_plan* p)
   return EBG_OK;
 }
 
-int32_t ebg_batch_plan_workgroup(const ebg_batch_plan* p) { return p ? p->wg : 0; }
+int32_t ebg_batch_plan_workgroup(const ebg_batch_plan* p)
+{
+  return p ? (p->wg | (p->persistent ? EBG_PLAN_PERSISTENT : 0)) : 0;
+}
 
 int64_t ebg_batch_plan_last_early_packets(const ebg_batch_plan* p)
 {

@@ -143,6 +143,11 @@ struct BatchArgs
   uint32_t*           early_list;
   uint32_t*           early_count;
   uint32_t            early_cap, early_blocks;
+  // Persistent launch (persist_blocks > 0): that many workgroups, each wavefront taking the next
+  // entry of one queue -- the early list slots (early_cap of them when early_blocks is set),
+  // then the packet table -- from *queue (zero at launch) until all queue_total are taken.
+  uint32_t*           queue;
+  uint32_t            queue_total, persist_blocks;
 };
 
 hipError_t launch_batch_mark(const BatchArgs& a, hipStream_t s);
@@ -152,6 +157,8 @@ hipError_t launch_batch_keys(const BatchItemDev* items, uint32_t num_items, uint
                              hipStream_t s);
 // wg = 64, 128 or 256 lanes per workgroup (1, 2 or 4 packets).
 hipError_t launch_batch32(const BatchArgs& a, int K, int banks, int wg, bool count, hipStream_t s);
+// Workgroups of wg lanes of the persistent batch32 that fit the device at once (occupancy x CUs).
+hipError_t batch32_persistent_blocks(int K, int banks, int wg, bool count, uint32_t* blocks);
 // FP64 pass over the compacted recheck list (a.recheck_list == null skips); a fixed grid
 // strides over groups of 64 entries.
 hipError_t launch_batch64(const BatchArgs& a, int K, int banks, hipStream_t s);

@@ -474,10 +474,16 @@ ebg_status ebg_batch_get_info(const ebg_batch* batch, ebg_batch_info* info);
 /* The batch's mesh i as a mesh handle (owned by the batch; valid while the batch lives). */
 ebg_status ebg_batch_mesh(const ebg_batch* batch, uint64_t i, ebg_mesh** mesh);
 
+/* Launch shape flag of ebg_batch_plan_create's `workgroup`: a persistent grid (as many
+ * workgroups as fit the device at once) whose wavefronts take packets from a queue, in the
+ * order of the one-wavefront-per-packet launch, until none is left. */
+#define EBG_PLAN_PERSISTENT 0x10000
+
 /* Plan one set of items: checks them, builds the packet schedule (64-query packets of one item
- * each, largest meshes first), allocates every buffer, and -- if opts->fp64_all is 0 and
- * `workgroup` is 0 -- picks the workgroup size (64, 128 or 256 lanes) by timing the traversal on
- * the items as they are (their points must be valid then). With capture = 1 the pipeline is
+ * each, largest meshes first), allocates every buffer, and -- if `workgroup` is 0 -- picks the
+ * launch shape (one wavefront per packet in workgroups of 64, 128 or 256 lanes, or the persistent
+ * grid with 64 or 256) by timing the traversal on the items as they are (their points must be
+ * valid then); otherwise `workgroup` is 64, 128 or 256, optionally | EBG_PLAN_PERSISTENT. With capture = 1 the pipeline is
  * recorded once into a hipGraph; ebg_batch_plan_launch then replays it. The item arrays must
  * stay allocated while the plan is used; their contents may change between launches.
  * opts: presort, fp64_recheck, recheck_eps, fp64_all, work are honoured; stream is used for
@@ -487,7 +493,7 @@ ebg_status ebg_batch_plan_create(ebg_batch* batch, const ebg_batch_item* items, 
                                  ebg_batch_plan** out);
 ebg_status ebg_batch_plan_launch(ebg_batch_plan* plan, void* stream);
 ebg_status ebg_batch_plan_destroy(ebg_batch_plan* plan);
-/* Workgroup size the plan uses, and the time of its last launch (HIP events on the launch
+/* Launch shape the plan uses (workgroup lanes | EBG_PLAN_PERSISTENT), and the time of its last launch (HIP events on the launch
  * stream; waits for it): for a plan without a graph, around the traversal kernel alone; for a
  * captured plan, around the whole graph replay (keys, sort, traversal, FP64 pass) -- events
  * recorded inside a graph are not timed. */

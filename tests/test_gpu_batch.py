@@ -217,3 +217,34 @@ def test_more_than_256_items_use_wide_keys(cuda):
         diag = diag_of(w.verts[mesh])
         assert np.abs(d0.cpu().numpy() - d1).max() <= 1e-6 * diag
         assert np.array_equal(np.sign(d0.cpu().numpy()), np.sign(d1))
+
+
+@pytest.mark.parametrize("wg", [64, 256])
+def test_persistent_plan_equals_grid_plan(cuda, wg):
+    """The persistent launch (a grid that fits the device at once, wavefronts taking packets from
+    a queue: the early list, then the table) gives the one-wavefront-per-packet launch's results
+    bit for bit, on replays of a captured plan, with early packets in play."""
+    import torch
+    rng = np.random.default_rng(12)
+    meshes, queries = [], []
+    for k in range(3):
+        v, f = wl.displaced_icosphere(5 if k else 3, 60 + k)
+        meshes.append((v, f))
+        d = rng.normal(size=(4000, 3))
+        d /= np.linalg.norm(d, axis=1, keepdims=True)
+        deep = (d * (0.3 * rng.random(4000) ** (1 / 3))[:, None]).astype(np.float32)
+        queries.append(np.concatenate([deep, wl.mixed_queries(7001, v, f, 70 + k)]))
+    b = MeshBatch(meshes, branching=4, leaf_size=16)
+    items = [(k, torch.from_numpy(q).to(cuda)) for k, q in enumerate(queries)]
+    grid = b.plan(items, capture=True, workgroup=wg)
+    pers = b.plan(items, capture=True, workgroup=wg, persistent=True)
+    assert not grid.persistent and pers.persistent and pers.workgroup == wg
+    ref = [o.clone() for o in grid.launch()]
+    torch.cuda.synchronize()
+    for rep in range(3):
+        outs = pers.launch()
+        torch.cuda.synchronize()
+        for k in range(len(meshes)):
+            assert torch.equal(outs[k], ref[k]), (rep, k)
+        torch.empty(1024, device=cuda).fill_(float(rep))  # eager work between replays
+    assert pers.last_early_packets() > 0
