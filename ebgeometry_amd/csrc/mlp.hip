@@ -376,42 +376,69 @@ __global__ __launch_bounds__(256) void features_kernel(const float* __restrict__
 // an asymmetric squared error in that log space, e = y - log(|d|/diag + c), loss = w e^2 with
 // w = under_weight where e < 0 (a too-small cull radius forces the exact fallback) and 1
 // otherwise. The prediction is (exp(y) - c) * diag.
-// One wavefront per 64 rows; a row's 128 BF16 activations are one coalesced 256-byte read (two
-// per lane), reduced across the wavefront. Rows in [n, rows) get a zero gradient.
-__global__ __launch_bounds__(256) void out_fwd_bwd_kernel(const uint16_t* __restrict__ h4, const float* __restrict__ master,
-                                                          const float* __restrict__ target, int n, int rows,
-                                                          float inv_diag, float inv_n, float under_weight,
-                                                          float* __restrict__ gout, float* __restrict__ loss_sum,
-                                                          uint16_t* __restrict__ d4)
+// One wavefront per 64 rows: the 64 x 128 BF16 activations come in with 16-byte coalesced loads
+// (4 rows per instruction) into an LDS tile whose rows are padded to 65 words, so that each lane
+// then reads its own row without bank conflicts; the gradient rows go out the same way. Rows in
+// [n, rows) get a zero gradient.
+constexpr int kOutLd = kMlpHidden / 2 + 1; // LDS words per tile row
+
+__global__ __launch_bounds__(64) void out_fwd_bwd_kernel(const uint16_t* __restrict__ h4, const float* __restrict__ master,
+                                                         const float* __restrict__ target, int n, int rows,
+                                                         float inv_diag, float inv_n, float under_weight,
+                                                         float* __restrict__ gout, float* __restrict__ loss_sum,
+                                                         uint16_t* __restrict__ d4)
 {
-  const int   lane = threadIdx.x & 63;
-  const int   row0 = (blockIdx.x * 4 + (threadIdx.x >> 6)) * 64;
-  const float wa = master[kOffW5 + 2 * lane], wb = master[kOffW5 + 2 * lane + 1];
-  const float b5 = master[kOffB5];
-  float       lsum = 0.0f;
-  for (int r = 0; r < 64; r++) {
-    const int i = row0 + r; // wave-uniform
-    if (i >= rows) break;
-    const uint32_t hh = *reinterpret_cast<const uint32_t*>(h4 + (size_t)i * kMlpHidden + 2 * lane);
-    float          y  = bf2f((uint16_t)(hh & 0xFFFFu)) * wa + bf2f((uint16_t)(hh >> 16)) * wb;
-    for (int o = 32; o > 0; o >>= 1) y += __shfl_xor(y, o, 64);
-    y += b5;
-    float gi = 0.0f;
-    if (i < n) {
-      const float lt = logf(fabsf(target[i]) * inv_diag + kMlpLogC);
-      const float e  = y - lt;
-      const float w  = e < 0.0f ? under_weight : 1.0f;
-      lsum += w * e * e;
-      gi = 2.0f * w * e * inv_n;
-    }
-    if (lane == 0) gout[i] = gi;
-    const uint16_t h0 = (uint16_t)(hh & 0xFFFFu), h1 = (uint16_t)(hh >> 16);
-    const uint16_t d0 = f2bf((h0 == 0 || (h0 & 0x8000u)) ? 0.0f : gi * wa);
-    const uint16_t d1 = f2bf((h1 == 0 || (h1 & 0x8000u)) ? 0.0f : gi * wb);
-    *reinterpret_cast<uint32_t*>(d4 + (size_t)i * kMlpHidden + 2 * lane) = (uint32_t)d0 | ((uint32_t)d1 << 16);
+  __shared__ uint32_t tile[64 * kOutLd];
+  __shared__ float    w5[kMlpHidden];
+  __shared__ float    gs[64];
+  const int lane = threadIdx.x;
+  const int row0 = blockIdx.x * 64;
+  w5[lane]      = master[kOffW5 + lane];
+  w5[lane + 64] = master[kOffW5 + 64 + lane];
+  const int sub = lane >> 4, c4 = (lane & 15) * 4; // 16 lanes per row, 4 words (8 BF16) each
+  for (int pass = 0; pass < 16; pass++) {
+    const int r = pass * 4 + sub, i = row0 + r;
+    uint4     v = make_uint4(0u, 0u, 0u, 0u);
+    if (i < rows) v = *reinterpret_cast<const uint4*>(h4 + (size_t)i * kMlpHidden + 2 * c4);
+    uint32_t* t = tile + r * kOutLd + c4;
+    t[0] = v.x, t[1] = v.y, t[2] = v.z, t[3] = v.w;
   }
-  // every lane holds the same row sums: one atomic per wavefront
-  if (lane == 0 && lsum != 0.0f) atomicAdd(loss_sum, lsum);
+  __syncthreads();
+  const int i  = row0 + lane;
+  float     y  = 0.0f;
+  for (int k = 0; k < kMlpHidden / 2; k++) {
+    const uint32_t hh = tile[lane * kOutLd + k];
+    y += bf2f((uint16_t)(hh & 0xFFFFu)) * w5[2 * k] + bf2f((uint16_t)(hh >> 16)) * w5[2 * k + 1];
+  }
+  y += master[kOffB5];
+  float gi = 0.0f, l = 0.0f;
+  if (i < n) {
+    const float lt = logf(fabsf(target[i]) * inv_diag + kMlpLogC);
+    const float e  = y - lt;
+    const float w  = e < 0.0f ? under_weight : 1.0f;
+    l  = w * e * e;
+    gi = 2.0f * w * e * inv_n;
+  }
+  if (i < rows) gout[i] = gi;
+  gs[lane] = gi;
+  __syncthreads();
+  for (int pass = 0; pass < 16; pass++) {
+    const int r = pass * 4 + sub, i2 = row0 + r;
+    if (i2 >= rows) continue;
+    const float     g = gs[r];
+    const uint32_t* t = tile + r * kOutLd + c4;
+    uint32_t        o[4];
+    for (int q = 0; q < 4; q++) {
+      const uint16_t h0 = (uint16_t)(t[q] & 0xFFFFu), h1 = (uint16_t)(t[q] >> 16);
+      const int      k  = 2 * (c4 + q);
+      const uint16_t d0 = f2bf((h0 == 0 || (h0 & 0x8000u)) ? 0.0f : g * w5[k]);
+      const uint16_t d1 = f2bf((h1 == 0 || (h1 & 0x8000u)) ? 0.0f : g * w5[k + 1]);
+      o[q]              = (uint32_t)d0 | ((uint32_t)d1 << 16);
+    }
+    *reinterpret_cast<uint4*>(d4 + (size_t)i2 * kMlpHidden + 2 * c4) = make_uint4(o[0], o[1], o[2], o[3]);
+  }
+  for (int off = 32; off > 0; off >>= 1) l += __shfl_xor(l, off, 64);
+  if (lane == 0 && l != 0.0f) atomicAdd(loss_sum, l);
 }
 
 // dw5[k] = sum_i h4[i,k] g[i] (block k < 128), db5 = sum_i g[i] (block 128)
@@ -778,8 +805,8 @@ static hipError_t loss_grad_launches(Mlp& m, MlpTrainBuffers& tb, int capacity, 
   }
   // loss (accumulated into tb.loss[0]; the caller clears it) + output layer backward
   if (!check_only) {
-    hipLaunchKernelGGL(out_fwd_bwd_kernel, dim3((rows + 255) / 256), dim3(256), 0, s, tb.h[3], m.master, target, n,
-                       rows, 1.0f / m.diag, 1.0f / (float)n, m.under_weight, tb.g, tb.loss, tb.dl[3]);
+    hipLaunchKernelGGL(out_fwd_bwd_kernel, dim3((rows + 63) / 64), dim3(64), 0, s, tb.h[3], m.master, target, n, rows,
+                       1.0f / m.diag, 1.0f / (float)n, m.under_weight, tb.g, tb.loss, tb.dl[3]);
     hipLaunchKernelGGL(out_wgrad_kernel, dim3(kMlpHidden + 1), dim3(256), 0, s, tb.h[3], tb.g, rows, m.grad);
   }
   // hidden layers, last to first

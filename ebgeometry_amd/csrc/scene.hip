@@ -158,9 +158,28 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(KM == 4 ? 8
     py = a.pts[3 * (size_t)qi + 1];
     pz = a.pts[3 * (size_t)qi + 2];
   }
-  float       v = INFINITY;
+  // Seeded (learned bound or caller radii): the walk starts from v = seed, bound seed^2, and is
+  // exact when some primitive comes in below the seed; the other lanes redo the walk unseeded
+  // in the same wavefront (the rest sit it out with bound -1).
+  const float seed = (active && a.seed_r) ? a.seed_r[qi] : INFINITY;
+  float       v    = seed;
   SceneCounts c;
-  scene_traverse<KM, NB, float, COUNT>(sc, px, py, pz, v, active ? INFINITY : -1.0f, c);
+  scene_traverse<KM, NB, float, COUNT>(sc, px, py, pz, v, active ? seed * seed : -1.0f, c);
+  if (a.seed_r) {
+    const bool     redo  = active && !(v < seed);
+    const uint64_t rmask = ballot(redo);
+    if (rmask) {
+      float       v2 = INFINITY;
+      SceneCounts c2;
+      scene_traverse<KM, NB, float, COUNT>(sc, px, py, pz, v2, redo ? INFINITY : -1.0f, c2);
+      if (redo) v = v2;
+      if (__lane_id() == 0 && a.fallback_count) atomicAdd(a.fallback_count, (uint32_t)__popcll(rmask));
+      if (COUNT) {
+        c.n_inner += c2.n_inner, c.n_prims += c2.n_prims, c.n_tris += c2.n_tris;
+        c.w_inner += c2.w_inner, c.w_prims += c2.w_prims, c.w_tris += c2.w_tris;
+      }
+    }
+  }
   if (active) a.out_d[qi] = v;
   if (a.recheck_list) wave_append(active && fabsf(v) < a.recheck_abs, qi, a.recheck_list, a.recheck_count);
   if (COUNT && a.work) {

@@ -249,10 +249,18 @@ __device__ __forceinline__ void tri_eval32(const TriRec& r, float px, float py, 
 // tri_eval32<false> behind a wavefront-level cull: d^2 = fd^2 + (in-plane distance)^2, and the
 // in-plane distance is at least the distance past any edge line the projected point lies beyond
 // (c_i points out of the triangle, |c_i| = |e_i|), so lb = fd^2 + max_i max(c_i.p_i, 0)^2 / |e_i|^2
-// <= d^2. When no lane's lb reaches its bound the wavefront skips the rest of the evaluation
-// (returns false); otherwise d2 is tri_eval32's, operation for operation. The margin 2^-16
-// keeps the cull from acting on rounding-level differences of lb and d2.
-__device__ __forceinline__ bool tri_eval32_culled(const TriRec& r, float px, float py, float pz, float bound, float& d2)
+// <= d^2. When no lane's lb reaches its limit `cull` the wavefront skips the rest of the
+// evaluation (returns false); otherwise d2 is tri_eval32's, operation for operation. `cull` is
+// cull_band(bound): lb and the evaluated d2 round differently (each within 2^-21 (d + 2 hmax) of
+// the exact distance), and with the band no triangle whose evaluated d2 would reach the bound is
+// skipped, so the result does not depend on the order in which the bound tightened.
+__device__ __forceinline__ float cull_band(float bound, float hmax)
+{
+  const float r = fmaf(__builtin_amdgcn_sqrtf(bound), 1.0f + 0x1p-19f, hmax * 0x1p-18f);
+  return r * r * (1.0f + 0x1p-20f);
+}
+
+__device__ __forceinline__ bool tri_eval32_culled(const TriRec& r, float px, float py, float pz, float cull, float& d2)
 {
   const float p1x = px - r.v0[0], p1y = py - r.v0[1], p1z = pz - r.v0[2];
   const float p2x = p1x - r.e0[0], p2y = p1y - r.e0[1], p2z = p1z - r.e0[2];
@@ -263,8 +271,8 @@ __device__ __forceinline__ bool tri_eval32_culled(const TriRec& r, float px, flo
   const float fd  = r.n[0] * p1x + r.n[1] * p1y + r.n[2] * p1z;
   const float o0 = fmaxf(s0, 0.0f), o1 = fmaxf(s1, 0.0f), o2 = fmaxf(s2, 0.0f);
   const float out = fmaxf(fmaxf(o0 * o0 * r.inv[0], o1 * o1 * r.inv[1]), o2 * o2 * r.inv[2]);
-  const float lb  = fmaf(fd, fd, out) * (1.0f - 0x1p-16f);
-  if (ballot(lb <= bound) == 0) return false;
+  const float lb  = fmaf(fd, fd, out);
+  if (ballot(lb <= cull) == 0) return false;
   const bool  inface = ((s0 > 0.0f) == (s1 > 0.0f)) && ((s1 > 0.0f) == (s2 > 0.0f));
   const float t0  = __builtin_amdgcn_fmed3f((p1x * r.e0[0] + p1y * r.e0[1] + p1z * r.e0[2]) * r.inv[0], 0.0f, 1.0f);
   const float t1  = __builtin_amdgcn_fmed3f((p2x * r.e1[0] + p2y * r.e1[1] + p2z * r.e1[2]) * r.inv[1], 0.0f, 1.0f);
@@ -574,11 +582,16 @@ __device__ __forceinline__ void traverse_from(const MeshDev& m, const Entry& sta
       // lane's bound (both are lower bounds on the distance to the triangle). Inactive lanes
       // have bound < 0, so rb is NaN and they never ask for a triangle. The culling data of 4
       // triangles arrives in one scalar round trip (the array is padded past the last leaf),
-      // then each needed triangle record in one more, where the plane is tested again against
-      // the bound as tightened by the leaf's earlier triangles.
+      // then each needed triangle record in one more, where a lower bound (the plane and the
+      // edge lines, tri_eval32_culled) is tested against the bound as tightened by the leaf's
+      // earlier triangles.
       const uint32_t first = uni(e.ref), cnt = uni(e.cnt);
       R              rb;
-      if constexpr (sizeof(R) == 4) rb = __builtin_amdgcn_sqrtf(bound) * 1.000001f; // 1-ulp hardware root, rounded up
+      [[maybe_unused]] float cull = 0.0f; // FP32 distance-only walks: the lower-bound cull's limit
+      if constexpr (sizeof(R) == 4) {
+        rb = __builtin_amdgcn_sqrtf(bound) * 1.000001f; // 1-ulp hardware root, rounded up
+        if constexpr (!FACE) cull = cull_band(bound, m.hmax);
+      }
       else rb = sqrt(bound);
       for (uint32_t c0 = 0; c0 < cnt; c0 += 4) {
         const SWords<32> sp   = sload<32>(m.tsph + 8 * (size_t)(first + c0));
@@ -613,7 +626,7 @@ __device__ __forceinline__ void traverse_from(const MeshDev& m, const Entry& sta
           if constexpr (sizeof(R) == 4) {
             const TriRec rec = sload_record(m.tris + t);
             if constexpr (!FACE) {
-              if (!tri_eval32_culled(rec, px, py, pz, bound, d2)) continue;
+              if (!tri_eval32_culled(rec, px, py, pz, cull, d2)) continue;
               if (COUNT) w_tris++;
             }
             else {
@@ -649,6 +662,7 @@ __device__ __forceinline__ void traverse_from(const MeshDev& m, const Entry& sta
             btri  = t;
             bfeat = f;
             bound = d2 < bound ? d2 : bound; // bound stays min(bound, best)
+            if constexpr (sizeof(R) == 4 && !FACE) cull = cull_band(bound, m.hmax);
           }
         }
       }
