@@ -368,7 +368,27 @@ def scene_bench(a, dev):
     byts = 16 * n + w[4] * 128 + w[7] * 128 + w[5] * 128
     gbs = byts / (kms * 1e-3) / 1e9
     log(f"scene {dt * 1e3:.2f} ms/step (traversal {kms:.2f} ms)")
-    return {"workload": "csg-union-4096-spheres-1024-boxes-8-meshes-32M", "queries": n,
+    # the per-scene learned bound: trained on 8M points labelled by the scene's own walk, then
+    # every query's walk seeded with it (exact: queries with nothing inside the seed are redone)
+    t0 = time.perf_counter()
+    loss = u.train_bound(num_samples=8_000_000, epochs=6, seed=3)
+    torch.cuda.synchronize()
+    train_s = time.perf_counter() - t0
+    out_l = torch.empty_like(out)
+    dtl, kml = timed(lambda: u.value(q, out=out_l, learned_bound=True), a.steps, a.warmup, u.last_traversal_ms)
+    torch.cuda.synchronize()
+    exact = bool(torch.equal(out, out_l))
+    wl_ = torch.zeros(8, dtype=torch.int64, device=dev)
+    u.value(q, out=out_l, work=wl_.data_ptr(), learned_bound=True)
+    torch.cuda.synchronize()
+    wl_ = [int(x) for x in wl_.cpu().numpy()]
+    log(f"scene learned {dtl * 1e3:.2f} ms/step (traversal {kml:.2f} ms), bound trained in {train_s:.1f}s")
+    learned = {"value": round(n / dtl, 1), "ms_per_step": round(dtl * 1e3, 3), "traversal_ms": round(kml, 3),
+               "speedup_learned_vs_plain": round(dt / dtl, 3), "train_samples": 8_000_000, "epochs": 6,
+               "train_seconds": round(train_s, 2), "final_loss": round(loss, 5), "equal_to_plain": exact,
+               "per_query": {"interior_nodes": round(wl_[0] / n, 2), "primitives": round(wl_[6] / n, 2),
+                             "triangles": round(wl_[1] / n, 2), "redone_unseeded": wl_[3]}}
+    return {"workload": "csg-union-4096-spheres-1024-boxes-8-meshes-32M", "queries": n, "learned_bound": learned,
             "value": round(n / dt, 1), "unit": "queries/s", "ms_per_step": round(dt * 1e3, 3),
             "roofline": {"bound": "hbm", "achieved": round(gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                          "frac": round(gbs / HBM_PEAK_GBS, 4), "traffic": None, "kernel": "scene32_kernel<4,1,false>",

@@ -101,6 +101,8 @@ SIGNATURES = [
     ("ebg_scene_query_grid", C.c_int, [_vp, _vp, _vp, _vp, _vp, C.POINTER(QueryOpts)]),
     ("ebg_scene_query_grid_host", C.c_int, [_vp, _vp, _vp, _vp, _vp, C.POINTER(QueryOpts)]),
     ("ebg_scene_last_traversal_ms", C.c_double, [_vp]),
+    ("ebg_scene_mlp_train", C.c_int, [_vp, C.POINTER(MlpOpts), C.POINTER(C.c_float)]),
+    ("ebg_scene_mlp_predict", C.c_int, [_vp, _vp, _u64, _vp, _vp]),
     ("ebg_csg_create", C.c_int, [C.POINTER(CsgOp), _u64, _u64, C.POINTER(CsgPrim), _u64, _vp, _u64, C.c_double,
                                  C.POINTER(_vp)]),
     ("ebg_csg_destroy", C.c_int, [_vp]),

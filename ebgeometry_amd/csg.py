@@ -309,7 +309,8 @@ class BVHUnionIF(ImplicitFunction):
 
     @staticmethod
     def query_opts(presort=True, fp64_recheck=True, recheck_eps=1e-4, fp64_all=False, stream=None,
-                   work=None) -> QueryOpts:
+                   work=None, learned_bound=False, bound_scale=1.25, bound_offset=1e-3,
+                   seed_radius=None) -> QueryOpts:
         o = QueryOpts()
         lib().ebg_query_opts_default(C.byref(o))
         o.presort = int(presort)
@@ -318,7 +319,38 @@ class BVHUnionIF(ImplicitFunction):
         o.fp64_all = int(fp64_all)
         o.stream = stream
         o.work = work
+        o.learned_bound = int(learned_bound)
+        o.bound_scale = float(bound_scale)
+        o.bound_offset = float(bound_offset)
+        o.seed_radius = seed_radius
         return o
+
+    def train_bound(self, num_samples: int = 8_000_000, epochs: int = 6, batch: int = 4096, lr: float = 2e-3,
+                    seed: int = 12345, near_sigma: float = 0.02, box_scale: float = 1.2, under_weight: float = 16.0,
+                    stream=None) -> float:
+        """Train the union's learned cull bound on the GPU (the scene's own walk labels the
+        samples); value(..., learned_bound=True) then seeds each query's walk. Returns the last
+        epoch's mean loss."""
+        self._ensure()
+        if self._program is not None:
+            raise NotImplementedError("the learned bound is for unions evaluated as a scene")
+        from . import _lib
+        o = _lib.MlpOpts()
+        lib().ebg_mlp_opts_default(C.byref(o))
+        o.num_samples, o.epochs, o.batch, o.lr, o.seed = num_samples, epochs, batch, lr, seed
+        o.near_sigma, o.box_scale, o.stream, o.under_weight = near_sigma, box_scale, stream, under_weight
+        loss = C.c_float(0.0)
+        check(lib().ebg_scene_mlp_train(self._h, C.byref(o), C.byref(loss)))
+        return float(loss.value)
+
+    def predict_bound(self, pts_tensor):
+        """The trained network's predicted |value| at CUDA float32 points (n, 3)."""
+        torch = _torch()
+        p = pts_tensor.contiguous().view(-1, 3)
+        out = torch.empty(p.shape[0], dtype=torch.float32, device=p.device)
+        check(lib().ebg_scene_mlp_predict(self._h, p.data_ptr(), p.shape[0], out.data_ptr(),
+                                          torch.cuda.current_stream(p.device).cuda_stream))
+        return out
 
     def value(self, points, out=None, **kw):
         """Union value at a point (float) or at every row of an (n, 3) array / CUDA tensor."""

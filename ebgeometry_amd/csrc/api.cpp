@@ -913,27 +913,12 @@ ebg_status ebg_mlp_train(ebg_mesh* m, const ebg_mlp_opts* opts, float* loss)
   const ebg_status st = ebg_query_sdf(m, pts.as<float>(), n, dist.as<float>(), nullptr, &q);
   if (st != EBG_OK) return st;
 
-  // Adam with a cosine learning-rate decay to 5% over all steps
-  EBG_HIP(m->train.ensure(o.batch));
-  const uint64_t spe   = (n + o.batch - 1) / o.batch;
-  const double   total = double(spe) * o.epochs;
-  float          last  = 0.0f;
-  uint64_t       k     = 0;
-  for (int ep = 0; ep < o.epochs; ep++) {
-    EBG_HIP(hipMemsetAsync(m->train.loss, 0, sizeof(float), s));
-    for (uint64_t b = 0; b < n; b += o.batch, k++) {
-      const int   nb = int(std::min<uint64_t>(o.batch, n - b));
-      const float lr = float(o.lr * (0.05 + 0.95 * 0.5 * (1.0 + std::cos(M_PI * double(k) / total))));
-      std::string      why;
-      const hipError_t le = ebgpu::mlp_loss_grad(mlp, m->train, pts.as<float>() + 3 * b, dist.as<float>() + b, nb, s, &why);
-      if (le != hipSuccess) return fail(le == hipErrorOutOfMemory ? EBG_OOM : EBG_HIP_ERROR, why.empty() ? hipGetErrorString(le) : why);
-      EBG_HIP(ebgpu::mlp_adam(mlp, lr, s));
-    }
-    float sum = 0.0f;
-    EBG_HIP(hipMemcpyAsync(&sum, m->train.loss, sizeof(float), hipMemcpyDeviceToHost, s));
-    EBG_HIP(hipStreamSynchronize(s));
-    last = sum / float(n);
-  }
+  float       last = 0.0f;
+  std::string why;
+  const hipError_t fe = ebgpu::mlp_fit(mlp, m->train, pts.as<float>(), dist.as<float>(), n, o.epochs, o.batch, o.lr, s,
+                                       &last, &why);
+  if (fe != hipSuccess)
+    return fail(fe == hipErrorOutOfMemory ? EBG_OOM : EBG_HIP_ERROR, why.empty() ? hipGetErrorString(fe) : why);
   mlp.trained = true;
   if (loss) *loss = last;
   return EBG_OK;

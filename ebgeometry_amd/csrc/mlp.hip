@@ -596,6 +596,32 @@ __global__ __launch_bounds__(256) void samples_kernel(MeshDev m, const float* __
   for (int c = 0; c < 3; c++) out[3 * i + c] = (float)p[c];
 }
 
+// Training samples of a union scene's bound: even i uniform in the training box, odd i uniform
+// in the world box of a random primitive (the PrimRec bounding boxes) grown by sigma on each side.
+__global__ __launch_bounds__(256) void scene_samples_kernel(const PrimRec* __restrict__ prims, uint32_t np, float lo0,
+                                                            float lo1, float lo2, float hi0, float hi1, float hi2,
+                                                            float box_scale, float sigma, uint64_t key, uint64_t n,
+                                                            float* __restrict__ out)
+{
+  const uint64_t i = (uint64_t)blockIdx.x * 256 + threadIdx.x;
+  if (i >= n) return;
+  const double lo[3] = {lo0, lo1, lo2}, hi[3] = {hi0, hi1, hi2};
+  double       p[3];
+  if ((i & 1u) == 0) {
+    for (int c = 0; c < 3; c++) {
+      const double ctr = 0.5 * (lo[c] + hi[c]), h = 0.5 * box_scale * (hi[c] - lo[c]);
+      p[c]             = ctr + h * (2.0 * u01(key, 8 * i + c) - 1.0);
+    }
+  }
+  else {
+    const uint32_t k  = min(np - 1, (uint32_t)(u01(key, 8 * i) * np));
+    const float*   bv = prims[k].bv;
+    for (int c = 0; c < 3; c++)
+      p[c] = (double)bv[c] - sigma + ((double)bv[3 + c] - bv[c] + 2.0 * sigma) * u01(key, 8 * i + 1 + c);
+  }
+  for (int c = 0; c < 3; c++) out[3 * i + c] = (float)p[c];
+}
+
 MlpDev dev_of(const Mlp& m)
 {
   MlpDev d;
@@ -911,6 +937,43 @@ hipError_t mlp_samples(const MeshDev& dev, const float* cdf, uint32_t nt, const 
   hipLaunchKernelGGL(samples_kernel, dim3((uint32_t)((n + 255) / 256)), dim3(256), 0, s, dev, cdf, nt, lo[0], lo[1],
                      lo[2], hi[0], hi[1], hi[2], box_scale, sigma, key, n, pts);
   return hipGetLastError();
+}
+
+hipError_t mlp_scene_samples(const PrimRec* prims, uint32_t num_prims, const float* lo, const float* hi,
+                             float box_scale, float sigma, uint64_t seed, uint64_t n, float* pts, hipStream_t s)
+{
+  if (n == 0 || num_prims == 0) return hipSuccess;
+  uint64_t key = seed * 0x9E3779B97F4A7C15ull + 0x2545F4914F6CDD1Dull;
+  hipLaunchKernelGGL(scene_samples_kernel, dim3((uint32_t)((n + 255) / 256)), dim3(256), 0, s, prims, num_prims, lo[0],
+                     lo[1], lo[2], hi[0], hi[1], hi[2], box_scale, sigma, key, n, pts);
+  return hipGetLastError();
+}
+
+hipError_t mlp_fit(Mlp& m, MlpTrainBuffers& tb, const float* pts, const float* dist, uint64_t n, int epochs, int batch,
+                   float lr0, hipStream_t s, float* last_loss, std::string* why)
+{
+  hipError_t e = tb.ensure(batch);
+  if (e != hipSuccess) return e;
+  // Adam with a cosine learning-rate decay to 5% over all steps
+  const uint64_t spe   = (n + batch - 1) / batch;
+  const double   total = double(spe) * epochs;
+  uint64_t       k     = 0;
+  float          last  = 0.0f;
+  for (int ep = 0; ep < epochs; ep++) {
+    if ((e = hipMemsetAsync(tb.loss, 0, sizeof(float), s)) != hipSuccess) return e;
+    for (uint64_t b = 0; b < n; b += batch, k++) {
+      const int   nb = int(std::min<uint64_t>(batch, n - b));
+      const float lr = float(lr0 * (0.05 + 0.95 * 0.5 * (1.0 + std::cos(M_PI * double(k) / total))));
+      if ((e = mlp_loss_grad(m, tb, pts + 3 * b, dist + b, nb, s, why)) != hipSuccess) return e;
+      if ((e = mlp_adam(m, lr, s)) != hipSuccess) return e;
+    }
+    float sum = 0.0f;
+    if ((e = hipMemcpyAsync(&sum, tb.loss, sizeof(float), hipMemcpyDeviceToHost, s)) != hipSuccess) return e;
+    if ((e = hipStreamSynchronize(s)) != hipSuccess) return e;
+    last = sum / float(n);
+  }
+  if (last_loss) *last_loss = last;
+  return hipSuccess;
 }
 
 } // namespace ebgpu

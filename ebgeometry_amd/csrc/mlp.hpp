@@ -118,5 +118,13 @@ hipError_t mlp_grad_public(const Mlp& m, float* out, hipStream_t s);
 // extent), second half area-uniform on the surface pushed along the face normal by N(0, sigma).
 hipError_t mlp_samples(const MeshDev& dev, const float* area_cdf, uint32_t num_tris, const float* lo, const float* hi,
                        float box_scale, float sigma, uint64_t seed, uint64_t n, float* pts, hipStream_t s);
+// Training points of a union scene's bound: half uniform in box_scale x [lo, hi], half uniform in
+// a random primitive's world box grown by sigma.
+hipError_t mlp_scene_samples(const PrimRec* prims, uint32_t num_prims, const float* lo, const float* hi,
+                             float box_scale, float sigma, uint64_t seed, uint64_t n, float* pts, hipStream_t s);
+// Train m on n (point, distance) pairs (device arrays): epochs passes in batches of `batch`,
+// Adam with a cosine learning-rate decay from lr0 to 5%; *last_loss = the last epoch's mean loss.
+hipError_t mlp_fit(Mlp& m, MlpTrainBuffers& tb, const float* pts, const float* dist, uint64_t n, int epochs, int batch,
+                   float lr0, hipStream_t s, float* last_loss, std::string* why);
 
 } // namespace ebgpu

@@ -4,6 +4,7 @@
 // FP32 packet traversal of the scene tree with nested mesh traversals -> FP64 pass over the
 // queries with |v| < recheck_eps * scene diagonal.
 #include "host_common.hpp"
+#include "mlp.hpp"
 #include "mesh_build.hpp"
 
 #include <hip/hip_runtime.h>
@@ -32,6 +33,8 @@ struct ebg_scene
   DevBuf                 gridpts;
   hipEvent_t             ev0 = nullptr, ev1 = nullptr;
   bool                   timed = false;
+  ebgpu::Mlp             mlp;   // learned cull bound of the union (ebg_scene_mlp_train)
+  ebgpu::MlpTrainBuffers train;
 
   ~ebg_scene()
   {
@@ -271,9 +274,9 @@ ebg_status ebg_scene_query(const ebg_scene* csc, const float* d_pts, uint64_t n,
   ebg_query_opts o;
   ebg_query_opts_default(&o);
   if (opts) o = *opts;
-  if (o.learned_bound || o.seed_radius) return fail(EBG_INVALID_ARG, "scenes take no learned or seeded bound");
   ebg_scene*  sc = const_cast<ebg_scene*>(csc);
   hipStream_t s  = static_cast<hipStream_t>(o.stream);
+  if (o.learned_bound && !sc->mlp.trained) return fail(EBG_NOT_READY, "learned bound requested but no network trained");
 
   std::lock_guard<std::mutex> lock(sc->mu);
   Workspace&                  ws = sc->ws;
@@ -289,12 +292,21 @@ ebg_status ebg_scene_query(const ebg_scene* csc, const float* d_pts, uint64_t n,
     EBG_HIP(ebgpu::presort(ws.view(), d_pts, n, s));
     idx = ws.perm.as<uint32_t>();
   }
+  // seeded walks: the union network's radii (the query order; it needs no presort), or the caller's
+  const float* seeds = o.seed_radius;
+  if (o.learned_bound) {
+    EBG_HIP(ebgpu::mlp_forward(sc->mlp, d_pts, nullptr, n, true, o.bound_scale, float(o.bound_offset * sc->diag),
+                               ws.view().seeds, s));
+    seeds = ws.view().seeds;
+  }
   const bool       recheck = o.fp64_recheck || o.fp64_all;
   ebgpu::QueryArgs a{};
-  a.pts           = d_pts;
-  a.idx           = idx;
-  a.n             = n;
-  a.out_d         = d_value;
+  a.pts            = d_pts;
+  a.idx            = idx;
+  a.n              = n;
+  a.seed_r         = seeds;
+  a.fallback_count = cnt + 1;
+  a.out_d          = d_value;
   a.recheck_abs   = o.fp64_all ? INFINITY : (recheck ? float(o.recheck_eps * sc->diag) : 0.0f);
   a.recheck_list  = recheck ? ws.recheck_list() : nullptr;
   a.recheck_count = cnt;
@@ -316,7 +328,72 @@ ebg_status ebg_scene_query(const ebg_scene* csc, const float* d_pts, uint64_t n,
     r.slack_rel = 1e-4f;
     EBG_HIP(ebgpu::launch_scene64(sc->dev, sc->KM, sc->banks, r, s));
   }
-  if (count) EBG_HIP(ebgpu::launch_add_counts(a.work, recheck ? cnt : nullptr, nullptr, s));
+  if (count) EBG_HIP(ebgpu::launch_add_counts(a.work, recheck ? cnt : nullptr, seeds ? cnt + 1 : nullptr, s));
+  return EBG_OK;
+}
+
+ebg_status ebg_scene_mlp_train(ebg_scene* sc, const ebg_mlp_opts* opts, float* loss)
+{
+  if (!sc) return fail(EBG_INVALID_ARG, "scene is null");
+  ebg_mlp_opts o;
+  ebg_mlp_opts_default(&o);
+  if (opts) o = *opts;
+  if (o.num_samples < 64 || o.num_samples > 0xFFFFFFFFull) return fail(EBG_INVALID_ARG, "num_samples out of range");
+  if (o.epochs < 1 || o.batch < 32 || o.batch > 65536) return fail(EBG_INVALID_ARG, "epochs/batch out of range");
+  if (!(o.lr > 0.0f) || !(o.box_scale >= 1.0f) || !(o.near_sigma >= 0.0f) || !(o.under_weight >= 1.0f))
+    return fail(EBG_INVALID_ARG, "lr, box_scale, near_sigma or under_weight out of range");
+  {
+    std::string why;
+    if (!ebgpu::mlp_check_train_schedule(o.num_samples, o.batch, &why)) return fail(EBG_INVALID_ARG, why);
+  }
+  hipStream_t s = static_cast<hipStream_t>(o.stream);
+  ebgpu::Mlp& mlp = sc->mlp;
+  EBG_HIP(ebgpu::mlp_alloc(mlp));
+  for (int c = 0; c < 3; c++) {
+    const double ctr = 0.5 * (sc->lo[c] + sc->hi[c]);
+    double       h   = 0.5 * o.box_scale * (sc->hi[c] - sc->lo[c]);
+    if (!(h > 0.0)) h = 1.0;
+    mlp.lo[c]    = float(ctr - h);
+    mlp.scale[c] = float(1.0 / h);
+  }
+  mlp.diag      = float(sc->diag);
+  mlp.box_scale = o.box_scale;
+  EBG_HIP(ebgpu::mlp_init(mlp, o.seed));
+  mlp.trained      = false;
+  mlp.under_weight = o.under_weight;
+
+  // (point, |value|) pairs from the scene's own BVH walk
+  const uint64_t n = o.num_samples;
+  DevBuf         pts, dist;
+  EBG_HIP(pts.reserve(12 * n));
+  EBG_HIP(dist.reserve(4 * n));
+  const float lo[3] = {float(sc->lo[0]), float(sc->lo[1]), float(sc->lo[2])};
+  const float hi[3] = {float(sc->hi[0]), float(sc->hi[1]), float(sc->hi[2])};
+  EBG_HIP(ebgpu::mlp_scene_samples(sc->prims.as<ebgpu::PrimRec>(), uint32_t(sc->num_prims), lo, hi, o.box_scale,
+                                   float(o.near_sigma * sc->diag), o.seed, n, pts.as<float>(), s));
+  ebg_query_opts q;
+  ebg_query_opts_default(&q);
+  q.stream            = o.stream;
+  const ebg_status st = ebg_scene_query(sc, pts.as<float>(), n, dist.as<float>(), &q);
+  if (st != EBG_OK) return st;
+  float       last = 0.0f;
+  std::string why;
+  const hipError_t fe = ebgpu::mlp_fit(mlp, sc->train, pts.as<float>(), dist.as<float>(), n, o.epochs, o.batch, o.lr,
+                                       s, &last, &why);
+  if (fe != hipSuccess)
+    return fail(fe == hipErrorOutOfMemory ? EBG_OOM : EBG_HIP_ERROR, why.empty() ? hipGetErrorString(fe) : why);
+  mlp.trained = true;
+  if (loss) *loss = last;
+  return EBG_OK;
+}
+
+ebg_status ebg_scene_mlp_predict(const ebg_scene* sc, const float* d_pts, uint64_t n, float* d_pred, void* stream)
+{
+  if (!sc) return fail(EBG_INVALID_ARG, "scene is null");
+  if (n == 0) return EBG_OK;
+  if (!d_pts || !d_pred) return fail(EBG_INVALID_ARG, "null pointer");
+  if (!sc->mlp.trained) return fail(EBG_NOT_READY, "no network trained");
+  EBG_HIP(ebgpu::mlp_forward(sc->mlp, d_pts, nullptr, n, false, 1.0f, 0.0f, d_pred, static_cast<hipStream_t>(stream)));
   return EBG_OK;
 }
 

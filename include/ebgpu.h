@@ -277,8 +277,12 @@ ebg_status ebg_scene_bounds(const ebg_scene* scene, double* lo3, double* hi3, do
 /* Batched BVHUnionIF::value: d_pts n float32 xyz, d_value n float32 (device pointers, async on
  * opts->stream). Honours presort, fp64_recheck / recheck_eps (relative to the scene diagonal),
  * fp64_all, stream and work ([0] interior nodes incl. mesh nodes, [1] triangles, [2] rechecked,
- * [4]/[5] wave-level fetches, [6] primitives evaluated, [7] wave-level primitive fetches).
- * learned_bound and seed_radius are not supported for scenes (EBG_INVALID_ARG). */
+ * [4]/[5] wave-level fetches, [6] primitives evaluated, [7] wave-level primitive fetches; with a
+ * seeded walk [3] = queries redone unseeded). learned_bound (after ebg_scene_mlp_train) or
+ * seed_radius seeds each query's walk with a radius, the value-based pruning of
+ * BVHUnionIF::value (Source/EBGeometry_CSGImplem.hpp:490-499) starting from it; a query none of
+ * whose primitives comes in below its radius is redone unseeded, so the result is the
+ * unseeded one either way. */
 ebg_status ebg_scene_query(const ebg_scene* scene, const float* d_pts, uint64_t n, float* d_value,
                            const ebg_query_opts* opts);
 ebg_status ebg_scene_query_host(const ebg_scene* scene, const float* pts, uint64_t n, float* value,
@@ -548,6 +552,13 @@ ebg_status ebg_mlp_train(ebg_mesh* mesh, const ebg_mlp_opts* opts, float* loss);
 /* Forward pass only: predicted unsigned distance for n device points. */
 ebg_status ebg_mlp_predict(const ebg_mesh* mesh, const float* d_pts, uint64_t n, float* d_pred,
                            void* stream);
+/* The same network for a union scene (C5's per-scene bound): trained on num_samples points, half
+ * uniform in box_scale x the scene box, half uniform in a random primitive's world box grown by
+ * near_sigma x the scene diagonal, labelled |value| by the scene's own walk
+ * (ebg_scene_query); arms ebg_query_opts.learned_bound for ebg_scene_query. */
+ebg_status ebg_scene_mlp_train(ebg_scene* scene, const ebg_mlp_opts* opts, float* loss);
+ebg_status ebg_scene_mlp_predict(const ebg_scene* scene, const float* d_pts, uint64_t n, float* d_pred,
+                                 void* stream);
 /* Parameters as float32 in a fixed order (see ebg_mlp_param_count); host pointers:
  * W1 (51x128, [in][out] row-major), b1 (128), W2, b2, W3, b3, W4, b4 (128x128 / 128 each),
  * w5 (128), b5 (1). set_params also arms the learned bound (it counts as trained). */

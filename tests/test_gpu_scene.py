@@ -197,3 +197,55 @@ def test_mesh_instances_in_separate_leaves(cuda):
         u = BVHUnionIF(prims, leaf_size=leaf)
         _, _, diag = u.bounds()
         check(u.value(q), ref, diag, f"meshes only, leaf {leaf}")
+
+
+def test_learned_scene_bound_is_exact_and_reports_work(cuda):
+    """The per-scene learned bound (C5's): train the union's network on points labelled by the
+    scene's own walk, then seed every query's walk with it; the values must equal the unseeded
+    walk's bit for bit (queries none of whose primitives falls inside the seed are redone), and
+    the work and time of both are reported."""
+    import time
+
+    import torch
+    sc = wl.csg_scene(5, n_spheres=4096, n_boxes=1024, n_meshes=8, mesh_level=4, extent=10.0)
+    u, _ = gpu_scene(sc)
+    t0 = time.perf_counter()
+    loss = u.train_bound(num_samples=400_000, epochs=4, batch=2048, seed=7)
+    torch.cuda.synchronize()
+    train_s = time.perf_counter() - t0
+    assert np.isfinite(loss)
+    q = torch.from_numpy(np.concatenate([wl.uniform_box(150_000, [0.0] * 3, [10.0] * 3, 8),
+                                         near_sphere_points(sc, 50_000, 9)])).to(cuda)
+    plain = u.value(q)
+    learned = u.value(q, learned_bound=True)
+    torch.cuda.synchronize()
+    assert torch.equal(plain, learned)
+    # a radius far below every answer: every query is redone, still exact
+    tiny = torch.full((q.shape[0],), 1e-9, dtype=torch.float32, device=cuda)
+    assert torch.equal(u.value(q, seed_radius=tiny.data_ptr()), plain)
+
+    def work(**kw):
+        w = torch.zeros(8, dtype=torch.int64, device=cuda)
+        u.value(q, work=w.data_ptr(), **kw)
+        torch.cuda.synchronize()
+        return w.cpu().numpy()
+
+    def ms(**kw):
+        for _ in range(2):
+            u.value(q, **kw)
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        for _ in range(5):
+            u.value(q, **kw)
+        torch.cuda.synchronize()
+        return (time.perf_counter() - t) / 5 * 1e3
+
+    wp, wl_ = work(), work(learned_bound=True)
+    n = q.shape[0]
+    pred = u.predict_bound(q)
+    rel = ((pred - plain.abs()).abs() / (plain.abs() + 0.01)).median().item()
+    print(f"scene bound trained on 400000 points in {train_s:.2f}s, loss {loss:.4f}, median relative error {rel:.3f}; "
+          f"per query plain {wp[6] / n:.2f} primitives {wp[0] / n:.2f} nodes, learned {wl_[6] / n:.2f} primitives "
+          f"{wl_[0] / n:.2f} nodes ({int(wl_[3])} redone); {ms():.3f} ms plain, {ms(learned_bound=True):.3f} ms learned "
+          f"for {n} queries", flush=True)
+    assert wl_[3] < n // 10
