@@ -68,8 +68,7 @@ __device__ __forceinline__ void knn_walk(const CloudDev& c, float px, float py, 
   const int rep  = rep_lane(active ? 0.0f : -1.0f);
   int       top  = 0;
   float     bound = active ? INFINITY : -1.0f;
-  st.put(top++, Entry{0u, 0u, c.root_lo[0], c.root_lo[1], c.root_lo[2], c.root_hi[0], c.root_hi[1], c.root_hi[2]},
-         lane);
+  st.put(top++, Entry{0u, 0u, c.root_lo[0], c.root_lo[1], c.root_lo[2], c.root_hi[0], c.root_hi[1], c.root_hi[2]});
   while (top > 0) {
     const Entry e = st.get(--top);
     // a box at the bound can hold no new entry (strict <), but a lane without k points yet has
@@ -77,7 +76,7 @@ __device__ __forceinline__ void knn_walk(const CloudDev& c, float px, float py, 
     const bool need = box_d2<float>(px, py, pz, e.lx, e.ly, e.lz, e.hx, e.hy, e.hz) < bound;
     if (ballot(need) == 0) continue;
     if (e.cnt == 0u) {
-      push_children<K, NB, float, false>(sload_record(nodes + uni(e.ref)), px, py, pz, bound, st, top, lane, rep);
+      push_children<K, NB, float, false>(sload_record(nodes + uni(e.ref)), px, py, pz, bound, st, top, rep);
       if (top > 64 * NB) {
         if (lane == 0 && c.status) atomicOr(c.status, kStatusStackOverflow);
         top = 64 * NB;

@@ -320,14 +320,13 @@ __device__ __forceinline__ void csg_bvh(const CsgDev& g, const CsgParams<R>& q, 
   st.init();
   const int lane = __lane_id();
   int       top  = 0;
-  st.put(top++, Entry{0u, 0u, B.root_lo[0], B.root_lo[1], B.root_lo[2], B.root_hi[0], B.root_hi[1], B.root_hi[2]},
-         lane);
+  st.put(top++, Entry{0u, 0u, B.root_lo[0], B.root_lo[1], B.root_lo[2], B.root_hi[0], B.root_hi[1], B.root_hi[2]});
   while (top > 0) {
     const Entry e    = st.get(--top);
     const bool  need = box_d2<R>(L.x, L.y, L.z, e.lx, e.ly, e.lz, e.hx, e.hy, e.hz) <= bound;
     if (ballot(need) == 0) continue;
     if (e.cnt == 0u) {
-      push_children<4, NB, R, true>(sload_record(nodes + uni(e.ref)), L.x, L.y, L.z, bound, st, top, lane, 0);
+      push_children<4, NB, R, true>(sload_record(nodes + uni(e.ref)), L.x, L.y, L.z, bound, st, top, 0);
       if (top > 64 * NB) {
         if (lane == 0 && g.status) atomicOr(g.status, kStatusStackOverflow);
         top = 64 * NB;

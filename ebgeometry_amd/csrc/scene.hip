@@ -61,7 +61,7 @@ __device__ __forceinline__ void scene_traverse(const SceneDev& sc, R px, R py, R
   const int  rep    = 0; // the scene tree orders children by the wave minimum
   int        top    = 0;
   Entry      root{0u, 0u, sc.root_lo[0], sc.root_lo[1], sc.root_lo[2], sc.root_hi[0], sc.root_hi[1], sc.root_hi[2]};
-  st.put(top++, root, lane);
+  st.put(top++, root);
 
   while (top > 0) {
     const Entry e    = st.get(--top);
@@ -71,7 +71,7 @@ __device__ __forceinline__ void scene_traverse(const SceneDev& sc, R px, R py, R
 
     if (e.cnt == 0u) {
       if (COUNT) { c.n_inner += need ? 1u : 0u; c.w_inner++; }
-      push_children<4, NB, R, true>(sload_record(nodes + uni(e.ref)), px, py, pz, bound, st, top, lane, rep);
+      push_children<4, NB, R, true>(sload_record(nodes + uni(e.ref)), px, py, pz, bound, st, top, rep);
       if (top > 64 * NB) { // beyond the build-time worst case: record it, keep what is held
         if (lane == 0 && sc.status) atomicOr(sc.status, kStatusStackOverflow);
         top = 64 * NB;

@@ -18,9 +18,18 @@ __device__ __forceinline__ float rdlanef(float v, int l)
 {
   return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), l));
 }
-// Lane write: `me` is true on exactly one lane; one v_cndmask per field.
-__device__ __forceinline__ uint32_t wrlane(uint32_t v, bool me, uint32_t old) { return me ? v : old; }
-__device__ __forceinline__ float    wrlanef(float v, bool me, float old) { return me ? v : old; }
+// Lane write of a wave-uniform value into lane l (wave-uniform): one v_writelane per field.
+// The compiler's writelane intrinsic (this clang has no builtin for it), so that hazards and
+// register classes stay the compiler's business.
+__device__ int amdgcn_writelane(int v, int lane, int old) __asm("llvm.amdgcn.writelane.i32");
+__device__ __forceinline__ uint32_t wrlane(uint32_t v, int l, uint32_t old)
+{
+  return (uint32_t)amdgcn_writelane((int)v, l, (int)old);
+}
+__device__ __forceinline__ float wrlanef(float v, int l, float old)
+{
+  return __int_as_float(amdgcn_writelane(__float_as_int(v), l, __float_as_int(old)));
+}
 __device__ __forceinline__ uint32_t uni(uint32_t v) { return (uint32_t)__builtin_amdgcn_readfirstlane((int)v); }
 
 // Lane mask of a predicate, taken straight from the compare (no round trip through an int).
@@ -141,21 +150,21 @@ struct LaneStack
     }
   }
 
-  // Write entry e at stack position pos (wave-uniform); lane = this lane's id.
-  __device__ __forceinline__ void put(int pos, const Entry& e, int lane)
+  // Write entry e (wave-uniform) at stack position pos (wave-uniform).
+  __device__ __forceinline__ void put(int pos, const Entry& e)
   {
-    const bool me = lane == (pos & 63);
+    const int l = pos & 63;
 #pragma unroll
     for (int i = 0; i < NB; i++) {
       if ((pos >> 6) == i) {
-        ref[i] = wrlane(e.ref, me, ref[i]);
-        cnt[i] = wrlane(e.cnt, me, cnt[i]);
-        lx[i]  = wrlanef(e.lx, me, lx[i]);
-        ly[i]  = wrlanef(e.ly, me, ly[i]);
-        lz[i]  = wrlanef(e.lz, me, lz[i]);
-        hx[i]  = wrlanef(e.hx, me, hx[i]);
-        hy[i]  = wrlanef(e.hy, me, hy[i]);
-        hz[i]  = wrlanef(e.hz, me, hz[i]);
+        ref[i] = wrlane(e.ref, l, ref[i]);
+        cnt[i] = wrlane(e.cnt, l, cnt[i]);
+        lx[i]  = wrlanef(e.lx, l, lx[i]);
+        ly[i]  = wrlanef(e.ly, l, ly[i]);
+        lz[i]  = wrlanef(e.lz, l, lz[i]);
+        hx[i]  = wrlanef(e.hx, l, hx[i]);
+        hy[i]  = wrlanef(e.hy, l, hy[i]);
+        hz[i]  = wrlanef(e.hz, l, hz[i]);
       }
     }
   }
@@ -495,7 +504,7 @@ struct FaceCands
 // FP64 passes over sparse lists, CSG scenes with primitives the size of a packet).
 template <int K, int NB, class R, bool WMIN>
 __device__ __forceinline__ void push_children(const Node<K>& nd, R px, R py, R pz, R bound, LaneStack<NB>& st,
-                                              int& top, int lane, int rep)
+                                              int& top, int rep)
 {
   uint32_t key[K];
   uint64_t msk[K];
@@ -524,7 +533,7 @@ __device__ __forceinline__ void push_children(const Node<K>& nd, R px, R py, R p
       rank += (key[j] > key[k] || (key[j] == key[k] && j < k)) ? 1 : 0;
     }
     Entry c{nd.ref[k], nd.cnt[k], nd.lox[k], nd.loy[k], nd.loz[k], nd.hix[k], nd.hiy[k], nd.hiz[k]};
-    st.put(top + rank, c, lane);
+    st.put(top + rank, c);
     pushed++;
   }
   top += pushed;
@@ -557,7 +566,7 @@ __device__ __forceinline__ void traverse_from(const MeshDev& m, const Entry& sta
   const int lane = __lane_id();
   const int rep  = rep_lane(bound);
   int       top  = 0;
-  st.put(top++, start, lane);
+  st.put(top++, start);
 
   while (top > 0) {
     const Entry e = st.get(--top);
@@ -567,7 +576,7 @@ __device__ __forceinline__ void traverse_from(const MeshDev& m, const Entry& sta
 
     if (e.cnt == 0u) {
       if (COUNT) { n_inner += need ? 1u : 0u; w_inner++; }
-      push_children<K, NB, R, WMIN>(sload_record(nodes + uni(e.ref)), px, py, pz, bound, st, top, lane, rep);
+      push_children<K, NB, R, WMIN>(sload_record(nodes + uni(e.ref)), px, py, pz, bound, st, top, rep);
       // The capacity is the tree's exact worst case, computed at build time; should a walk ever
       // exceed it, the pushes beyond it were dropped: record that in the mesh's status word
       // (the host reports it) and go on with the entries that are held.
