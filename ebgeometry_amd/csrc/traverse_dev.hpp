@@ -129,6 +129,12 @@ __device__ __forceinline__ T sload_record(const T* p)
 // VGPR-lane stack
 // ---------------------------------------------------------------------------------------
 
+// A stack entry's count word: the leaf's triangle count (0 for an interior node) in the low bits,
+// the mesh walk's bound stamp above them.
+constexpr uint32_t kStampShift = 8;
+constexpr uint32_t kCountMask  = (1u << kStampShift) - 1u;
+static_assert(kMaxLeaf <= (int)kCountMask, "leaf counts must fit below the stamp");
+
 struct Entry
 {
   uint32_t ref, cnt;
@@ -167,6 +173,24 @@ struct LaneStack
         hz[i]  = wrlanef(e.hz, l, hz[i]);
       }
     }
+  }
+
+  // The reference and count words of entry pos alone (wave-uniform).
+  __device__ __forceinline__ uint32_t ref_at(int pos) const
+  {
+    uint32_t v = 0u;
+#pragma unroll
+    for (int i = 0; i < NB; i++)
+      if ((pos >> 6) == i) v = rdlane(ref[i], pos & 63);
+    return v;
+  }
+  __device__ __forceinline__ uint32_t cnt_at(int pos) const
+  {
+    uint32_t v = 0u;
+#pragma unroll
+    for (int i = 0; i < NB; i++)
+      if ((pos >> 6) == i) v = rdlane(cnt[i], pos & 63);
+    return v;
   }
 
   __device__ __forceinline__ Entry get(int pos) const
@@ -501,10 +525,11 @@ struct FaceCands
 // readlane instead of a wavefront minimum (DPP and 4 readlanes per child), and as good an
 // order there (16M queries, 327,680-face displaced sphere: 1.6% fewer triangle evaluations, 5.7% less time). With WMIN the key is the
 // minimum over the lanes that need the child: better for packets whose lanes are spread (the
-// FP64 passes over sparse lists, CSG scenes with primitives the size of a packet).
+// FP64 passes over sparse lists, CSG scenes with primitives the size of a packet). `tag` is OR-ed
+// into each pushed count word (the mesh walk's stamp, above the leaf count's bits).
 template <int K, int NB, class R, bool WMIN>
 __device__ __forceinline__ void push_children(const Node<K>& nd, R px, R py, R pz, R bound, LaneStack<NB>& st,
-                                              int& top, int rep)
+                                              int& top, int rep, uint32_t tag = 0u)
 {
   uint32_t key[K];
   uint64_t msk[K];
@@ -532,7 +557,7 @@ __device__ __forceinline__ void push_children(const Node<K>& nd, R px, R py, R p
       if (j == k || !msk[j]) continue;
       rank += (key[j] > key[k] || (key[j] == key[k] && j < k)) ? 1 : 0;
     }
-    Entry c{nd.ref[k], nd.cnt[k], nd.lox[k], nd.loy[k], nd.loz[k], nd.hix[k], nd.hiy[k], nd.hiz[k]};
+    Entry c{nd.ref[k], nd.cnt[k] | tag, nd.lox[k], nd.loy[k], nd.loz[k], nd.hix[k], nd.hiy[k], nd.hiz[k]};
     st.put(top + rank, c);
     pushed++;
   }
@@ -567,16 +592,29 @@ __device__ __forceinline__ void traverse_from(const MeshDev& m, const Entry& sta
   const int rep  = rep_lane(bound);
   int       top  = 0;
   st.put(top++, start);
+  // Stamp of the bounds: it advances after every leaf that tightened some lane's bound, and each
+  // pushed entry carries it above the count's bits. An entry whose stamp is current was pushed
+  // because some lane's box distance^2 was within its bound, and no bound has changed since, so
+  // the test at the pop would pass again: it is skipped (the box fields are not even read). A
+  // skipped test can only cost work, never change a result; the counting walk always tests.
+  uint32_t stamp = 1u; // the start entry carries 0: always tested
 
   while (top > 0) {
-    const Entry e = st.get(--top);
-    const R     d2e  = box_d2<R>(px, py, pz, e.lx, e.ly, e.lz, e.hx, e.hy, e.hz);
-    const bool  need = d2e <= bound;
-    if (ballot(need) == 0) continue;
+    --top;
+    const uint32_t ecnt = st.cnt_at(top);
+    bool           need = true;
+    if (COUNT || (ecnt >> kStampShift) != stamp) {
+      const Entry e   = st.get(top);
+      const R     d2e = box_d2<R>(px, py, pz, e.lx, e.ly, e.lz, e.hx, e.hy, e.hz);
+      need            = d2e <= bound;
+      if (ballot(need) == 0) continue;
+    }
+    const uint32_t eref = st.ref_at(top), cnt = ecnt & kCountMask;
 
-    if (e.cnt == 0u) {
+    if (cnt == 0u) {
       if (COUNT) { n_inner += need ? 1u : 0u; w_inner++; }
-      push_children<K, NB, R, WMIN>(sload_record(nodes + uni(e.ref)), px, py, pz, bound, st, top, rep);
+      push_children<K, NB, R, WMIN>(sload_record(nodes + uni(eref)), px, py, pz, bound, st, top, rep,
+                                    stamp << kStampShift);
       // The capacity is the tree's exact worst case, computed at build time; should a walk ever
       // exceed it, the pushes beyond it were dropped: record that in the mesh's status word
       // (the host reports it) and go on with the entries that are held.
@@ -594,7 +632,8 @@ __device__ __forceinline__ void traverse_from(const MeshDev& m, const Entry& sta
       // then each needed triangle record in one more, where a lower bound (the plane and the
       // edge lines, tri_eval32_culled) is tested against the bound as tightened by the leaf's
       // earlier triangles.
-      const uint32_t first = uni(e.ref), cnt = uni(e.cnt);
+      const uint32_t first  = uni(eref);
+      const R        bound0 = bound;
       R              rb;
       [[maybe_unused]] float cull = 0.0f; // FP32 distance-only walks: the lower-bound cull's limit
       if constexpr (sizeof(R) == 4) {
@@ -675,6 +714,7 @@ __device__ __forceinline__ void traverse_from(const MeshDev& m, const Entry& sta
           }
         }
       }
+      if (ballot(bound < bound0)) stamp++;
     }
   }
   // FP32: classify the winning triangle's closest feature (per-lane record load)
