@@ -69,7 +69,7 @@ __global__ __launch_bounds__(256) void batch_keys_kernel(const BatchItemDev* __r
 
 // One packet: the wavefront's 64 queries of one item against its mesh.
 template <int K, int NB, bool COUNT>
-__device__ __forceinline__ void batch_packet(const BatchArgs& a, uint32_t p)
+__device__ __forceinline__ void batch_packet(const BatchArgs& a, uint32_t p, PairLds* pl)
 {
   const SWords<4>    pk = sload<4>(a.packets + 4 * (size_t)p);
   const uint32_t     item = pk.w[0], first = pk.w[1];
@@ -90,7 +90,8 @@ __device__ __forceinline__ void batch_packet(const BatchArgs& a, uint32_t p)
   }
   float    best = INFINITY;
   uint32_t btri = kInvalid, bfeat = 0, n_inner = 0, n_tris = 0, w_inner = 0, w_tris = 0;
-  traverse<K, NB, float, COUNT>(m, px, py, pz, bound, best, btri, bfeat, n_inner, n_tris, w_inner, w_tris);
+  traverse<K, NB, float, COUNT, false, false, true>(m, px, py, pz, bound, best, btri, bfeat, n_inner, n_tris, w_inner,
+                                                    w_tris, nullptr, pl);
   const bool found = active && btri != kInvalid;
   float      d     = 0.0f;
   if (found) {
@@ -120,6 +121,8 @@ __device__ __forceinline__ void batch_packet(const BatchArgs& a, uint32_t p)
 template <int K, int NB, bool COUNT, bool PERSIST>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(K == 4 ? 8 : 1))) void batch32_kernel(const BatchArgs a)
 {
+  extern __shared__ PairLds pair_lds[]; // one per wavefront of the workgroup
+  PairLds* const pl = &pair_lds[uni(threadIdx.x >> 6)];
   if constexpr (PERSIST) {
     const uint32_t ne   = a.early_blocks ? min(*a.early_count, a.early_cap) : 0u;
     const uint32_t skip = a.early_blocks ? a.early_cap : 0u; // queue entries before the table
@@ -137,7 +140,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(K == 4 ? 8 
         p = w - skip;
         if (p >= a.num_packets || (a.early_blocks && a.flag[p])) continue;
       }
-      batch_packet<K, NB, COUNT>(a, p);
+      batch_packet<K, NB, COUNT>(a, p, pl);
     }
   }
   else {
@@ -163,7 +166,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(K == 4 ? 8 
     }
     else p = w;
     if (p >= a.num_packets) return; // whole wavefront (p is wave-uniform)
-    batch_packet<K, NB, COUNT>(a, p);
+    batch_packet<K, NB, COUNT>(a, p, pl);
   }
 }
 
@@ -260,17 +263,18 @@ __global__ __launch_bounds__(256) void batch_mark_kernel(const BatchArgs a)
 template <int K, int NB>
 hipError_t launch32(const BatchArgs& a, int wg, bool count, hipStream_t s)
 {
+  const size_t lds = (size_t)(wg / 64) * sizeof(PairLds);
   if (a.persist_blocks) {
     if (!a.queue) return hipErrorInvalidValue;
-    if (count) hipLaunchKernelGGL((batch32_kernel<K, NB, true, true>), dim3(a.persist_blocks), dim3(wg), 0, s, a);
-    else hipLaunchKernelGGL((batch32_kernel<K, NB, false, true>), dim3(a.persist_blocks), dim3(wg), 0, s, a);
+    if (count) hipLaunchKernelGGL((batch32_kernel<K, NB, true, true>), dim3(a.persist_blocks), dim3(wg), lds, s, a);
+    else hipLaunchKernelGGL((batch32_kernel<K, NB, false, true>), dim3(a.persist_blocks), dim3(wg), lds, s, a);
     return hipGetLastError();
   }
   const uint32_t wpb    = (uint32_t)wg / 64u;
   const uint32_t blocks = (a.num_packets + wpb - 1) / wpb + a.early_blocks;
   if (blocks == 0) return hipSuccess;
-  if (count) hipLaunchKernelGGL((batch32_kernel<K, NB, true, false>), dim3(blocks), dim3(wg), 0, s, a);
-  else hipLaunchKernelGGL((batch32_kernel<K, NB, false, false>), dim3(blocks), dim3(wg), 0, s, a);
+  if (count) hipLaunchKernelGGL((batch32_kernel<K, NB, true, false>), dim3(blocks), dim3(wg), lds, s, a);
+  else hipLaunchKernelGGL((batch32_kernel<K, NB, false, false>), dim3(blocks), dim3(wg), lds, s, a);
   return hipGetLastError();
 }
 
@@ -278,8 +282,9 @@ template <int K, int NB>
 hipError_t persist_blocks(int wg, bool count, uint32_t* blocks)
 {
   int        nb = 0, dev = 0, cus = 0;
-  hipError_t e  = count ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, batch32_kernel<K, NB, true, true>, wg, 0)
-                        : hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, batch32_kernel<K, NB, false, true>, wg, 0);
+  const size_t lds = (size_t)(wg / 64) * sizeof(PairLds);
+  hipError_t   e   = count ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, batch32_kernel<K, NB, true, true>, wg, lds)
+                           : hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, batch32_kernel<K, NB, false, true>, wg, lds);
   if (e != hipSuccess) return e;
   if ((e = hipGetDevice(&dev)) != hipSuccess) return e;
   if ((e = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev)) != hipSuccess) return e;

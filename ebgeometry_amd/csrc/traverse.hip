@@ -78,8 +78,9 @@ __device__ __forceinline__ float decide_face64(const MeshDev& m, const FaceCands
 // One packet: the 64 queries pk*64 .. pk*64+63 of the (presorted) query order, one per lane.
 // FACE (closest faces requested): each lane decides its face exactly (decide_face64); a lane
 // whose band held more triangles than its list goes to the FP64 pass with the near-surface ones.
+// Distance-only variants (!FACE) evaluate leaves pair-compacted in the wavefront's PairLds.
 template <int K, int NB, bool COUNT, int MODE, bool FACE>
-__device__ __forceinline__ void sdf32_packet(const MeshDev& m, const QueryArgs& a, uint64_t n, uint64_t pk)
+__device__ __forceinline__ void sdf32_packet(const MeshDev& m, const QueryArgs& a, uint64_t n, uint64_t pk, PairLds* pl)
 {
   const uint64_t tid    = pk * 64u + (uint64_t)__lane_id();
   const bool     active = tid < n;
@@ -103,8 +104,8 @@ __device__ __forceinline__ void sdf32_packet(const MeshDev& m, const QueryArgs& 
   const float seed2 = bound;
   FaceCands   fc;
   if constexpr (FACE) fc.init(m.hmax);
-  traverse<K, NB, float, COUNT, false, FACE>(m, px, py, pz, bound, best, btri, bfeat, n_inner, n_tris, w_inner, w_tris,
-                                             &fc);
+  traverse<K, NB, float, COUNT, false, FACE, !FACE>(m, px, py, pz, bound, best, btri, bfeat, n_inner, n_tris, w_inner,
+                                                    w_tris, &fc, pl);
 
   // Exact only if some triangle lies within the seeded radius: otherwise nodes between the
   // seeded radius and the true distance were pruned, and the query must be redone unseeded.
@@ -120,8 +121,8 @@ __device__ __forceinline__ void sdf32_packet(const MeshDev& m, const QueryArgs& 
       uint32_t  btri2 = kInvalid, bfeat2 = 0, f_inner = 0, f_tris = 0;
       FaceCands fc2;
       if constexpr (FACE) fc2.init(m.hmax);
-      traverse<K, NB, float, COUNT, false, FACE>(m, px, py, pz, redo ? INFINITY : -1.0f, best2, btri2, bfeat2, f_inner,
-                                                 f_tris, w_inner, w_tris, &fc2);
+      traverse<K, NB, float, COUNT, false, FACE, !FACE>(m, px, py, pz, redo ? INFINITY : -1.0f, best2, btri2, bfeat2,
+                                                        f_inner, f_tris, w_inner, w_tris, &fc2, pl);
       if (redo) {
         best  = best2;
         btri  = btri2;
@@ -178,7 +179,7 @@ __device__ __forceinline__ void sdf32_packet(const MeshDev& m, const QueryArgs& 
 // (d^2, triangle) is folded into the packet's keys. d^2 >= 0, so its bits order like the value;
 // on equal d^2 the lower triangle index wins.
 template <int K, int NB, bool COUNT>
-__device__ __forceinline__ void split_item(const MeshDev& m, const QueryArgs& a, uint32_t j, uint32_t r)
+__device__ __forceinline__ void split_item(const MeshDev& m, const QueryArgs& a, uint32_t j, uint32_t r, PairLds* pl)
 {
   const uint64_t pk     = a.split_list[j];
   const uint32_t lane   = __lane_id();
@@ -195,8 +196,8 @@ __device__ __forceinline__ void split_item(const MeshDev& m, const QueryArgs& a,
   const Entry     start{rt.ref, rt.cnt, rt.lo[0], rt.lo[1], rt.lo[2], rt.hi[0], rt.hi[1], rt.hi[2]};
   float           best = INFINITY;
   uint32_t        btri = kInvalid, bfeat = 0, n_inner = 0, n_tris = 0, w_inner = 0, w_tris = 0;
-  traverse_from<K, NB, float, COUNT>(m, start, px, py, pz, active ? INFINITY : -1.0f, best, btri, bfeat, n_inner,
-                                     n_tris, w_inner, w_tris);
+  traverse_from<K, NB, float, COUNT, false, false, true>(m, start, px, py, pz, active ? INFINITY : -1.0f, best, btri,
+                                                        bfeat, n_inner, n_tris, w_inner, w_tris, nullptr, pl);
   if (active && btri != kInvalid)
     atomicMin(a.split_keys + 64 * (size_t)j + lane, ((unsigned long long)__float_as_uint(best) << 32) | btri);
   if (COUNT && a.work) {
@@ -219,13 +220,15 @@ template <int K, int NB, bool COUNT, int MODE, bool FACE>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(K == 4 && !FACE ? 8 : 1))) void sdf32_kernel(const MeshDev m, const QueryArgs a)
 {
   const uint32_t wave = uni(threadIdx.x >> 6); // known uniform: list entries load as scalars
+  extern __shared__ PairLds pair_lds[];        // one per wavefront (distance-only variants)
+  PairLds* const pl = FACE ? nullptr : &pair_lds[wave];
   uint32_t       b = blockIdx.x, nb = gridDim.x;
   if (a.split_blocks) {
     if (b < a.split_blocks) {
       const uint32_t items = min(*a.split_count, a.split_cap) * a.split_nroots;
       for (uint32_t w = b * 4u + wave; w < items; w += a.split_blocks * 4u) {
         const uint32_t j = w / a.split_nroots;
-        split_item<K, NB, COUNT>(m, a, j, w - j * a.split_nroots);
+        split_item<K, NB, COUNT>(m, a, j, w - j * a.split_nroots, pl);
       }
       return;
     }
@@ -235,7 +238,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(K == 4 && !
   if (a.early_blocks) {
     if (b < a.early_blocks) {
       const uint32_t j = b * 4u + wave;
-      if (j < min(*a.early_count, a.early_cap)) sdf32_packet<K, NB, COUNT, MODE, FACE>(m, a, a.n, a.early_list[j]);
+      if (j < min(*a.early_count, a.early_cap)) sdf32_packet<K, NB, COUNT, MODE, FACE>(m, a, a.n, a.early_list[j], pl);
       return;
     }
     b -= a.early_blocks;
@@ -246,7 +249,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(K == 4 && !
   const uint64_t pk  = (uint64_t)blk * 4u + wave;
   if (pk * 64u >= n) return;                    // whole wavefront past the end
   if (a.heavy_flag && a.heavy_flag[pk]) return; // whole wavefront: a split or early packet
-  sdf32_packet<K, NB, COUNT, MODE, FACE>(m, a, n, pk);
+  sdf32_packet<K, NB, COUNT, MODE, FACE>(m, a, n, pk, pl);
 }
 
 // One wavefront per listed packet: the winning triangle of each query, as the plain pass ends.
@@ -359,8 +362,9 @@ __global__ __launch_bounds__(256) void sdf64_kernel(const MeshDev m, const Query
 template <int K, int NB, int MODE, bool FACE>
 void launch32f(const MeshDev& m, const QueryArgs& a, bool count, uint32_t blocks, hipStream_t s)
 {
-  if (count) hipLaunchKernelGGL((sdf32_kernel<K, NB, true, MODE, FACE>), dim3(blocks), dim3(256), 0, s, m, a);
-  else hipLaunchKernelGGL((sdf32_kernel<K, NB, false, MODE, FACE>), dim3(blocks), dim3(256), 0, s, m, a);
+  const size_t lds = FACE ? 0 : 4 * sizeof(PairLds);
+  if (count) hipLaunchKernelGGL((sdf32_kernel<K, NB, true, MODE, FACE>), dim3(blocks), dim3(256), lds, s, m, a);
+  else hipLaunchKernelGGL((sdf32_kernel<K, NB, false, MODE, FACE>), dim3(blocks), dim3(256), lds, s, m, a);
 }
 
 // closest faces requested: the exact-face variant (its split packets are off: see launch32)

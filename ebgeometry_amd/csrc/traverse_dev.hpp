@@ -35,6 +35,21 @@ __device__ __forceinline__ uint32_t uni(uint32_t v) { return (uint32_t)__builtin
 // Lane mask of a predicate, taken straight from the compare (no round trip through an int).
 __device__ __forceinline__ uint64_t ballot(bool p) { return __builtin_amdgcn_ballot_w64(p); }
 
+// Number of set bits of m below this lane, plus base.
+__device__ __forceinline__ uint32_t mbcnt(uint64_t m, uint32_t base)
+{
+  return __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, base));
+}
+
+// Hand-off of LDS data between the lanes of one wavefront: a wavefront's LDS instructions
+// complete in order, so only the compiler has to be kept from moving accesses across this point.
+__device__ __forceinline__ void wave_lds_handoff()
+{
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
 // Minimum / maximum over the 64 lanes of an unsigned value (all lanes active): DPP within rows
 // of 16, then the four row results combined in scalar registers.
 __device__ __forceinline__ uint32_t wave_min_u32(uint32_t v)
@@ -574,17 +589,113 @@ __device__ __forceinline__ int rep_lane(R bound)
   return ((act >> 32) & 1ull) ? 32 : (act ? __builtin_ctzll(act) : 0);
 }
 
+// Per-wavefront LDS of the pair-compacted leaf evaluation (leaf_pairs): the leaf's triangle
+// records, every (lane, triangle) pair the screen passed, and each lane's best key.
+struct alignas(64) PairLds
+{
+  float4   rec[kMaxLeaf * 8];   // kMaxLeaf TriRecs of 128 B
+  uint64_t best[64];            // (d^2 bits << 32) | triangle, per lane
+  uint16_t pair[kMaxLeaf * 64]; // (lane << 4) | slot in the leaf
+};
+static_assert(kMaxLeaf <= 16, "leaf slots are 4 bits");
+static_assert(sizeof(PairLds) == 4608, "PairLds layout");
+
+// FP32 distance-only leaf, compacted: the screen (bounding sphere and plane against each lane's
+// bound, as in the wavefront-wide leaf) is run for every triangle of the leaf, and each passing
+// (lane, triangle) pair is appended to a list in LDS; then each lane evaluates one pair at a time
+// -- its triangle's record read from LDS, its query point from the pair's lane (ds_bpermute) --
+// and folds (d^2, triangle) into the pair lane's best key with an LDS atomic minimum. A
+// wavefront-wide evaluation costs a full evaluation's instructions whenever one lane needs the
+// triangle; here 64 needed pairs share those instructions, whichever lanes and triangles they
+// come from. The result is the same function's d^2 minimised over the same pairs (the screen is
+// the same conservative test, against bounds tightened once per leaf instead of per triangle),
+// so distances are those of the wavefront-wide leaf; among equal d^2 the lower triangle index
+// wins. The leaf's records are fetched (one 32-byte piece per lane) only when some pair needs
+// them: loading every visited leaf ahead of the screen, to hide the load's latency behind it,
+// fetched three leaves for each one used (1.4x the DRAM traffic) and was 1% slower.
+template <bool COUNT>
+__device__ __forceinline__ void leaf_pairs(const MeshDev& m, uint32_t first, uint32_t cnt, float px, float py,
+                                           float pz, float& bound, float& best, uint32_t& btri, uint32_t& n_tris,
+                                           uint32_t& w_tris, PairLds& L)
+{
+  const uint32_t lane = __lane_id();
+  const uint32_t q0   = 2u * lane; // this lane's 16-byte pieces q0, q0 + 1 of the leaf's records
+  const bool     mine = q0 < 8u * cnt;
+  const float4* __restrict__ src = reinterpret_cast<const float4*>(m.tris + first);
+
+  const float rb    = __builtin_amdgcn_sqrtf(bound) * 1.000001f; // 1-ulp hardware root, rounded up
+  uint32_t    total = 0;                                        // pairs listed (wave-uniform)
+  for (uint32_t c0 = 0; c0 < cnt; c0 += 4) {
+    const SWords<32> sp = sload<32>(m.tsph + 8 * (size_t)(first + c0));
+#pragma unroll
+    for (int i = 0; i < 4; i++) {
+      if (c0 + i < cnt) {
+        const float sx = px - __uint_as_float(sp.w[8 * i]), sy = py - __uint_as_float(sp.w[8 * i + 1]),
+                    sz = pz - __uint_as_float(sp.w[8 * i + 2]);
+        const float lim  = rb + __uint_as_float(sp.w[8 * i + 3]);
+        const bool  in_s = sx * sx + sy * sy + sz * sz <= lim * lim;
+        const float pd   = fmaf(__uint_as_float(sp.w[8 * i + 4]), px,
+                                fmaf(__uint_as_float(sp.w[8 * i + 5]), py,
+                                     fmaf(__uint_as_float(sp.w[8 * i + 6]), pz, __uint_as_float(sp.w[8 * i + 7]))));
+        const bool  in_p = fabsf(pd) <= rb;
+        if (COUNT) n_tris += (in_s && in_p) ? 1u : 0u;
+        const uint64_t pm = ballot(in_s) & ballot(in_p);
+        if (pm) {
+          if ((pm >> lane) & 1ull) L.pair[mbcnt(pm, total)] = (uint16_t)((lane << 4) | (c0 + (uint32_t)i));
+          total += (uint32_t)__popcll(pm);
+        }
+      }
+    }
+  }
+  if (total == 0) return;
+  if (COUNT) w_tris += cnt;
+  if (mine) {
+    L.rec[q0]     = src[q0];
+    L.rec[q0 + 1] = src[q0 + 1];
+  }
+  L.best[lane] = ((uint64_t)__float_as_uint(best) << 32) | btri;
+  wave_lds_handoff();
+  for (uint32_t b0 = 0; b0 < total; b0 += 64) {
+    const uint32_t k  = b0 + lane;
+    const bool     ok = k < total;
+    uint32_t       e  = lane << 4; // idle lanes: their own point
+    if (ok) e = L.pair[k];
+    // all 64 lanes take part in the permutes (an inactive source lane would not be read)
+    const int   sa = (int)((e >> 4) << 2);
+    const float qx = __int_as_float(__builtin_amdgcn_ds_bpermute(sa, __float_as_int(px)));
+    const float qy = __int_as_float(__builtin_amdgcn_ds_bpermute(sa, __float_as_int(py)));
+    const float qz = __int_as_float(__builtin_amdgcn_ds_bpermute(sa, __float_as_int(pz)));
+    if (ok) {
+      const uint32_t slot = e & 15u;
+      const TriRec&  r    = *reinterpret_cast<const TriRec*>(&L.rec[8 * slot]);
+      float          d2;
+      uint32_t       f;
+      tri_eval32<false>(r, qx, qy, qz, d2, f);
+      atomicMin(reinterpret_cast<unsigned long long*>(&L.best[e >> 4]),
+                ((unsigned long long)__float_as_uint(d2) << 32) | (unsigned long long)(first + slot));
+    }
+  }
+  wave_lds_handoff();
+  const uint64_t kb = L.best[lane];
+  best              = __uint_as_float((uint32_t)(kb >> 32));
+  btri              = (uint32_t)kb;
+  bound             = best < bound ? best : bound; // bound stays min(bound, best)
+}
+
 // R = float: FP32 pass on TriRec records. R = double: FP64 recheck on raw positions.
 // The walk starts at `start` (the root, or a subtree root of a split packet).
 // FACE (FP32 only): the cull bound is the band of the minimum (FaceCands::band) rather than the
 // minimum itself, the plane culls are off (their FP32 error is one of the coordinates, not of
 // the distance), and every triangle inside the band is kept in fc.
-template <int K, int NB, class R, bool COUNT, bool WMIN = (sizeof(R) == 8), bool FACE = false>
+// PAIRS (FP32 distance-only walks): leaves are evaluated by leaf_pairs in the wavefront's PairLds.
+template <int K, int NB, class R, bool COUNT, bool WMIN = (sizeof(R) == 8), bool FACE = false, bool PAIRS = false>
 __device__ __forceinline__ void traverse_from(const MeshDev& m, const Entry& start, R px, R py, R pz, R bound, R& best,
                                               uint32_t& btri, uint32_t& bfeat, uint32_t& n_inner, uint32_t& n_tris,
-                                              uint32_t& w_inner, uint32_t& w_tris, FaceCands* fc = nullptr)
+                                              uint32_t& w_inner, uint32_t& w_tris, FaceCands* fc = nullptr,
+                                              PairLds* pl = nullptr)
 {
   static_assert(!FACE || sizeof(R) == 4, "closest-face candidates: FP32 walk only");
+  static_assert(!PAIRS || (sizeof(R) == 4 && !FACE), "pair-compacted leaves: FP32 distance-only walks");
   const Node<K>* __restrict__ nodes = static_cast<const Node<K>*>(m.nodes);
   LaneStack<NB>               st;
   st.init();
@@ -622,6 +733,11 @@ __device__ __forceinline__ void traverse_from(const MeshDev& m, const Entry& sta
         if (lane == 0 && m.status) atomicOr(m.status, kStatusStackOverflow);
         top = 64 * NB;
       }
+    }
+    else if constexpr (PAIRS) {
+      const R bound0 = bound;
+      leaf_pairs<COUNT>(m, uni(eref), cnt, px, py, pz, bound, best, btri, n_tris, w_tris, *pl);
+      if (ballot(bound < bound0)) stamp++;
     }
     else {
       // leaf: a triangle is evaluated by the whole wavefront (keeping the minimum is exact for
@@ -726,14 +842,14 @@ __device__ __forceinline__ void traverse_from(const MeshDev& m, const Entry& sta
   }
 }
 
-template <int K, int NB, class R, bool COUNT, bool WMIN = (sizeof(R) == 8), bool FACE = false>
+template <int K, int NB, class R, bool COUNT, bool WMIN = (sizeof(R) == 8), bool FACE = false, bool PAIRS = false>
 __device__ __forceinline__ void traverse(const MeshDev& m, R px, R py, R pz, R bound, R& best, uint32_t& btri,
                                          uint32_t& bfeat, uint32_t& n_inner, uint32_t& n_tris, uint32_t& w_inner, uint32_t& w_tris,
-                                         FaceCands* fc = nullptr)
+                                         FaceCands* fc = nullptr, PairLds* pl = nullptr)
 {
   const Entry root{0u, 0u, m.root_lo[0], m.root_lo[1], m.root_lo[2], m.root_hi[0], m.root_hi[1], m.root_hi[2]};
-  traverse_from<K, NB, R, COUNT, WMIN, FACE>(m, root, px, py, pz, bound, best, btri, bfeat, n_inner, n_tris, w_inner,
-                                             w_tris, fc);
+  traverse_from<K, NB, R, COUNT, WMIN, FACE, PAIRS>(m, root, px, py, pz, bound, best, btri, bfeat, n_inner, n_tris,
+                                                    w_inner, w_tris, fc, pl);
 }
 
 } // namespace dev
