@@ -203,8 +203,13 @@ static ebg_status finalize_mesh(ebg_mesh* m, const ebgpu::PackedMesh& pm)
   d.hmax          = pm.hmax;
   EBG_HIP(upload(m->status, std::vector<uint32_t>(16, 0u)));
   d.status = m->status.as<uint32_t>();
+  // leaf slabs, from the device arrays (the same for every builder)
+  EBG_HIP(m->lthick.reserve(4 * std::max<uint64_t>(m->nt, 1)));
+  EBG_HIP(ebgpu::launch_leaf_slabs(m->nodes.p, pm.num_nodes, pm.K, d.tsph, d.tripos, m->lthick.as<float>(), nullptr));
+  EBG_HIP(hipStreamSynchronize(nullptr));
+  d.lthick        = m->lthick.as<float>();
   m->device_bytes = m->nodes.bytes + m->tris.bytes + m->tripos.bytes + m->tsph.bytes + m->feat.bytes + m->vn32.bytes +
-                    m->en32.bytes + m->vn64.bytes + m->en64.bytes + m->fn64.bytes;
+                    m->en32.bytes + m->vn64.bytes + m->en64.bytes + m->fn64.bytes + m->lthick.bytes;
   return EBG_OK;
 }
 

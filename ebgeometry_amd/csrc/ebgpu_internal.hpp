@@ -92,7 +92,10 @@ struct MeshDev
   uint32_t        num_tris;
   uint32_t*       status; // device word: bit 0 set when a traversal found its stack too small
   float           hmax;   // longest triangle edge (rounded up): scale of the FP32 distance error
-  uint32_t        pad;
+  uint32_t        pad0;
+  const float*    lthick; // per leaf, at its first triangle: slab half-thickness about that
+                          // triangle's plane holding the whole leaf (rounded up); may be null
+  uint32_t        pad1[2];
 };
 static_assert(sizeof(MeshDev) % 16 == 0, "MeshDev is read with 16-byte scalar loads");
 constexpr uint32_t kStatusStackOverflow = 1u;

@@ -188,6 +188,9 @@ hipError_t launch_knn(const CloudDev& c, int K, int banks, const KnnArgs& a, hip
 hipError_t launch_fill_u32(uint32_t* p, uint64_t n, uint32_t v, hipStream_t s);
 // v[i] <- map[v[i]] where v[i] < nmap (polygon meshes: triangle index -> face index)
 hipError_t launch_map_u32(uint32_t* v, uint64_t n, const uint32_t* map, uint32_t nmap, hipStream_t s);
+// Per leaf, at its first triangle: the slab half-thickness about that triangle's plane
+hipError_t launch_leaf_slabs(const void* nodes, uint32_t num_nodes, int K, const float* tsph, const float* tripos,
+                             float* lthick, hipStream_t s);
 // n doubles -> floats (round to nearest even) and n floats -> doubles
 hipError_t launch_f64_to_f32(const double* in, float* out, uint64_t n, hipStream_t s);
 

@@ -623,10 +623,21 @@ __device__ __forceinline__ void leaf_pairs(const MeshDev& m, uint32_t first, uin
   const bool     mine = q0 < 8u * cnt;
   const float4* __restrict__ src = reinterpret_cast<const float4*>(m.tris + first);
 
-  const float rb    = __builtin_amdgcn_sqrtf(bound) * 1.000001f; // 1-ulp hardware root, rounded up
-  uint32_t    total = 0;                                        // pairs listed (wave-uniform)
+  const float rb = __builtin_amdgcn_sqrtf(bound) * 1.000001f; // 1-ulp hardware root, rounded up
+  SWords<32>  sp = sload<32>(m.tsph + 8 * (size_t)first); // culling records of the first 4 triangles
+  if (m.lthick) {
+    // the leaf's slab (its first triangle's plane, widened to hold the whole leaf) is a lower
+    // bound on every lane's distance to it: the screen is skipped when no lane reaches it. The
+    // plane comes with the first culling records (the same round trip).
+    typedef const __attribute__((address_space(4))) float ConstF;
+    const float t  = *(ConstF*)(m.lthick + first);
+    const float pd = fmaf(__uint_as_float(sp.w[4]), px,
+                          fmaf(__uint_as_float(sp.w[5]), py, fmaf(__uint_as_float(sp.w[6]), pz, __uint_as_float(sp.w[7]))));
+    if (ballot(fabsf(pd) <= rb + t) == 0) return;
+  }
+  uint32_t total = 0; // pairs listed (wave-uniform)
   for (uint32_t c0 = 0; c0 < cnt; c0 += 4) {
-    const SWords<32> sp = sload<32>(m.tsph + 8 * (size_t)(first + c0));
+    if (c0) sp = sload<32>(m.tsph + 8 * (size_t)(first + c0));
 #pragma unroll
     for (int i = 0; i < 4; i++) {
       if (c0 + i < cnt) {
