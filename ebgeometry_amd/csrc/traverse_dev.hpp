@@ -265,7 +265,7 @@ __device__ __forceinline__ void tri_eval32(const TriRec& r, float px, float py, 
   const bool  s0 = (r.c0[0] * p1x + r.c0[1] * p1y + r.c0[2] * p1z) > 0.0f;
   const bool  s1 = (r.c1[0] * p2x + r.c1[1] * p2y + r.c1[2] * p2z) > 0.0f;
   const bool  s2 = (r.c2[0] * p3x + r.c2[1] * p3y + r.c2[2] * p3z) > 0.0f;
-  const bool  inface = (s0 == s1) && (s1 == s2);
+  const bool  inface = (s0 == s1) & (s1 == s2); // no branch: all three compares are needed anyway
   const float fd     = r.n[0] * p1x + r.n[1] * p1y + r.n[2] * p1z;
 
   // Outside the face region: the closest point of each edge at its clamped parameter (t <= 0
@@ -652,7 +652,8 @@ __device__ __forceinline__ void leaf_pairs(const MeshDev& m, uint32_t first, uin
         if (COUNT) n_tris += (in_s && in_p) ? 1u : 0u;
         const uint64_t pm = ballot(in_s) & ballot(in_p);
         if (pm) {
-          if ((pm >> lane) & 1ull) L.pair[mbcnt(pm, total)] = (uint16_t)((lane << 4) | (c0 + (uint32_t)i));
+          // this lane's bit of pm is in_s && in_p: the write is predicated on the compares
+          if (in_s && in_p) L.pair[mbcnt(pm, total)] = (uint16_t)((lane << 4) | (c0 + (uint32_t)i));
           total += (uint32_t)__popcll(pm);
         }
       }
