@@ -232,11 +232,14 @@ def compare(d, d_ref, diag):
 def packet_roofline(n, w, K, kernel_ms, kernel, traffic):
     """Roofline of a packet-traversal launch, priced on the bytes it must fetch: per wavefront,
     each interior node record (32 K bytes: K child boxes, references and counts) and each
-    triangle record it evaluates (128 B), fetched once for the wavefront's 64 queries, plus per
-    query its point (12 B) and its result (4 B). The counts come from the counting variant of
-    the same launch (work[4], work[5]). The per-query figure beside it (every query's own node
-    and triangle reads, SURVEY-style B_q) is what 64 separate traversals would read; a packet
-    shares those reads, so that figure is not an HBM rate and is reported for reference only."""
+    triangle record it copies for evaluation (128 B), fetched once for the wavefront's 64
+    queries, plus per query its point (12 B) and its result (4 B). The counts come from the
+    counting variant of the same launch (work[4], work[5]). The per-query figure beside it
+    (every query's own node and triangle reads, 16 + 28 K per node + 36 per triangle) is what 64
+    separate traversals would read; a packet shares those reads, so that figure is not an HBM
+    rate and is reported for reference only. With w[8], w[9] (the pair counters) the evaluation
+    work is reported too: (query, triangle) pairs per query, 64-lane evaluation batches per
+    wavefront, and the share of batch lanes used."""
     byts = 16 * n + w[4] * 32 * K + w[5] * 128
     gbs = byts / (kernel_ms * 1e-3) / 1e9
     lane = 16 * n + w[0] * 28 * K + w[1] * 36
@@ -247,7 +250,11 @@ def packet_roofline(n, w, K, kernel_ms, kernel, traffic):
             "per_query": {"interior_nodes": round(w[0] / n, 2), "triangles": round(w[1] / n, 2),
                           "fp64_rechecked": w[2], "fallbacks": w[3],
                           "fallback_interior_nodes": round(w[6] / n, 3), "fallback_triangles": round(w[7] / n, 3)},
-            "per_query_bytes_if_unshared_GBs": round(lane / (kernel_ms * 1e-3) / 1e9, 1)}
+            "per_query_bytes_if_unshared_GBs": round(lane / (kernel_ms * 1e-3) / 1e9, 1),
+            "pair_evaluation": (None if len(w) < 10 or not w[9] else
+                                {"pairs_per_query": round(w[8] / n, 2),
+                                 "batches_per_wavefront": round(w[9] / (n / 64), 1),
+                                 "lanes_used_per_batch": round(w[8] / (64 * w[9]), 3)})}
 
 
 def c1_bench(a, dev, cref_cls):
@@ -426,8 +433,8 @@ def batched_bench(a):
     eager = b.plan(items, capture=False, workgroup=plan.workgroup, persistent=plan.persistent)
     _, kms = timed(eager.launch, a.steps, a.warmup, eager.last_traversal_ms)
     del eager
-    wk = torch.zeros(8, dtype=torch.int64, device="cuda:0")
-    b.query(items, work=wk.data_ptr())
+    wk = torch.zeros(10, dtype=torch.int64, device="cuda:0")  # work[0..7], then the pair counters
+    b.query(items, work=wk.data_ptr(), pair_work=wk[8:].data_ptr())
     torch.cuda.synchronize()
     c = [int(x) for x in wk.cpu().numpy()]
     K = info["branching"]
@@ -507,8 +514,8 @@ def main():
     kname = f"sdf32_kernel<{K},{info['stack_capacity'] // 64},false,{{mode}}>"
 
     def work(**kw):
-        w = torch.zeros(8, dtype=torch.int64, device=dev)
-        mesh.query(qt, out=out_l, work=w.data_ptr(), **kw)
+        w = torch.zeros(10, dtype=torch.int64, device=dev)  # work[0..7], then the pair counters
+        mesh.query(qt, out=out_l, work=w.data_ptr(), pair_work=w[8:].data_ptr(), **kw)
         torch.cuda.synchronize()
         return [int(x) for x in w.cpu().numpy()]
 

@@ -30,7 +30,8 @@ class MeshInfo(C.Structure):
 class QueryOpts(C.Structure):
     _fields_ = [("presort", _i32), ("fp64_recheck", _i32), ("recheck_eps", C.c_double),
                 ("learned_bound", _i32), ("fp64_all", _i32), ("bound_scale", C.c_float),
-                ("bound_offset", C.c_float), ("stream", _vp), ("work", _vp), ("seed_radius", _vp)]
+                ("bound_offset", C.c_float), ("stream", _vp), ("work", _vp), ("seed_radius", _vp),
+                ("pair_work", _vp)]
 
 
 class MlpOpts(C.Structure):

@@ -564,6 +564,7 @@ ebg_status query_pipeline(ebg_mesh* m, const float* d_pts, uint64_t n, float* d_
   a.fallback_list  = nullptr;
   a.fallback_count = cnt + 1;
   a.work           = reinterpret_cast<unsigned long long*>(o.work);
+  a.pair_work      = reinterpret_cast<unsigned long long*>(o.pair_work);
   const bool count = o.work != nullptr;
 
   if (heavy_first) {
@@ -605,6 +606,7 @@ ebg_status query_pipeline(ebg_mesh* m, const float* d_pts, uint64_t n, float* d_
     r.recheck_mask     = nullptr;
     r.fallback_list    = nullptr;
     r.work             = nullptr;
+    r.pair_work        = nullptr;
     // bound (|d32| (1 + rel) + abs)^2 always contains the FP64 minimum: FP32 distances are
     // accurate to a few ulp of the coordinates
     r.slack_abs = float(1e-5 * m->diag);

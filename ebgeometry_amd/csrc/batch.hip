@@ -90,8 +90,9 @@ __device__ __forceinline__ void batch_packet(const BatchArgs& a, uint32_t p, Pai
   }
   float    best = INFINITY;
   uint32_t btri = kInvalid, bfeat = 0, n_inner = 0, n_tris = 0, w_inner = 0, w_tris = 0;
+  PairCounts pc{0u, 0u};
   traverse<K, NB, float, COUNT, false, false, true>(m, px, py, pz, bound, best, btri, bfeat, n_inner, n_tris, w_inner,
-                                                    w_tris, nullptr, pl);
+                                                    w_tris, nullptr, pl, &pc);
   const bool found = active && btri != kInvalid;
   float      d     = 0.0f;
   if (found) {
@@ -111,6 +112,10 @@ __device__ __forceinline__ void batch_packet(const BatchArgs& a, uint32_t p, Pai
       atomicAdd(a.work + 1, (unsigned long long)st);
       atomicAdd(a.work + 4, (unsigned long long)w_inner);
       atomicAdd(a.work + 5, (unsigned long long)w_tris);
+      if (a.pair_work) {
+        atomicAdd(a.pair_work, (unsigned long long)pc.pairs);
+        atomicAdd(a.pair_work + 1, (unsigned long long)pc.batches);
+      }
     }
   }
 }

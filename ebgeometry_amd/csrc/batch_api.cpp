@@ -46,6 +46,7 @@ struct ebg_batch_plan
   uint32_t        persist_blocks = 0;
   bool            presort = true, fp64 = true;
   unsigned long long* work = nullptr;
+  unsigned long long* pair_work = nullptr;
   hipGraph_t      graph = nullptr;
   hipGraphExec_t  exec  = nullptr;
   hipEvent_t      ev0 = nullptr, ev1 = nullptr;
@@ -188,6 +189,7 @@ ebg_status enqueue(ebg_batch_plan* p, hipStream_t s, bool events)
   a.recheck_list  = p->fp64 ? p->recheck.as<uint32_t>() : nullptr;
   a.recheck_count = p->counter.as<uint32_t>();
   a.work          = p->work;
+  a.pair_work     = p->work ? p->pair_work : nullptr;
   // counters: [0] recheck list, [1] early list, [2] persistent queue (a fill kernel, so that a
   // captured plan clears them on every replay)
   if (p->fp64 || p->early_cap || p->persistent) EBG_HIP(ebgpu::launch_fill_u32(a.recheck_count, 3, 0u, s));
@@ -450,6 +452,7 @@ ebg_status ebg_batch_plan_create(ebg_batch* b, const ebg_batch_item* items, uint
   p->presort     = o.presort != 0;
   p->fp64        = (o.fp64_recheck || o.fp64_all) && p->num_packets > 0;
   p->work        = reinterpret_cast<unsigned long long*>(o.work);
+  p->pair_work   = reinterpret_cast<unsigned long long*>(o.pair_work);
   p->wg          = workgroup ? workgroup : 256;
   p->persistent  = want_persistent;
   auto bail      = [&](hipError_t e, const char* what) {

@@ -104,8 +104,9 @@ __device__ __forceinline__ void sdf32_packet(const MeshDev& m, const QueryArgs& 
   const float seed2 = bound;
   FaceCands   fc;
   if constexpr (FACE) fc.init(m.hmax);
+  PairCounts pc{0u, 0u};
   traverse<K, NB, float, COUNT, false, FACE, !FACE>(m, px, py, pz, bound, best, btri, bfeat, n_inner, n_tris, w_inner,
-                                                    w_tris, &fc, pl);
+                                                    w_tris, &fc, pl, &pc);
 
   // Exact only if some triangle lies within the seeded radius: otherwise nodes between the
   // seeded radius and the true distance were pruned, and the query must be redone unseeded.
@@ -122,7 +123,7 @@ __device__ __forceinline__ void sdf32_packet(const MeshDev& m, const QueryArgs& 
       FaceCands fc2;
       if constexpr (FACE) fc2.init(m.hmax);
       traverse<K, NB, float, COUNT, false, FACE, !FACE>(m, px, py, pz, redo ? INFINITY : -1.0f, best2, btri2, bfeat2,
-                                                        f_inner, f_tris, w_inner, w_tris, &fc2, pl);
+                                                        f_inner, f_tris, w_inner, w_tris, &fc2, pl, &pc);
       if (redo) {
         best  = best2;
         btri  = btri2;
@@ -171,6 +172,10 @@ __device__ __forceinline__ void sdf32_packet(const MeshDev& m, const QueryArgs& 
       atomicAdd(a.work + 1, (unsigned long long)st);
       atomicAdd(a.work + 4, (unsigned long long)w_inner);
       atomicAdd(a.work + 5, (unsigned long long)w_tris);
+      if (a.pair_work) {
+        atomicAdd(a.pair_work, (unsigned long long)pc.pairs);
+        atomicAdd(a.pair_work + 1, (unsigned long long)pc.batches);
+      }
     }
   }
 }

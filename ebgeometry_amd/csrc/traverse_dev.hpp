@@ -600,6 +600,13 @@ struct alignas(64) PairLds
 static_assert(kMaxLeaf <= 16, "leaf slots are 4 bits");
 static_assert(sizeof(PairLds) == 4608, "PairLds layout");
 
+// Counting walks: the pairs the pair-compacted leaves evaluated and their 64-lane batches
+// (wave-uniform).
+struct PairCounts
+{
+  uint32_t pairs, batches;
+};
+
 // FP32 distance-only leaf, compacted: the screen (bounding sphere and plane against each lane's
 // bound, as in the wavefront-wide leaf) is run for every triangle of the leaf, and each passing
 // (lane, triangle) pair is appended to a list in LDS; then each lane evaluates one pair at a time
@@ -616,7 +623,7 @@ static_assert(sizeof(PairLds) == 4608, "PairLds layout");
 template <bool COUNT>
 __device__ __forceinline__ void leaf_pairs(const MeshDev& m, uint32_t first, uint32_t cnt, float px, float py,
                                            float pz, float& bound, float& best, uint32_t& btri, uint32_t& n_tris,
-                                           uint32_t& w_tris, PairLds& L)
+                                           uint32_t& w_tris, PairLds& L, PairCounts* pc)
 {
   const uint32_t lane = __lane_id();
   const uint32_t q0   = 2u * lane; // this lane's 16-byte pieces q0, q0 + 1 of the leaf's records
@@ -660,7 +667,13 @@ __device__ __forceinline__ void leaf_pairs(const MeshDev& m, uint32_t first, uin
     }
   }
   if (total == 0) return;
-  if (COUNT) w_tris += cnt;
+  if (COUNT) {
+    w_tris += cnt;
+    if (pc) {
+      pc->pairs += total;
+      pc->batches += (total + 63u) / 64u;
+    }
+  }
   if (mine) {
     L.rec[q0]     = src[q0];
     L.rec[q0 + 1] = src[q0 + 1];
@@ -704,7 +717,7 @@ template <int K, int NB, class R, bool COUNT, bool WMIN = (sizeof(R) == 8), bool
 __device__ __forceinline__ void traverse_from(const MeshDev& m, const Entry& start, R px, R py, R pz, R bound, R& best,
                                               uint32_t& btri, uint32_t& bfeat, uint32_t& n_inner, uint32_t& n_tris,
                                               uint32_t& w_inner, uint32_t& w_tris, FaceCands* fc = nullptr,
-                                              PairLds* pl = nullptr)
+                                              PairLds* pl = nullptr, PairCounts* pc = nullptr)
 {
   static_assert(!FACE || sizeof(R) == 4, "closest-face candidates: FP32 walk only");
   static_assert(!PAIRS || (sizeof(R) == 4 && !FACE), "pair-compacted leaves: FP32 distance-only walks");
@@ -748,7 +761,7 @@ __device__ __forceinline__ void traverse_from(const MeshDev& m, const Entry& sta
     }
     else if constexpr (PAIRS) {
       const R bound0 = bound;
-      leaf_pairs<COUNT>(m, uni(eref), cnt, px, py, pz, bound, best, btri, n_tris, w_tris, *pl);
+      leaf_pairs<COUNT>(m, uni(eref), cnt, px, py, pz, bound, best, btri, n_tris, w_tris, *pl, pc);
       if (ballot(bound < bound0)) stamp++;
     }
     else {
@@ -857,11 +870,11 @@ __device__ __forceinline__ void traverse_from(const MeshDev& m, const Entry& sta
 template <int K, int NB, class R, bool COUNT, bool WMIN = (sizeof(R) == 8), bool FACE = false, bool PAIRS = false>
 __device__ __forceinline__ void traverse(const MeshDev& m, R px, R py, R pz, R bound, R& best, uint32_t& btri,
                                          uint32_t& bfeat, uint32_t& n_inner, uint32_t& n_tris, uint32_t& w_inner, uint32_t& w_tris,
-                                         FaceCands* fc = nullptr, PairLds* pl = nullptr)
+                                         FaceCands* fc = nullptr, PairLds* pl = nullptr, PairCounts* pc = nullptr)
 {
   const Entry root{0u, 0u, m.root_lo[0], m.root_lo[1], m.root_lo[2], m.root_hi[0], m.root_hi[1], m.root_hi[2]};
   traverse_from<K, NB, R, COUNT, WMIN, FACE, PAIRS>(m, root, px, py, pz, bound, best, btri, bfeat, n_inner, n_tris,
-                                                    w_inner, w_tris, fc, pl);
+                                                    w_inner, w_tris, fc, pl, pc);
 }
 
 } // namespace dev

@@ -122,7 +122,8 @@ class TriMeshSDF(SignedDistanceFunction):
 
     @staticmethod
     def query_opts(presort=True, fp64_recheck=True, recheck_eps=1e-4, learned_bound=False, fp64_all=False,
-                   bound_scale=1.25, bound_offset=1e-3, stream=None, work=None, seed_radius=None) -> QueryOpts:
+                   bound_scale=1.25, bound_offset=1e-3, stream=None, work=None, seed_radius=None,
+                   pair_work=None) -> QueryOpts:
         o = QueryOpts()
         lib().ebg_query_opts_default(C.byref(o))
         o.presort = int(presort)
@@ -135,6 +136,7 @@ class TriMeshSDF(SignedDistanceFunction):
         o.stream = stream
         o.work = work
         o.seed_radius = seed_radius
+        o.pair_work = pair_work
         return o
 
     # -- queries --------------------------------------------------------------------------
