@@ -616,8 +616,8 @@ struct PairCounts
 // triangle; here 64 needed pairs share those instructions, whichever lanes and triangles they
 // come from. The result is the same function's d^2 minimised over the same pairs (the screen is
 // the same conservative test, against bounds tightened once per leaf instead of per triangle),
-// so distances are those of the wavefront-wide leaf; among equal d^2 the lower triangle index
-// wins. The leaf's records are fetched (one 32-byte piece per lane) only when some pair needs
+// so the minimum is that of the wavefront-wide leaf (up to FMA contraction, which the compiler
+// may choose differently in another kernel); among equal d^2 the lower triangle index wins. The leaf's records are fetched (one 32-byte piece per lane) only when some pair needs
 // them: loading every visited leaf ahead of the screen, to hide the load's latency behind it,
 // fetched three leaves for each one used (1.4x the DRAM traffic) and was 1% slower.
 template <bool COUNT>
