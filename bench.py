@@ -93,11 +93,11 @@ def timed(fn, steps, warmup, kernel_ms=None):
     return dt, (float(np.mean(ks)) if ks else None)
 
 
-def profiled_traffic(name):
-    """HBM bytes per launch of a kernel from the committed rocprofv3 PMC summary (FETCH_SIZE and
+def profiled_traffic(name, summary="sdf32_displaced_sphere_16M_hbm_traffic.txt"):
+    """HBM bytes per launch of a kernel from a committed rocprofv3 PMC summary (FETCH_SIZE and
     WRITE_SIZE per dispatch, each scaled by the same counter's reading on a copy kernel of known
     size). None when the summary is absent."""
-    path = os.path.join(ROOT, "profiles", "sdf32_displaced_sphere_16M_hbm_traffic.txt")
+    path = os.path.join(ROOT, "profiles", summary)
     if not os.path.exists(path):
         return None
     key = f"traffic_bytes_per_launch.{name}:"
@@ -231,16 +231,17 @@ def compare(d, d_ref, diag):
 
 def packet_roofline(n, w, K, kernel_ms, kernel, traffic):
     """Roofline of a packet-traversal launch, priced on the bytes it must fetch: per wavefront,
-    each interior node record (32 K bytes: K child boxes, references and counts) and each
-    triangle record it copies for evaluation (128 B), fetched once for the wavefront's 64
+    each interior node record (32 K bytes: K child boxes, references and counts), each triangle
+    record it copies for evaluation (128 B), each culling-record load of the screen (4
+    triangles, 128 B) and each leaf-slab load (4 B), fetched once for the wavefront's 64
     queries, plus per query its point (12 B) and its result (4 B). The counts come from the
-    counting variant of the same launch (work[4], work[5]). The per-query figure beside it
+    counting variant of the same launch (work[4], work[5], pair counters [2], [3]). The per-query figure beside it
     (every query's own node and triangle reads, 16 + 28 K per node + 36 per triangle) is what 64
     separate traversals would read; a packet shares those reads, so that figure is not an HBM
     rate and is reported for reference only. With w[8], w[9] (the pair counters) the evaluation
     work is reported too: (query, triangle) pairs per query, 64-lane evaluation batches per
     wavefront, and the share of batch lanes used."""
-    byts = 16 * n + w[4] * 32 * K + w[5] * 128
+    byts = 16 * n + w[4] * 32 * K + w[5] * 128 + (w[10] * 128 + w[11] * 4 if len(w) >= 12 else 0)
     gbs = byts / (kernel_ms * 1e-3) / 1e9
     lane = 16 * n + w[0] * 28 * K + w[1] * 36
     return {"bound": "hbm", "achieved": round(gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
@@ -251,6 +252,7 @@ def packet_roofline(n, w, K, kernel_ms, kernel, traffic):
                           "fp64_rechecked": w[2], "fallbacks": w[3],
                           "fallback_interior_nodes": round(w[6] / n, 3), "fallback_triangles": round(w[7] / n, 3)},
             "per_query_bytes_if_unshared_GBs": round(lane / (kernel_ms * 1e-3) / 1e9, 1),
+            "per_wavefront_culling_loads": (None if len(w) < 12 else round(w[10] / (n / 64), 1)),
             "pair_evaluation": (None if len(w) < 10 or not w[9] else
                                 {"pairs_per_query": round(w[8] / n, 2),
                                  "batches_per_wavefront": round(w[9] / (n / 64), 1),
@@ -436,7 +438,7 @@ def batched_bench(a):
     eager = b.plan(items, capture=False, workgroup=plan.workgroup, persistent=plan.persistent)
     _, kms = timed(eager.launch, a.steps, a.warmup, eager.last_traversal_ms)
     del eager
-    wk = torch.zeros(10, dtype=torch.int64, device="cuda:0")  # work[0..7], then the pair counters
+    wk = torch.zeros(12, dtype=torch.int64, device="cuda:0")  # work[0..7], then the pair counters
     b.query(items, work=wk.data_ptr(), pair_work=wk[8:].data_ptr())
     torch.cuda.synchronize()
     c = [int(x) for x in wk.cpu().numpy()]
@@ -466,7 +468,8 @@ def batched_bench(a):
                       "tune_ms": info["tune_ms"]},
            "graph_replay_ms": round(replay_ms, 3),
            "roofline": packet_roofline(nq, c, K, kms, f"batch32_kernel<{K},{info['stack_capacity'] // 64},false,"
-                                       f"{str(plan.persistent).lower()}>", None),
+                                       f"{str(plan.persistent).lower()}>",
+                                       profiled_traffic("batch32", "batch32_many_meshes_256_counters.txt")),
            "correctness": {"sample": f"{checked} queries (200 strided of every 16th mesh) against the oracle",
                            "max_abs_err_over_diag": worst, "sign_mismatches": signs,
                            "within_1e-5_diag": bool(worst <= 1e-5)}}
@@ -517,7 +520,7 @@ def main():
     kname = f"sdf32_kernel<{K},{info['stack_capacity'] // 64},false,{{mode}}>"
 
     def work(**kw):
-        w = torch.zeros(10, dtype=torch.int64, device=dev)  # work[0..7], then the pair counters
+        w = torch.zeros(12, dtype=torch.int64, device=dev)  # work[0..7], then the pair counters
         mesh.query(qt, out=out_l, work=w.data_ptr(), pair_work=w[8:].data_ptr(), **kw)
         torch.cuda.synchronize()
         return [int(x) for x in w.cpu().numpy()]

@@ -90,7 +90,7 @@ __device__ __forceinline__ void batch_packet(const BatchArgs& a, uint32_t p, Pai
   }
   float    best = INFINITY;
   uint32_t btri = kInvalid, bfeat = 0, n_inner = 0, n_tris = 0, w_inner = 0, w_tris = 0;
-  PairCounts pc{0u, 0u};
+  PairCounts pc{0u, 0u, 0u, 0u};
   traverse<K, NB, float, COUNT, false, false, true>(m, px, py, pz, bound, best, btri, bfeat, n_inner, n_tris, w_inner,
                                                     w_tris, nullptr, pl, &pc);
   const bool found = active && btri != kInvalid;
@@ -115,6 +115,8 @@ __device__ __forceinline__ void batch_packet(const BatchArgs& a, uint32_t p, Pai
       if (a.pair_work) {
         atomicAdd(a.pair_work, (unsigned long long)pc.pairs);
         atomicAdd(a.pair_work + 1, (unsigned long long)pc.batches);
+        atomicAdd(a.pair_work + 2, (unsigned long long)pc.groups);
+        atomicAdd(a.pair_work + 3, (unsigned long long)pc.leaves);
       }
     }
   }

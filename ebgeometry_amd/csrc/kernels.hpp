@@ -26,7 +26,7 @@ struct QueryArgs
   uint32_t*           fallback_list;  // queries whose seeded radius held no triangle
   uint32_t*           fallback_count;
   unsigned long long* work;      // [inner nodes, triangles] accumulated (null = not counted)
-  unsigned long long* pair_work; // counting launches: [pairs, pair batches] of the compacted leaves
+  unsigned long long* pair_work; // counting launches: [pairs, pair batches, culling loads, slab loads]
   float               slack_abs; // FP64 recheck: bound = (|d| + slack_abs + slack_rel*|d|)^2
   float               slack_rel;
   uint32_t*           packet_cost; // counting launches: per packet, the nodes + triangles its wavefront fetched
@@ -136,7 +136,7 @@ struct BatchArgs
   uint32_t*           recheck_list; // packet << 6 | lane, in packet order (launch_recheck_compact)
   uint32_t*           recheck_count;
   unsigned long long* work;        // [0] inner, [1] triangles, [4]/[5] per wavefront (null = off)
-  unsigned long long* pair_work;   // counting launches: [pairs, pair batches] of the compacted leaves
+  unsigned long long* pair_work;   // counting launches: [pairs, pair batches, culling loads, slab loads]
   // Early packets (early_blocks = 0: none): batch_mark lists the packets whose most expensive
   // query (the cost-grid value of its cell in its mesh's grid) reaches the mesh's threshold;
   // the first early_blocks workgroups of batch32 take them, the rest skip flagged packets.

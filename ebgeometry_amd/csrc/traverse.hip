@@ -104,7 +104,7 @@ __device__ __forceinline__ void sdf32_packet(const MeshDev& m, const QueryArgs& 
   const float seed2 = bound;
   FaceCands   fc;
   if constexpr (FACE) fc.init(m.hmax);
-  PairCounts pc{0u, 0u};
+  PairCounts pc{0u, 0u, 0u, 0u};
   traverse<K, NB, float, COUNT, false, FACE, !FACE>(m, px, py, pz, bound, best, btri, bfeat, n_inner, n_tris, w_inner,
                                                     w_tris, &fc, pl, &pc);
 
@@ -175,6 +175,8 @@ __device__ __forceinline__ void sdf32_packet(const MeshDev& m, const QueryArgs& 
       if (a.pair_work) {
         atomicAdd(a.pair_work, (unsigned long long)pc.pairs);
         atomicAdd(a.pair_work + 1, (unsigned long long)pc.batches);
+        atomicAdd(a.pair_work + 2, (unsigned long long)pc.groups);
+        atomicAdd(a.pair_work + 3, (unsigned long long)pc.leaves);
       }
     }
   }

@@ -600,11 +600,12 @@ struct alignas(64) PairLds
 static_assert(kMaxLeaf <= 16, "leaf slots are 4 bits");
 static_assert(sizeof(PairLds) == 4608, "PairLds layout");
 
-// Counting walks: the pairs the pair-compacted leaves evaluated and their 64-lane batches
-// (wave-uniform).
+// Counting walks (wave-uniform): the pairs the pair-compacted leaves evaluated, their 64-lane
+// batches, the culling-record loads of the screen (4 triangles, 128 B each) and the leaves whose
+// slab was tested (one 4-B load each).
 struct PairCounts
 {
-  uint32_t pairs, batches;
+  uint32_t pairs, batches, groups, leaves;
 };
 
 // FP32 distance-only leaf, compacted: the screen (bounding sphere and plane against each lane's
@@ -640,8 +641,14 @@ __device__ __forceinline__ void leaf_pairs(const MeshDev& m, uint32_t first, uin
     const float t  = *(ConstF*)(m.lthick + first);
     const float pd = fmaf(__uint_as_float(sp.w[4]), px,
                           fmaf(__uint_as_float(sp.w[5]), py, fmaf(__uint_as_float(sp.w[6]), pz, __uint_as_float(sp.w[7]))));
+    if (COUNT && pc) {
+      pc->leaves++;
+      pc->groups++;
+    }
     if (ballot(fabsf(pd) <= rb + t) == 0) return;
+    if (COUNT && pc) pc->groups += (cnt + 3u) / 4u - 1u;
   }
+  else if (COUNT && pc) pc->groups += (cnt + 3u) / 4u;
   uint32_t total = 0; // pairs listed (wave-uniform)
   for (uint32_t c0 = 0; c0 < cnt; c0 += 4) {
     if (c0) sp = sload<32>(m.tsph + 8 * (size_t)(first + c0));

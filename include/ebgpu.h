@@ -161,10 +161,13 @@ typedef struct ebg_query_opts
   const float* seed_radius; /* optional device array (indexed by query): a caller-supplied
                               cull radius per query, used like the learned bound (queries
                               with no triangle inside it are redone unseeded). NULL = none. */
-  uint64_t* pair_work;     /* optional device array of 2 uint64, accumulated by counting calls
+  uint64_t* pair_work;     /* optional device array of 4 uint64, accumulated by counting calls
                               (work != NULL) of the FP32 distance-only mesh queries: [0] the
                               (query, triangle) pairs evaluated by the pair-compacted leaves,
-                              [1] their 64-lane evaluation batches. NULL = not counted.      */
+                              [1] their 64-lane evaluation batches, [2] the screen's loads of
+                              culling records (4 triangles, 128 B each, per wavefront), [3] the
+                              leaves whose slab was tested (one 4-B load each, per wavefront).
+                              NULL = not counted.                                            */
 } ebg_query_opts;
 
 /* Fill *opts with the defaults above. */
