@@ -62,6 +62,10 @@ def parse():
     ap.add_argument("--no-scene", action="store_true", help="skip the CSG scene measurement")
     ap.add_argument("--no-batched", action="store_true", help="skip the 256-mesh batch measurement")
     ap.add_argument("--no-learned-c1", action="store_true", help="skip the learned bound on the C1 mesh")
+    ap.add_argument("--seeded-only", metavar="BOUND_FILE", default=None,
+                    help="profiling aid: the learned-bound pipeline alone, with the network read from BOUND_FILE "
+                         "(trained and written there first when the file does not exist), so that a profiled "
+                         "run launches no training kernel")
     ap.add_argument("--plain-only", action="store_true",
                     help="profiling aid: time only the plain pipeline (no training, no learned bound)")
     ap.add_argument("--batched", action="store_true",
@@ -530,6 +534,19 @@ def main():
         t_plain, trav_plain = timed(plain, a.steps, a.warmup, mesh.last_traversal_ms)
         rp = packet_roofline(n, work(), K, trav_plain, kname.format(mode=0), None)
         print(json.dumps({"profile_run": "plain-only", "ms_per_step": round(t_plain * 1e3, 3), "roofline": rp}),
+              flush=True)
+        return
+    if a.seeded_only:
+        if os.path.exists(a.seeded_only):
+            mesh.load_bound(a.seeded_only)
+        else:
+            mesh.train_bound(num_samples=a.train_samples, epochs=a.epochs, seed=3)
+            mesh.save_bound(a.seeded_only)
+            print(json.dumps({"profile_run": "seeded-only", "bound_written": a.seeded_only}), flush=True)
+            return
+        t_learn, trav_learn = timed(learned, a.steps, a.warmup, mesh.last_traversal_ms)
+        rl = packet_roofline(n, work(learned_bound=True), K, trav_learn, kname.format(mode=1), None)
+        print(json.dumps({"profile_run": "seeded-only", "ms_per_step": round(t_learn * 1e3, 3), "roofline": rl}),
               flush=True)
         return
 

@@ -15,6 +15,7 @@ point (returns a float) or an (n, 3) array / tensor (returns an array / tensor).
 from __future__ import annotations
 
 import ctypes as C
+import os
 from enum import IntEnum
 
 import numpy as np
@@ -231,6 +232,16 @@ class TriMeshSDF(SignedDistanceFunction):
         check(lib().ebg_mlp_predict(self._h, p.data_ptr(), p.shape[0], out.data_ptr(),
                                     torch.cuda.current_stream(p.device).cuda_stream))
         return out
+
+    def save_bound(self, path) -> None:
+        """Write the trained network (parameters, input box, loss settings) to a file, tagged with
+        a hash of the vertex and triangle arrays the mesh was built from (ebg_mlp_save)."""
+        check(lib().ebg_mlp_save(self._h, os.fsencode(path)))
+
+    def load_bound(self, path) -> None:
+        """Read a network written by save_bound for a mesh of the same arrays and arm the learned
+        bound (ebg_mlp_load; a file of another mesh is refused)."""
+        check(lib().ebg_mlp_load(self._h, os.fsencode(path)))
 
     def get_bound_params(self) -> np.ndarray:
         n = int(lib().ebg_mlp_param_count())

@@ -188,3 +188,29 @@ def test_caller_seeded_radius_is_exact(cuda):
         torch.cuda.synchronize()
         assert torch.equal(torch.sign(d0), torch.sign(d1))
         assert (d0 - d1).abs().max().item() <= 1e-6 * 4
+
+
+def test_bound_save_and_load(cuda, mesh, tmp_path):
+    """A saved network loads into a mesh built from the same arrays (same parameters, same
+    predictions, and its learned-bound queries equal the plain ones); a mesh of other arrays
+    refuses the file, and so does a truncated file."""
+    import torch
+    from ebgeometry_amd._lib import EbgError
+    m, v, f = mesh
+    theta = small_theta(7)
+    m.set_bound_params(theta)
+    path = tmp_path / "bound.bin"
+    m.save_bound(path)
+    m2 = TriMeshSDF(v, f)
+    m2.load_bound(path)
+    assert np.array_equal(m2.get_bound_params(), theta)
+    q = torch.from_numpy(wl.mixed_queries(20_000, v, f, 8)).to(cuda)
+    assert torch.equal(m.predict_bound(q), m2.predict_bound(q))
+    assert torch.equal(m2.query(q, learned_bound=True), m2.query(q))
+    v3, f3 = wl.displaced_icosphere(3, 4)
+    with pytest.raises(EbgError, match="another mesh"):
+        TriMeshSDF(v3, f3).load_bound(path)
+    short = tmp_path / "short.bin"
+    short.write_bytes(path.read_bytes()[:100])
+    with pytest.raises(EbgError):
+        TriMeshSDF(v, f).load_bound(short)
