@@ -618,9 +618,12 @@ struct PairCounts
 // come from. The result is the same function's d^2 minimised over the same pairs (the screen is
 // the same conservative test, against bounds tightened once per leaf instead of per triangle),
 // so the minimum is that of the wavefront-wide leaf (up to FMA contraction, which the compiler
-// may choose differently in another kernel); among equal d^2 the lower triangle index wins. The leaf's records are fetched (one 32-byte piece per lane) only when some pair needs
-// them: loading every visited leaf ahead of the screen, to hide the load's latency behind it,
-// fetched three leaves for each one used (1.4x the DRAM traffic) and was 1% slower.
+// may choose differently in another kernel); among equal d^2 the lower triangle index wins.
+// Before the screen, the leaf's slab (MeshDev::lthick, about its first triangle's plane) skips
+// leaves that no lane comes within. The leaf's records are fetched (one 32-byte piece per lane)
+// only when some pair needs them: loading every visited leaf ahead of the screen, to hide the
+// load's latency behind it, fetched three leaves for each one used (1.4x the DRAM traffic) and
+// was 1% slower.
 template <bool COUNT>
 __device__ __forceinline__ void leaf_pairs(const MeshDev& m, uint32_t first, uint32_t cnt, float px, float py,
                                            float pz, float& bound, float& best, uint32_t& btri, uint32_t& n_tris,
