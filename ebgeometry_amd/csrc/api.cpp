@@ -208,8 +208,12 @@ static ebg_status finalize_mesh(ebg_mesh* m, const ebgpu::PackedMesh& pm)
   EBG_HIP(ebgpu::launch_leaf_slabs(m->nodes.p, pm.num_nodes, pm.K, d.tsph, d.tripos, m->lthick.as<float>(), nullptr));
   EBG_HIP(hipStreamSynchronize(nullptr));
   d.lthick        = m->lthick.as<float>();
+  EBG_HIP(m->frames.reserve(sizeof(ebgpu::FrameRec) * std::max<uint64_t>(m->nt, 1)));
+  EBG_HIP(ebgpu::launch_frames(d.tripos, m->nt, m->frames.as<ebgpu::FrameRec>(), nullptr));
+  EBG_HIP(hipStreamSynchronize(nullptr));
+  d.frames        = m->frames.as<ebgpu::FrameRec>();
   m->device_bytes = m->nodes.bytes + m->tris.bytes + m->tripos.bytes + m->tsph.bytes + m->feat.bytes + m->vn32.bytes +
-                    m->en32.bytes + m->vn64.bytes + m->en64.bytes + m->fn64.bytes + m->lthick.bytes;
+                    m->en32.bytes + m->vn64.bytes + m->en64.bytes + m->fn64.bytes + m->lthick.bytes + m->frames.bytes;
   return EBG_OK;
 }
 

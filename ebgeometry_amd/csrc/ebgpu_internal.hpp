@@ -74,6 +74,25 @@ struct alignas(64) TriRec
 };
 static_assert(sizeof(TriRec) == 128, "TriRec layout");
 
+// The same triangle in a frame of its own plane, for distance-only evaluation: vertices
+// relabelled so that A -> B is the longest edge, ex along it, ey in the plane towards C, ez their
+// cross product; in (ex, ey) A = (0, 0), B = (a, 0), C = (b, c) with c >= 0. Built in FP64.
+struct alignas(64) FrameRec
+{
+  float o[3];      // A
+  float ex[3];     // unit vectors of the frame
+  float ey[3];
+  float ez[3];
+  float a, b, c;   // B = (a, 0), C = (b, c)
+  float bma;       // b - a
+  float ca;        // c * a
+  float k1u, k1v;  // B -> C: t = k1u u + k1v v + k1c, ((b - a), c, -a (b - a)) / |C - B|^2 (0 for a
+  float k1c;       //          zero-length edge)
+  float k2u, k2v;  // A -> C: t = k2u u + k2v v, (b, c) / |C - A|^2 (0 for a zero-length edge)
+  float pad[10];
+};
+static_assert(sizeof(FrameRec) == 128, "FrameRec layout");
+
 // Device view of a mesh, passed by value to kernels.
 struct MeshDev
 {
@@ -95,7 +114,7 @@ struct MeshDev
   uint32_t        pad0;
   const float*    lthick; // per leaf, at its first triangle: slab half-thickness about that
                           // triangle's plane holding the whole leaf (rounded up); may be null
-  uint32_t        pad1[2];
+  const FrameRec*  frames; // per triangle (leaf order); may be null
 };
 static_assert(sizeof(MeshDev) % 16 == 0, "MeshDev is read with 16-byte scalar loads");
 constexpr uint32_t kStatusStackOverflow = 1u;

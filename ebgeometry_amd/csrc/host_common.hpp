@@ -219,6 +219,7 @@ struct ebg_mesh
   ebgpu::DevBuf          nodes, tris, tripos, tsph, feat, vn32, en32, vn64, en64, fn64;
   ebgpu::DevBuf          area_cdf; // normalised cumulative triangle area, leaf order (training samples)
   ebgpu::DevBuf          lthick;   // MeshDev::lthick (one float per triangle, set at leaf starts)
+  ebgpu::DevBuf          frames;   // MeshDev::frames (one FrameRec per triangle)
   // cost grid of the split schedule (built at the first large presorted query): per cell, the
   // work of a packet of 64 queries spread over the cell
   ebgpu::DevBuf          cost_grid;

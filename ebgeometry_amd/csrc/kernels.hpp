@@ -193,6 +193,8 @@ hipError_t launch_map_u32(uint32_t* v, uint64_t n, const uint32_t* map, uint32_t
 // Per leaf, at its first triangle: the slab half-thickness about that triangle's plane
 hipError_t launch_leaf_slabs(const void* nodes, uint32_t num_nodes, int K, const float* tsph, const float* tripos,
                              float* lthick, hipStream_t s);
+// Per triangle (leaf order): its record in a frame of its own plane, from the FP32 vertices
+hipError_t launch_frames(const float* tripos, uint64_t nt, FrameRec* out, hipStream_t s);
 // n doubles -> floats (round to nearest even) and n floats -> doubles
 hipError_t launch_f64_to_f32(const double* in, float* out, uint64_t n, hipStream_t s);
 

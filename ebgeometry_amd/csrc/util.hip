@@ -94,7 +94,92 @@ __global__ __launch_bounds__(256) void leaf_slab_kernel(const Node<K>* __restric
   lthick[first]  = __double2float_ru(w);
 }
 
+// The in-plane frame record of each triangle (FrameRec), in FP64 from the FP32 vertices: the
+// longest edge becomes A -> B (a zero-length longest edge, a point triangle, takes the x axis),
+// ey is the unit part of C - A normal to ex (when C lies on the line AB, any unit vector normal
+// to ex: then c = 0 and no point is inside the face), ez = ex x ey. The edge parameters of a
+// zero-length edge are 0, so that edge's closest point is its start.
+__global__ __launch_bounds__(256) void frame_kernel(const float* __restrict__ tripos, uint64_t nt,
+                                                    FrameRec* __restrict__ out)
+{
+  const uint64_t i = (uint64_t)blockIdx.x * 256 + threadIdx.x;
+  if (i >= nt) return;
+  const float* x = tripos + 12 * i;
+  double       P[3][3];
+  for (int v = 0; v < 3; v++)
+    for (int c = 0; c < 3; c++) P[v][c] = x[3 * v + c];
+  double L[3];
+  for (int j = 0; j < 3; j++) {
+    const int k = j == 2 ? 0 : j + 1;
+    L[j]        = 0.0;
+    for (int c = 0; c < 3; c++) L[j] += (P[k][c] - P[j][c]) * (P[k][c] - P[j][c]);
+  }
+  int j = 0;
+  if (L[1] > L[j]) j = 1;
+  if (L[2] > L[j]) j = 2;
+  const double* A = P[j];
+  const double* B = P[j == 2 ? 0 : j + 1];
+  const double* C = P[j == 0 ? 2 : j - 1];
+  const double  a = sqrt(L[j]);
+  double        ex[3] = {1.0, 0.0, 0.0}, ey[3], ez[3];
+  if (a > 0.0)
+    for (int c = 0; c < 3; c++) ex[c] = (B[c] - A[c]) / a;
+  double b = 0.0;
+  for (int c = 0; c < 3; c++) b += (C[c] - A[c]) * ex[c];
+  double rr[3], cc = 0.0;
+  for (int c = 0; c < 3; c++) {
+    rr[c] = (C[c] - A[c]) - b * ex[c];
+    cc += rr[c] * rr[c];
+  }
+  cc = sqrt(cc);
+  if (!(cc > 0.0)) {
+    // the coordinate axis least aligned with ex, made normal to it
+    int m = 0;
+    if (fabs(ex[1]) < fabs(ex[m])) m = 1;
+    if (fabs(ex[2]) < fabs(ex[m])) m = 2;
+    double nr = 0.0;
+    for (int c = 0; c < 3; c++) {
+      rr[c] = (c == m ? 1.0 : 0.0) - ex[m] * ex[c];
+      nr += rr[c] * rr[c];
+    }
+    nr = sqrt(nr);
+    for (int c = 0; c < 3; c++) ey[c] = rr[c] / nr;
+    cc = 0.0;
+  }
+  else
+    for (int c = 0; c < 3; c++) ey[c] = rr[c] / cc;
+  ez[0] = ex[1] * ey[2] - ex[2] * ey[1];
+  ez[1] = ex[2] * ey[0] - ex[0] * ey[2];
+  ez[2] = ex[0] * ey[1] - ex[1] * ey[0];
+  const double L1 = (b - a) * (b - a) + cc * cc, L2 = b * b + cc * cc;
+  FrameRec     r{};
+  for (int c = 0; c < 3; c++) {
+    r.o[c]  = (float)A[c];
+    r.ex[c] = (float)ex[c];
+    r.ey[c] = (float)ey[c];
+    r.ez[c] = (float)ez[c];
+  }
+  r.a   = (float)a;
+  r.b   = (float)b;
+  r.c   = (float)cc;
+  r.bma = (float)(b - a);
+  r.ca  = (float)(cc * a);
+  r.k1u = L1 > 0.0 ? (float)((b - a) / L1) : 0.0f;
+  r.k1v = L1 > 0.0 ? (float)(cc / L1) : 0.0f;
+  r.k1c = L1 > 0.0 ? (float)(-a * (b - a) / L1) : 0.0f;
+  r.k2u = L2 > 0.0 ? (float)(b / L2) : 0.0f;
+  r.k2v = L2 > 0.0 ? (float)(cc / L2) : 0.0f;
+  out[i] = r;
+}
+
 } // namespace
+
+hipError_t launch_frames(const float* tripos, uint64_t nt, FrameRec* out, hipStream_t s)
+{
+  if (nt == 0) return hipSuccess;
+  hipLaunchKernelGGL(frame_kernel, dim3((uint32_t)((nt + 255) / 256)), dim3(256), 0, s, tripos, nt, out);
+  return hipGetLastError();
+}
 
 hipError_t launch_grid_points(const GridSpec& g, float* out, hipStream_t s)
 {

@@ -196,12 +196,12 @@ def test_bad_mesh_is_rejected(cuda):
 
 def test_pair_leaves_equal_wavefront_leaves(cuda):
     """The mesh queries evaluate leaves pair-compacted (every (query, triangle) pair the screen
-    passes is listed and evaluated one per lane); the union scene walks the same mesh with every
-    triangle evaluated by the whole wavefront. Both minimise the same FP32 distance function over
-    a conservative superset of the triangles that can win, so they find the same minimum; the two
-    kernels compile the same expressions separately (FMA contraction may differ), so a result may
-    differ by the FP32 evaluation error of a distance, 2^-21 (|d| + 2 hmax) (hmax the longest
-    edge; seen: 5 of 50,000 queries, up to 5 ulp near the surface), never by a missed triangle."""
+    passes is listed and evaluated one per lane, on the triangle's in-plane frame); the union scene
+    walks the same mesh with every triangle evaluated by the whole wavefront, on its edge vectors.
+    Both minimise an FP32 distance function over a conservative superset of the triangles that can
+    win; the two functions agree to rounding, so a result may differ by the FP32 evaluation error
+    of a distance, 2^-21 (|d| + 2 hmax) (hmax the longest edge), never by a missed triangle, and
+    the signs are equal."""
     from ebgeometry_amd import AABB, BVHUnionIF
     v, f = wl.displaced_icosphere(5, 21)
     q = wl.mixed_queries(50_000, v, f, 22)
@@ -217,7 +217,6 @@ def test_pair_leaves_equal_wavefront_leaves(cuda):
     bad = np.nonzero(gap > tol)[0]
     assert bad.size == 0, (f"{bad.size} of {q.shape[0]} differ beyond the FP32 distance error, e.g. "
                            f"{d_pairs[bad[0]]!r} vs {d_wave[bad[0]]!r} at {q[bad[0]]}")
-    assert np.count_nonzero(gap) <= q.shape[0] // 1000
     assert np.array_equal(np.sign(d_pairs), np.sign(d_wave))
 
 
