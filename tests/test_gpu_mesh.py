@@ -220,6 +220,38 @@ def test_pair_leaves_equal_wavefront_leaves(cuda):
     assert np.array_equal(np.sign(d_pairs), np.sign(d_wave))
 
 
+def sliver_sphere(level, eps):
+    """Icosphere whose every face (a, b, c) is split at p = m + eps (c - m), m the midpoint of ab,
+    into the slivers (a, b, p) and (c, a, p) (aspect about 1/eps) and the ordinary (b, c, p):
+    closed, consistently wound, no zero-area face."""
+    v, f = wl.icosphere(level)
+    v = np.asarray(v, np.float64)
+    out_v, out_f = list(v), []
+    for a, b, c in np.asarray(f, np.int64):
+        mid = 0.5 * (v[a] + v[b])
+        out_v.append(mid + eps * (v[c] - mid))
+        p = len(out_v) - 1
+        out_f += [(a, b, p), (b, c, p), (c, a, p)]
+    return np.asarray(out_v, np.float32), np.asarray(out_f, np.uint32)
+
+
+def test_sliver_triangles(cuda):
+    """Pairs are evaluated on each triangle's in-plane frame, built in FP64 from its longest edge;
+    on a mesh of slivers (aspect ~1000) beside ordinary triangles the FP32 pass alone keeps the
+    1e-5 * diag bar and the signs outside the FP64 recheck band, and with the recheck every sign."""
+    v, f = sliver_sphere(2, 1e-3)
+    q = wl.mixed_queries(10_000, v, f, 31)
+    d_ref, _ = port.Mesh(v.astype(np.float64), f).sdf(q.astype(np.float64))
+    m = TriMeshSDF(v, f)
+    diag = diag_of(v)
+    for kw in ({}, {"fp64_recheck": False}):
+        d = np.asarray(m.query(q, **kw))
+        err = np.abs(d.astype(np.float64) - d_ref)
+        assert err.max() <= D_TOL * diag, (kw, err.max() / diag)
+        checked = (np.abs(d_ref) > 1e-4 * diag) if kw else (d_ref != 0)
+        assert np.array_equal(np.sign(d[checked]), np.sign(d_ref[checked])), kw
+
+
 def test_packet_composition_does_not_change_results(cuda):
     """Presorted and unsorted queries form different packets (different lanes share pairs and
     bounds, and the presorted call splits its costliest packets into subtree walks whose winners
